@@ -1,0 +1,140 @@
+/*
+ * pylith_amd.h -- C ABI of the MI355X (gfx950) finite-element residual / Jacobian-action path.
+ *
+ * The entry points replace, one for one, the PETSc calls PyLith's assembly layer makes
+ * (libsrc/pylith/feassemble/IntegratorDomain.cc, IntegratorInterface.cc) and the field
+ * scatters of libsrc/pylith/topology/Field.cc.  Every function returns an int error code
+ * (0 = success, as PetscErrorCode); plx_dm_last_error() gives the message of the last
+ * failure on a handle.  No C++ exception crosses this boundary.
+ *
+ * Vectors (locX, locF, ...) are DEVICE pointers owned by the caller, laid out like a PETSc
+ * local vector of PyLith's solution section: the displacement field interleaved per mesh node
+ * (dof = 3*node + component), followed by the Lagrange-multiplier field when a fault is present
+ * (dof = plx_dm lagrange offset + 3*fault_node + component).  Global vectors drop constrained
+ * dofs in ascending local order (PETSc's global section of a single process).
+ *
+ * All work is enqueued on the HIP stream bound to the handle (plx_dm_set_stream); calls return
+ * without waiting for the device unless documented otherwise.  One handle per host thread.
+ */
+#ifndef PYLITH_AMD_H
+#define PYLITH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct plx_dm_s *plx_dm;
+
+/* Cell types (PyLith mesh files: hex vertex order 0-3 one face cyclic, 4-7 opposite face). */
+enum plx_cell_type { PLX_HEX_Q1 = 1, PLX_TET_P2 = 2 };
+
+/* Scalar type of the vectors and of the arithmetic of one call. */
+enum plx_scalar_type { PLX_FP64 = 0, PLX_FP32 = 1 };
+
+/* ADD_VALUES accumulates (PETSc semantics); INSERT_VALUES overwrites the dofs in the closure
+ * of the key's cells, i.e. fuses the caller's zeroing of the local residual
+ * (problems/TimeDependent.cc:657) with the assembly. */
+enum plx_insert_mode { PLX_ADD_VALUES = 0, PLX_INSERT_VALUES = 1 };
+
+/* pylith::feassemble::Integrator::EquationPart (feassemble/Integrator.hh:26-31). */
+enum plx_equation_part { PLX_LHS = 0, PLX_RHS = 1, PLX_LHS_LUMPED_INV = 2, PLX_LHS_WEIGHTED = 3 };
+
+/* Device twins of PyLith pointwise kernels.  A host integration maps the PetscPointFn /
+ * PetscPointJacFn address it registers to one of these ids. */
+enum plx_pointfn {
+    PLX_FN_NONE = 0,
+    PLX_F1V_ISOLINEAR_ELASTICITY_3D = 1,      /* IsotropicLinearElasticity3D::f1v_infinitesimalStrain  (IsotropicLinearElasticity.hh:803) */
+    PLX_JF3VU_ISOLINEAR_ELASTICITY_3D = 2,    /* IsotropicLinearElasticity3D::Jf3vu_infinitesimalStrain (IsotropicLinearElasticity.hh:890) */
+    PLX_F1V_GENMAXWELL_3D = 3,                /* IsotropicLinearGenMaxwell3D::f1v_infinitesimalStrain  (IsotropicLinearGenMaxwell.hh:1054) */
+    PLX_JF3VU_GENMAXWELL_3D = 4,              /* IsotropicLinearGenMaxwell3D::Jf3vu_infinitesimalStrain (IsotropicLinearGenMaxwell.hh:1143) */
+    PLX_VISCOUS_STRAIN_GENMAXWELL_3D = 5,     /* IsotropicLinearGenMaxwell3D::viscousStrain_infinitesimalStrain_asVector (:1507) */
+    PLX_TOTAL_STRAIN_3D = 6                   /* Elasticity3D::infinitesimalStrain_asVector (Elasticity.hh:586) */
+};
+
+/* ---------------------------------------------------------------------------------------------
+ * Mesh, section and closure index.
+ * Replaces DMPlexCreateFromCellListPetsc (meshio/MeshBuilder.cc:155) + the solution section
+ * (topology/Field.cc:277-283) + DMPlexCreateClosureIndex (topology/CoordsVisitor.icc:145-157).
+ *   closure [num_cells][nodes_per_cell] node ids (hex: 8 vertices; tet P2: 4 vertices then the
+ *           edge nodes of edges (0,1),(0,2),(0,3),(1,2),(1,3),(2,3));
+ *   cone    [num_cells][verts_per_cell] vertex ids into coords (NULL: the first verts_per_cell
+ *           closure entries);
+ *   coords  [num_vertices][3];
+ *   labels  [num_cells] material label values (NULL: all 0).
+ * All of these are HOST arrays; they are copied.  Fails with an error for a cell whose
+ * Jacobian determinant is not positive at some quadrature point. */
+int plx_dm_create(plx_dm *dm, int cell_type, int num_cells, int num_vertices, int num_nodes,
+                  const int32_t *closure, const int32_t *cone, const double *coords,
+                  const int32_t *labels);
+int plx_dm_destroy(plx_dm dm);
+int plx_dm_set_stream(plx_dm dm, void *hip_stream);
+const char *plx_dm_last_error(plx_dm dm);
+/* Local and global vector lengths (dofs). */
+int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size);
+
+/* Constrained dofs (Dirichlet, ConstraintSimple/ConstraintSpatialDB; PetscDSAddBoundary at
+ * feassemble/ConstraintSimple.cc:122): `dofs` are local dof indices (HOST array). */
+int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs);
+
+/* ---------------------------------------------------------------------------------------------
+ * Weak form: PetscWeakFormAddResidual / PetscWeakFormAddJacobian as called by
+ * IntegratorDomain::setKernelsResidual / setKernelsJacobian (IntegratorDomain.cc:145-146,183-184).
+ * The key is (label_value, part) of the cells' material label. */
+int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1);
+int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, int jf2, int jf3);
+
+/* DMSetAuxiliaryVec (IntegratorDomain.cc:447).  aux is a HOST array laid out like PyLith's
+ * auxiliary local vector: per cell (aux_per_qp = 0) or per cell and quadrature point
+ * (aux_per_qp = 1), subfields concatenated in order with the given component counts.
+ * The values are captured (copied to the device) at this call. */
+int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux,
+                         const int32_t *aux_sizes, int aux_per_qp, const double *aux);
+
+/* PetscDSSetConstants via Integrator::_setKernelConstants (Integrator.cc:264-288). */
+int plx_dm_set_constants(plx_dm dm, int label_value, int part, int num_constants,
+                         const double *constants);
+
+/* ---------------------------------------------------------------------------------------------
+ * DMPlexComputeResidualByKey(dm, key, cellsIS, t, locX, locX_t, t, locF)
+ * (IntegratorDomain.cc:448-449): locF (+)= sum over the key's cells of
+ * w|J| (f0 . phi + f1 : grad phi).  locX_t may be NULL (quasi-static).  scalar_type selects
+ * FP64 or FP32 vectors and arithmetic. */
+int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, double t,
+                         const void *locX, const void *locX_t, void *locF, int mode);
+
+/* DMPlexComputeJacobianActionByKey(dm, key, cellsIS, t, s_tshift, locX, locX_t, locY, locF)
+ * (IntegratorDomain.cc:540-541): locF (+)= J(locX) locY without assembling J. */
+int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar_type,
+                                double t, double s_tshift, const void *locX, const void *locX_t,
+                                const void *locY, void *locF, int mode);
+
+/* Point-block (3x3 per node) diagonal of J, FP64, blocks[node][3][3] (ADD).  The blocks of
+ * PETSc's vpbjacobi preconditioner that PyLith selects (materials/Elasticity.cc:307-318). */
+int plx_compute_jacobian_pbdiag(plx_dm dm, int label_value, int part, double t,
+                                double s_tshift, const double *locX, double *blocks);
+
+/* ---------------------------------------------------------------------------------------------
+ * Field scatters (topology/Field.cc:394-422) and Dirichlet insertion
+ * (DMPlexInsertBoundaryValuesEssential, feassemble/ConstraintSimple.cc:161-162). */
+/* DMLocalToGlobal: glob (+)= loc over unconstrained dofs. */
+int plx_local_to_global(plx_dm dm, int scalar_type, const void *loc, void *glob, int mode);
+/* DMGlobalToLocal(INSERT): loc[unconstrained] = glob; constrained entries untouched. */
+int plx_global_to_local(plx_dm dm, int scalar_type, const void *glob, void *loc);
+/* loc[constrained dof i] = values[i] (device array, order of plx_dm_set_constraints). */
+int plx_insert_boundary_values(plx_dm dm, int scalar_type, const void *values, void *loc);
+
+/* ---------------------------------------------------------------------------------------------
+ * Reductions and BLAS-1 on device vectors, FP64 accumulation (deterministic two-pass tree).
+ * Results of dot/norm are written to a device scalar and copied to *result (synchronises the
+ * handle's stream). */
+int plx_vec_dot(plx_dm dm, int scalar_type, int64_t n, const void *x, const void *y,
+                double *result);
+int plx_vec_norm2(plx_dm dm, int scalar_type, int64_t n, const void *x, double *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYLITH_AMD_H */

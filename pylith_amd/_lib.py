@@ -1,0 +1,74 @@
+"""ctypes binding of ``libpylith_amd.so`` (the C ABI declared in include/pylith_amd.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpylith_amd.so")
+
+HEX_Q1, TET_P2 = 1, 2
+FP64, FP32 = 0, 1
+ADD_VALUES, INSERT_VALUES = 0, 1
+LHS, RHS, LHS_LUMPED_INV, LHS_WEIGHTED = 0, 1, 2, 3
+FN_NONE = 0
+F1V_ISOLINEAR_ELASTICITY_3D = 1
+JF3VU_ISOLINEAR_ELASTICITY_3D = 2
+F1V_GENMAXWELL_3D = 3
+JF3VU_GENMAXWELL_3D = 4
+VISCOUS_STRAIN_GENMAXWELL_3D = 5
+TOTAL_STRAIN_3D = 6
+
+_c_int, _c_dbl, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dblp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "plx_dm_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _i32p, _i32p,
+                               _dblp, _i32p]),
+    "plx_dm_destroy": (_c_int, [_vp]),
+    "plx_dm_set_stream": (_c_int, [_vp, _vp]),
+    "plx_dm_last_error": (ctypes.c_char_p, [_vp]),
+    "plx_dm_get_sizes": (_c_int, [_vp, _i64p, _i64p]),
+    "plx_dm_set_constraints": (_c_int, [_vp, _c_int, _i32p]),
+    "plx_dm_add_residual": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int]),
+    "plx_dm_add_jacobian": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    "plx_dm_set_auxiliary": (_c_int, [_vp, _c_int, _c_int, _c_int, _i32p, _c_int, _dblp]),
+    "plx_dm_set_constants": (_c_int, [_vp, _c_int, _c_int, _c_int, _dblp]),
+    "plx_compute_residual": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _c_int]),
+    "plx_compute_jacobian_action": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp,
+                                             _vp, _vp, _c_int]),
+    "plx_compute_jacobian_pbdiag": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp]),
+    "plx_local_to_global": (_c_int, [_vp, _c_int, _vp, _vp, _c_int]),
+    "plx_global_to_local": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "plx_insert_boundary_values": (_c_int, [_vp, _c_int, _vp, _vp]),
+    "plx_vec_dot": (_c_int, [_vp, _c_int, ctypes.c_int64, _vp, _vp, _dblp]),
+    "plx_vec_norm2": (_c_int, [_vp, _c_int, ctypes.c_int64, _vp, _dblp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(the MI355X path has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+# Test hook (include/pylith_amd_debug.h): host emulation of the tile kernels.
+SIGNATURES["plx_debug_emulate_apply"] = (_c_int, [_c_int, _c_int, _c_int, _c_int, _i32p, _i32p,
+                                                  _dblp, _i32p, _c_int, _dblp, _c_int, _vp, _vp,
+                                                  _c_int])

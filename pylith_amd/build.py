@@ -1,0 +1,38 @@
+"""Build the in-tree gfx950 shared library ``pylith_amd/libpylith_amd.so`` with hipcc.
+
+Every HIP / C++ source under ``pylith_amd/csrc`` is compiled for gfx950 in one hipcc
+invocation.  Packed-FP32 instruction selection is disabled for all device code.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpylith_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+DEVICE_FLAGS = [
+    "--offload-arch=gfx950",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+COMMON_FLAGS = ["-O3", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Wno-unused-function"]
+
+
+def sources():
+    names = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    return [os.path.join(CSRC, f) for f in names]
+
+
+def build(verbose=False):
+    cmd = [HIPCC, *COMMON_FLAGS, *DEVICE_FLAGS, "-x", "hip", *sources(), "-o", LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

@@ -1,0 +1,378 @@
+// plx_dm.cpp -- host side of the C ABI: mesh upload, tile plans, weak-form keys, aux capture.
+//
+// Mirrors the set-up half of PyLith's assembly path: DMPlex + section + closure index
+// (topology/Field.cc:277-283, topology/CoordsVisitor.icc:145-157), kernel registration
+// (feassemble/IntegratorDomain.cc:134-190) and DMSetAuxiliaryVec (IntegratorDomain.cc:447).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "plx_internal.hpp"
+
+namespace plx {
+
+int set_error(plx_dm dm, int code, const std::string &msg) {
+    if (dm) dm->last_error = msg;
+    return code;
+}
+
+KeyPlan *find_key(plx_dm dm, int label_value, int part) {
+    for (KeyPlan *k : dm->keys)
+        if (k->label_value == label_value && k->part == part) return k;
+    return nullptr;
+}
+
+template <typename T>
+static int upload(plx_dm dm, T **dst, const T *src, size_t n) {
+    *dst = nullptr;
+    if (n == 0) return 0;
+    PLX_HIP(dm, hipMalloc((void **)dst, n * sizeof(T)));
+    PLX_HIP(dm, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void free_key(KeyPlan *k) {
+    void *ptrs[] = {k->d_cells,       k->d_tile_cell_ptr, k->d_tile_node_ptr, k->d_tile_nown,
+                    k->d_tile_nodes,  k->d_tile_vert_ptr, k->d_tile_verts,    k->d_tile_halo_off,
+                    k->d_lclosure,    k->d_lcone,         k->d_shared_node,   k->d_color,
+                    k->d_tile_ncolor,   k->d_shared_ptr,
+                    k->d_shared_slot, k->d_halo,          k->d_mat64,         k->d_mat32,
+                    k->d_state};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete k;
+}
+
+// Build the key's tile plan on the host (plx_plan.cpp) and upload it.
+static int build_plan(plx_dm dm, KeyPlan *kp) {
+    MeshView mv{dm->nb, dm->nv, dm->num_cells, dm->num_nodes, dm->num_vertices,
+                dm->closure.data(), dm->cone.data(), dm->labels.data(),
+                dm->cell_type == PLX_HEX_Q1};
+    HostPlan hp;
+    std::string msg;
+    if (build_host_plan(mv, kp->label_value, tile_limits(dm->cell_type), &hp, &msg))
+        return set_error(dm, 1, msg);
+    kp->cells = hp.cells;
+    kp->num_tiles = hp.num_tiles;
+    kp->max_nodes = hp.max_nodes;
+    kp->max_verts = hp.max_verts;
+    kp->max_colors = hp.max_colors;
+    kp->halo_slots = hp.halo_slots;
+    kp->num_shared = (int)hp.shared_node.size();
+    int err = 0;
+    if ((err = upload(dm, &kp->d_cells, hp.cells.data(), hp.cells.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_cell_ptr, hp.tile_cell_ptr.data(), hp.tile_cell_ptr.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_node_ptr, hp.tile_node_ptr.data(), hp.tile_node_ptr.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_nown, hp.tile_nown.data(), hp.tile_nown.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_nodes, hp.tile_nodes.data(), hp.tile_nodes.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_vert_ptr, hp.tile_vert_ptr.data(), hp.tile_vert_ptr.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_verts, hp.tile_verts.data(), hp.tile_verts.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_halo_off, hp.tile_halo_off.data(), hp.tile_halo_off.size()))) return err;
+    if ((err = upload(dm, &kp->d_lclosure, hp.lclosure.data(), hp.lclosure.size()))) return err;
+    if ((err = upload(dm, &kp->d_lcone, hp.lcone.data(), hp.lcone.size()))) return err;
+    if ((err = upload(dm, &kp->d_color, hp.color.data(), hp.color.size()))) return err;
+    if ((err = upload(dm, &kp->d_tile_ncolor, hp.tile_ncolor.data(), hp.tile_ncolor.size()))) return err;
+    if ((err = upload(dm, &kp->d_shared_node, hp.shared_node.data(), hp.shared_node.size()))) return err;
+    if ((err = upload(dm, &kp->d_shared_ptr, hp.shared_ptr.data(), hp.shared_ptr.size()))) return err;
+    if ((err = upload(dm, &kp->d_shared_slot, hp.shared_slot.data(), hp.shared_slot.size()))) return err;
+    if (hp.halo_slots > 0) PLX_HIP(dm, hipMalloc(&kp->d_halo, (size_t)hp.halo_slots * 3 * sizeof(double)));
+    return 0;
+}
+
+static int get_or_create_key(plx_dm dm, int label_value, int part, KeyPlan **out) {
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp) {
+        kp = new KeyPlan();
+        kp->label_value = label_value;
+        kp->part = part;
+        const int err = build_plan(dm, kp);
+        if (err) {
+            free_key(kp);
+            return err;
+        }
+        dm->keys.push_back(kp);
+    }
+    *out = kp;
+    return 0;
+}
+
+static bool known_residual_pair(int f0, int f1) {
+    return f0 == PLX_FN_NONE && (f1 == PLX_F1V_ISOLINEAR_ELASTICITY_3D || f1 == PLX_F1V_GENMAXWELL_3D);
+}
+
+}  // namespace plx
+
+using namespace plx;
+
+extern "C" {
+
+int plx_dm_create(plx_dm *out, int cell_type, int num_cells, int num_vertices, int num_nodes,
+                  const int32_t *closure, const int32_t *cone, const double *coords,
+                  const int32_t *labels) {
+    if (!out) return 1;
+    *out = nullptr;
+    if ((cell_type != PLX_HEX_Q1 && cell_type != PLX_TET_P2) || num_cells <= 0 ||
+        num_vertices <= 0 || num_nodes <= 0 || !closure || !coords)
+        return 1;
+    plx_dm dm = new plx_dm_s();
+    dm->cell_type = cell_type;
+    dm->nb = cell_type == PLX_HEX_Q1 ? 8 : 10;
+    dm->nv = cell_type == PLX_HEX_Q1 ? 8 : 4;
+    dm->nq = cell_type == PLX_HEX_Q1 ? 8 : 14;
+    dm->num_cells = num_cells;
+    dm->num_vertices = num_vertices;
+    dm->num_nodes = num_nodes;
+    dm->local_size = 3 * (int64_t)num_nodes;
+    dm->closure.assign(closure, closure + (size_t)num_cells * dm->nb);
+    if (cone) {
+        dm->cone.assign(cone, cone + (size_t)num_cells * dm->nv);
+    } else {
+        dm->cone.resize((size_t)num_cells * dm->nv);
+        for (int c = 0; c < num_cells; ++c)
+            for (int b = 0; b < dm->nv; ++b) dm->cone[(size_t)c * dm->nv + b] = closure[(size_t)c * dm->nb + b];
+    }
+    if (cell_type == PLX_HEX_Q1 && dm->cone != dm->closure) {
+        delete dm;  // Q1 is isoparametric: the vertices are the nodes
+        return 1;
+    }
+    for (int32_t n : dm->closure)
+        if (n < 0 || n >= num_nodes) {
+            delete dm;
+            return 1;
+        }
+    for (int32_t v : dm->cone)
+        if (v < 0 || v >= num_vertices) {
+            delete dm;
+            return 1;
+        }
+    dm->coords.assign(coords, coords + (size_t)num_vertices * 3);
+    if (labels)
+        dm->labels.assign(labels, labels + num_cells);
+    else
+        dm->labels.assign(num_cells, 0);
+    if (hipMalloc(&dm->d_coords, dm->coords.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(dm->d_coords, dm->coords.data(), dm->coords.size() * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&dm->d_scalar, 8 * sizeof(double)) != hipSuccess) {
+        plx_dm_destroy(dm);
+        return 76;
+    }
+    dm->con.global_size = dm->local_size;
+    int bad = 0;
+    const int verr = validate_geometry(dm, &bad);
+    if (verr || bad) {
+        plx_dm_destroy(dm);
+        return verr ? verr : 2;
+    }
+    *out = dm;
+    return 0;
+}
+
+int plx_dm_destroy(plx_dm dm) {
+    if (!dm) return 0;
+    for (KeyPlan *k : dm->keys) free_key(k);
+    if (dm->d_coords) (void)hipFree(dm->d_coords);
+    if (dm->d_closure) (void)hipFree(dm->d_closure);
+    if (dm->d_cone) (void)hipFree(dm->d_cone);
+    if (dm->d_scalar) (void)hipFree(dm->d_scalar);
+    if (dm->d_partials) (void)hipFree(dm->d_partials);
+    if (dm->con.d_dofs) (void)hipFree(dm->con.d_dofs);
+    if (dm->con.d_g2l) (void)hipFree(dm->con.d_g2l);
+    delete dm;
+    return 0;
+}
+
+int plx_dm_set_stream(plx_dm dm, void *stream) {
+    if (!dm) return 1;
+    dm->stream = (hipStream_t)stream;
+    return 0;
+}
+
+const char *plx_dm_last_error(plx_dm dm) { return dm ? dm->last_error.c_str() : "null handle"; }
+
+int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size) {
+    if (!dm) return 1;
+    if (local_size) *local_size = dm->local_size;
+    if (global_size) *global_size = dm->con.global_size;
+    return 0;
+}
+
+int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs) {
+    if (!dm || num_constrained < 0 || (num_constrained > 0 && !dofs)) return set_error(dm, 1, "bad constraint arguments");
+    std::vector<char> is_con(dm->local_size, 0);
+    for (int i = 0; i < num_constrained; ++i) {
+        if (dofs[i] < 0 || dofs[i] >= dm->local_size) return set_error(dm, 1, "constrained dof out of range");
+        if (is_con[dofs[i]]) return set_error(dm, 1, "constrained dof listed twice");
+        is_con[dofs[i]] = 1;
+    }
+    std::vector<int32_t> g2l;
+    g2l.reserve(dm->local_size - num_constrained);
+    for (int64_t i = 0; i < dm->local_size; ++i)
+        if (!is_con[i]) g2l.push_back((int32_t)i);
+    if (dm->con.d_dofs) (void)hipFree(dm->con.d_dofs);
+    if (dm->con.d_g2l) (void)hipFree(dm->con.d_g2l);
+    dm->con.d_dofs = nullptr;
+    dm->con.d_g2l = nullptr;
+    dm->con.dofs.assign(dofs, dofs + num_constrained);
+    dm->con.global_size = (int64_t)g2l.size();
+    int err = 0;
+    if ((err = upload(dm, &dm->con.d_dofs, dm->con.dofs.data(), dm->con.dofs.size()))) return err;
+    if ((err = upload(dm, &dm->con.d_g2l, g2l.data(), g2l.size()))) return err;
+    return 0;
+}
+
+int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1) {
+    if (!dm) return 1;
+    if (!known_residual_pair(f0, f1)) return set_error(dm, 1, "no device twin for this residual kernel pair");
+    KeyPlan *kp = nullptr;
+    int err = get_or_create_key(dm, label_value, part, &kp);
+    if (err) return err;
+    kp->f0 = f0;
+    kp->f1 = f1;
+    return 0;
+}
+
+int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, int jf2, int jf3) {
+    if (!dm) return 1;
+    if (jf0 || jf1 || jf2 ||
+        (jf3 != PLX_JF3VU_ISOLINEAR_ELASTICITY_3D && jf3 != PLX_JF3VU_GENMAXWELL_3D))
+        return set_error(dm, 1, "no device twin for this Jacobian kernel set");
+    KeyPlan *kp = nullptr;
+    int err = get_or_create_key(dm, label_value, part, &kp);
+    if (err) return err;
+    kp->jf0 = jf0;
+    kp->jf1 = jf1;
+    kp->jf2 = jf2;
+    kp->jf3 = jf3;
+    return 0;
+}
+
+int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const double *constants) {
+    if (!dm || n < 0 || (n > 0 && !constants)) return 1;
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp) return set_error(dm, 1, "unknown key");
+    kp->constants.assign(constants, constants + n);
+    return 0;
+}
+
+int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux, const int32_t *aux_sizes,
+                         int aux_per_qp, const double *aux) {
+    if (!dm || num_aux < 2 || !aux_sizes || !aux) return set_error(dm, 1, "bad auxiliary arguments");
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp) return set_error(dm, 1, "unknown key");
+    int naux_tot = 0;
+    std::vector<int> off(num_aux);
+    for (int i = 0; i < num_aux; ++i) {
+        if (aux_sizes[i] <= 0) return set_error(dm, 1, "bad aux subfield size");
+        off[i] = naux_tot;
+        naux_tot += aux_sizes[i];
+    }
+    const int per = aux_per_qp ? dm->nq : 1;
+    const int n = (int)kp->cells.size();
+    // Elastic moduli: the last two subfields (IsotropicLinearElasticity::setContext,
+    // IsotropicLinearElasticity.hh:116-117) or, for generalized Maxwell, subfields numA-6 and
+    // numA-5 (IsotropicLinearGenMaxwell.hh:133-134).  Per-cell values; per-qp aux must be
+    // uniform within a cell for these subfields.
+    const bool genmax = kp->f1 == PLX_F1V_GENMAXWELL_3D || kp->jf3 == PLX_JF3VU_GENMAXWELL_3D;
+    if (genmax && num_aux < 7) return set_error(dm, 1, "generalized Maxwell needs 7 aux subfields");
+    const int i_mu = genmax ? num_aux - 6 : num_aux - 2;
+    const int i_k = genmax ? num_aux - 5 : num_aux - 1;
+    std::vector<double> m64(2 * (size_t)n);
+    std::vector<float> m32(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const double *a = aux + (size_t)kp->cells[i] * per * naux_tot;
+        const double mu = a[off[i_mu]], K = a[off[i_k]];
+        if (!(mu > 0.0) || !(K > 0.0)) return set_error(dm, 1, "non-positive shear or bulk modulus");
+        for (int q = 1; q < per; ++q)
+            if (a[q * naux_tot + off[i_mu]] != mu || a[q * naux_tot + off[i_k]] != K)
+                return set_error(dm, 1, "moduli must be uniform within a cell");
+        m64[i] = K - 2.0 / 3.0 * mu;  // lambda
+        m64[n + i] = mu;
+        m32[i] = (float)m64[i];
+        m32[n + i] = (float)mu;
+    }
+    if (kp->d_mat64) (void)hipFree(kp->d_mat64);
+    if (kp->d_mat32) (void)hipFree(kp->d_mat32);
+    int err = 0;
+    if ((err = upload(dm, &kp->d_mat64, m64.data(), m64.size()))) return err;
+    if ((err = upload(dm, &kp->d_mat32, m32.data(), m32.size()))) return err;
+    kp->num_aux = num_aux;
+    kp->aux_sizes.assign(aux_sizes, aux_sizes + num_aux);
+    kp->aux_per_qp = aux_per_qp;
+    kp->aux_host.assign(aux, aux + (size_t)dm->num_cells * per * naux_tot);
+    kp->has_aux = true;
+    return 0;
+}
+
+int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, double t,
+                         const void *locX, const void *locX_t, void *locF, int mode) {
+    (void)t;
+    (void)locX_t;
+    if (!dm || !locX || !locF) return set_error(dm, 1, "null vector");
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp || !kp->f1) return set_error(dm, 1, "no residual kernels registered for key");
+    if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
+    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D)
+        return set_error(dm, 1, "residual kernel not supported by this entry point");
+    return launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
+}
+
+int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar_type, double t,
+                                double s_tshift, const void *locX, const void *locX_t,
+                                const void *locY, void *locF, int mode) {
+    (void)t;
+    (void)s_tshift;
+    (void)locX;
+    (void)locX_t;
+    if (!dm || !locY || !locF) return set_error(dm, 1, "null vector");
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp || !kp->jf3) return set_error(dm, 1, "no Jacobian kernels registered for key");
+    if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
+    if (kp->jf3 != PLX_JF3VU_ISOLINEAR_ELASTICITY_3D)
+        return set_error(dm, 1, "Jacobian kernel not supported by this entry point");
+    return launch_apply(dm, kp, scalar_type, true, locY, locF, mode);
+}
+
+int plx_compute_jacobian_pbdiag(plx_dm dm, int label_value, int part, double t, double s_tshift,
+                                const double *locX, double *blocks) {
+    (void)t;
+    (void)s_tshift;
+    (void)locX;
+    if (!dm || !blocks) return set_error(dm, 1, "null vector");
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp || !kp->jf3) return set_error(dm, 1, "no Jacobian kernels registered for key");
+    if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
+    return launch_pbdiag(dm, kp, blocks);
+}
+
+int plx_local_to_global(plx_dm dm, int scalar_type, const void *loc, void *glob, int mode) {
+    if (!dm || !loc || !glob) return set_error(dm, 1, "null vector");
+    return launch_l2g(dm, scalar_type, loc, glob, mode);
+}
+
+int plx_global_to_local(plx_dm dm, int scalar_type, const void *glob, void *loc) {
+    if (!dm || !loc || !glob) return set_error(dm, 1, "null vector");
+    return launch_g2l(dm, scalar_type, glob, loc);
+}
+
+int plx_insert_boundary_values(plx_dm dm, int scalar_type, const void *values, void *loc) {
+    if (!dm || !loc || (!values && !dm->con.dofs.empty())) return set_error(dm, 1, "null vector");
+    return launch_insert_bc(dm, scalar_type, values, loc);
+}
+
+int plx_vec_dot(plx_dm dm, int scalar_type, int64_t n, const void *x, const void *y, double *result) {
+    if (!dm || !x || !y || !result || n < 0) return set_error(dm, 1, "bad dot arguments");
+    return launch_dot(dm, scalar_type, n, x, y, result);
+}
+
+int plx_vec_norm2(plx_dm dm, int scalar_type, int64_t n, const void *x, double *result) {
+    if (!dm || !x || !result || n < 0) return set_error(dm, 1, "bad norm arguments");
+    double d = 0.0;
+    const int err = launch_dot(dm, scalar_type, n, x, x, &d);
+    if (err) return err;
+    *result = std::sqrt(d);
+    return 0;
+}
+
+}  // extern "C"

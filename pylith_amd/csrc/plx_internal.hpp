@@ -1,0 +1,132 @@
+// plx_internal.hpp -- device data layout of the MI355X assembly path (private to the library).
+//
+// A DM holds the mesh once on the device.  Each weak-form key (label value, equation part)
+// owns a tile plan: the key's cells cut into tiles of spatially adjacent cells, one tile per
+// workgroup.  A tile lists the nodes its cells touch; nodes touched by no other tile are
+// "owned" and written straight to the output vector, the rest are "shared" and leave the
+// tile as partial sums in a halo buffer that a second, node-ordered pass reduces in a fixed
+// order.  Local closures are 16-bit indices into the tile's node list so the cell kernels
+// gather from and scatter into LDS only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/pylith_amd.h"
+#include "plx_plan.hpp"
+
+namespace plx {
+
+constexpr int kMaxQuad = 14;
+
+// Device-side arrays of one key's tile plan (passed by value to kernels).
+struct TilePlanDev {
+    int num_tiles;
+    int stride;                      // SoA stride of per-cell arrays (= number of key cells)
+    const int32_t *tile_cell_ptr;    // [num_tiles+1] cell positions
+    const int32_t *tile_node_ptr;    // [num_tiles+1]
+    const int32_t *tile_nown;        // [num_tiles] owned nodes lead each tile's node list
+    const int32_t *tile_nodes;       // global node ids
+    const int32_t *tile_vert_ptr;    // [num_tiles+1]
+    const int32_t *tile_verts;       // global vertex ids
+    const int32_t *tile_halo_off;    // [num_tiles] first halo slot of the tile's shared nodes
+    const uint16_t *lclosure;        // [nb][stride] tile-local node index
+    const uint16_t *lcone;           // [nv][stride] tile-local vertex index
+    const double *coords;            // [num_vertices][3]
+};
+
+struct KeyPlan {
+    int label_value = 0;
+    int part = 0;
+    int f0 = 0, f1 = 0;
+    int jf0 = 0, jf1 = 0, jf2 = 0, jf3 = 0;
+    std::vector<int32_t> cells;      // key cells, tile order
+    int num_tiles = 0;
+    int max_nodes = 0, max_verts = 0;
+    int64_t halo_slots = 0;
+    int num_shared = 0;
+    // device arrays
+    int32_t *d_cells = nullptr;
+    int32_t *d_tile_cell_ptr = nullptr, *d_tile_node_ptr = nullptr, *d_tile_nown = nullptr;
+    int32_t *d_tile_nodes = nullptr, *d_tile_vert_ptr = nullptr, *d_tile_verts = nullptr;
+    int32_t *d_tile_halo_off = nullptr;
+    uint16_t *d_lclosure = nullptr, *d_lcone = nullptr;
+    uint8_t *d_color = nullptr;      // [ncells] colour within the tile (cells sorted by colour)
+    int32_t *d_tile_ncolor = nullptr;
+    int max_colors = 0;
+    int32_t *d_shared_node = nullptr, *d_shared_ptr = nullptr, *d_shared_slot = nullptr;
+    void *d_halo = nullptr;          // halo partial sums, sized for FP64
+    // auxiliary field, repacked per key cell (SoA, tile order)
+    int num_aux = 0;
+    std::vector<int32_t> aux_sizes;
+    int aux_per_qp = 0;
+    bool has_aux = false;
+    double *d_mat64 = nullptr;       // [2][stride]: lambda, mu (elastic part)
+    float *d_mat32 = nullptr;        // same in FP32
+    double *d_state = nullptr;       // per-qp rheology state (GenMaxwell), SoA [comp][stride*nq]
+    int state_ncomp = 0;
+    std::vector<double> aux_host;    // captured aux (original layout) for state kernels
+    std::vector<double> constants;
+};
+
+struct Constraints {
+    std::vector<int32_t> dofs;       // constrained local dofs (caller order)
+    int32_t *d_dofs = nullptr;
+    int32_t *d_g2l = nullptr;        // [global] local index
+    int64_t global_size = 0;
+};
+
+}  // namespace plx
+
+struct plx_dm_s {
+    int cell_type = 0;
+    int nb = 0, nv = 0, nq = 0;
+    int num_cells = 0, num_vertices = 0, num_nodes = 0;
+    int64_t local_size = 0;
+    std::vector<int32_t> closure, cone, labels;
+    std::vector<double> coords;
+    double *d_coords = nullptr;
+    int32_t *d_closure = nullptr, *d_cone = nullptr;  // uploaded on first use (set-up kernels)
+    hipStream_t stream = nullptr;
+    std::vector<plx::KeyPlan *> keys;
+    plx::Constraints con;
+    double *d_scalar = nullptr;      // reduction scratch
+    double *d_partials = nullptr;
+    int partials_len = 0;
+    std::string last_error;
+};
+
+namespace plx {
+
+// Error plumbing: every C-ABI function funnels failures through set_error.
+int set_error(plx_dm dm, int code, const std::string &msg);
+
+#define PLX_HIP(dm, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return plx::set_error(dm, 76, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+KeyPlan *find_key(plx_dm dm, int label_value, int part);
+
+// Kernel launchers (plx_assembly.hip).
+int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
+                 void *r, int mode);
+int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
+int launch_state_update(plx_dm dm, KeyPlan *kp, const double *u);
+int validate_geometry(plx_dm dm, int *num_bad);
+
+// Vector kernels (plx_vector.hip).
+int launch_l2g(plx_dm dm, int scalar_type, const void *loc, void *glob, int mode);
+int launch_g2l(plx_dm dm, int scalar_type, const void *glob, void *loc);
+int launch_insert_bc(plx_dm dm, int scalar_type, const void *values, void *loc);
+int launch_dot(plx_dm dm, int scalar_type, int64_t n, const void *x, const void *y,
+               double *result);
+
+}  // namespace plx

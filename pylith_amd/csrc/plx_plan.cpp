@@ -1,0 +1,220 @@
+// plx_plan.cpp -- tile plan of one weak-form key, built on the host (no device calls).
+//
+// The key's cells (PyLith's DSLabelAccess pointsIS: cells with the key's label value,
+// feassemble/DSLabelAccess.icc:53-84) are cut, in cell order, into tiles bounded by the
+// workgroup size and the LDS capacity.  Per tile: greedy cell colouring (cells sharing a node
+// get different colours; cells sorted by colour), the node list with owned nodes first, 16-bit
+// local closures, and halo slots for nodes other tiles also touch.
+#include <algorithm>
+
+#include "plx_plan.hpp"
+
+namespace plx {
+
+int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
+                    std::string *err) {
+    const int nb = m.nb, nv = m.nv;
+    std::vector<int32_t> &cells = hp->cells;
+    cells.clear();
+    for (int c = 0; c < m.num_cells; ++c)
+        if ((m.labels ? m.labels[c] : 0) == label_value) cells.push_back(c);
+    const int ncells = (int)cells.size();
+    if (ncells == 0) {
+        *err = "no cells carry the key's label value";
+        return 1;
+    }
+
+    // Greedy tiling in cell order, bounded by LDS capacity.
+    std::vector<int32_t> node_mark(m.num_nodes, -1), vert_mark(m.num_vertices, -1);
+    std::vector<int32_t> &tile_cell_ptr = hp->tile_cell_ptr;
+    tile_cell_ptr.assign(1, 0);
+    {
+        int tcells = 0, tnodes = 0, tverts = 0, tile = 0;
+        for (int i = 0; i < ncells; ++i) {
+            const int32_t *cl = m.closure + (size_t)cells[i] * nb;
+            const int32_t *cn = m.cone + (size_t)cells[i] * nv;
+            int newn = 0, newv = 0;
+            for (int b = 0; b < nb; ++b) newn += node_mark[cl[b]] != tile;
+            for (int b = 0; b < nv; ++b) newv += vert_mark[cn[b]] != tile;
+            if (tcells == lim.cells || tnodes + newn > lim.nodes || tverts + newv > lim.verts) {
+                tile_cell_ptr.push_back(i);
+                tcells = tnodes = tverts = 0;
+                ++tile;
+            }
+            for (int b = 0; b < nb; ++b)
+                if (node_mark[cl[b]] != tile) {
+                    node_mark[cl[b]] = tile;
+                    ++tnodes;
+                }
+            for (int b = 0; b < nv; ++b)
+                if (vert_mark[cn[b]] != tile) {
+                    vert_mark[cn[b]] = tile;
+                    ++tverts;
+                }
+            ++tcells;
+        }
+        tile_cell_ptr.push_back(ncells);
+    }
+    const int ntiles = (int)tile_cell_ptr.size() - 1;
+    hp->num_tiles = ntiles;
+
+    // How many tiles touch each node.
+    std::vector<int32_t> tile_count(m.num_nodes, 0);
+    std::fill(node_mark.begin(), node_mark.end(), -1);
+    for (int t = 0; t < ntiles; ++t)
+        for (int i = tile_cell_ptr[t]; i < tile_cell_ptr[t + 1]; ++i) {
+            const int32_t *cl = m.closure + (size_t)cells[i] * nb;
+            for (int b = 0; b < nb; ++b)
+                if (node_mark[cl[b]] != t) {
+                    node_mark[cl[b]] = t;
+                    ++tile_count[cl[b]];
+                }
+        }
+
+    // Greedy colouring inside each tile, then cells sorted by colour.
+    hp->color.assign(ncells, 0);
+    hp->tile_ncolor.assign(ntiles, 0);
+    hp->max_colors = 0;
+    {
+        std::vector<uint64_t> used(2 * (size_t)m.num_nodes, 0);
+        std::vector<std::pair<int, int32_t>> order;
+        for (int t = 0; t < ntiles; ++t) {
+            const int b0 = tile_cell_ptr[t], b1 = tile_cell_ptr[t + 1];
+            order.clear();
+            int nc = 0;
+            for (int i = b0; i < b1; ++i) {
+                const int32_t *cl = m.closure + (size_t)cells[i] * nb;
+                uint64_t m0 = 0, m1 = 0;
+                for (int b = 0; b < nb; ++b) {
+                    m0 |= used[2 * (size_t)cl[b]];
+                    m1 |= used[2 * (size_t)cl[b] + 1];
+                }
+                int c = 0;
+                while (c < 128 && ((c < 64 ? (m0 >> c) : (m1 >> (c - 64))) & 1)) ++c;
+                if (c >= 128) {
+                    *err = "tile colouring needs more than 128 colours";
+                    return 1;
+                }
+                for (int b = 0; b < nb; ++b)
+                    used[2 * (size_t)cl[b] + (c >> 6)] |= (uint64_t)1 << (c & 63);
+                order.emplace_back(c, cells[i]);
+                nc = std::max(nc, c + 1);
+            }
+            for (int i = b0; i < b1; ++i) {
+                const int32_t *cl = m.closure + (size_t)cells[i] * nb;
+                for (int b = 0; b < nb; ++b) used[2 * (size_t)cl[b]] = used[2 * (size_t)cl[b] + 1] = 0;
+            }
+            std::stable_sort(order.begin(), order.end(),
+                             [](const std::pair<int, int32_t> &a, const std::pair<int, int32_t> &b) {
+                                 return a.first < b.first;
+                             });
+            for (int i = b0; i < b1; ++i) {
+                hp->color[i] = (uint8_t)order[i - b0].first;
+                cells[i] = order[i - b0].second;
+            }
+            hp->tile_ncolor[t] = nc;
+            hp->max_colors = std::max(hp->max_colors, nc);
+        }
+    }
+
+    // Per-tile node / vertex lists and local closures.
+    hp->tile_node_ptr.assign(1, 0);
+    hp->tile_vert_ptr.assign(1, 0);
+    hp->tile_nown.clear();
+    hp->tile_nodes.clear();
+    hp->tile_verts.clear();
+    hp->tile_halo_off.clear();
+    hp->lclosure.assign((size_t)nb * ncells, 0);
+    hp->lcone.assign((size_t)nv * ncells, 0);
+    std::vector<int32_t> local_of(m.num_nodes, -1), vlocal_of(m.num_vertices, -1);
+    std::vector<std::pair<int32_t, int32_t>> shared_pairs;  // (node, halo slot)
+    int64_t halo = 0;
+    int max_nodes = 0, max_verts = 0;
+    std::vector<int32_t> own, sh, vl;
+    for (int t = 0; t < ntiles; ++t) {
+        own.clear();
+        sh.clear();
+        vl.clear();
+        for (int i = tile_cell_ptr[t]; i < tile_cell_ptr[t + 1]; ++i) {
+            const int32_t *cl = m.closure + (size_t)cells[i] * nb;
+            const int32_t *cn = m.cone + (size_t)cells[i] * nv;
+            for (int b = 0; b < nb; ++b)
+                if (local_of[cl[b]] == -1) {
+                    local_of[cl[b]] = 0;
+                    (tile_count[cl[b]] == 1 ? own : sh).push_back(cl[b]);
+                }
+            for (int b = 0; b < nv; ++b)
+                if (vlocal_of[cn[b]] == -1) {
+                    vlocal_of[cn[b]] = 0;
+                    vl.push_back(cn[b]);
+                }
+        }
+        std::sort(own.begin(), own.end());
+        std::sort(sh.begin(), sh.end());
+        std::sort(vl.begin(), vl.end());
+        int k = 0;
+        for (int32_t n : own) {
+            local_of[n] = k++;
+            hp->tile_nodes.push_back(n);
+        }
+        for (int32_t n : sh) {
+            shared_pairs.emplace_back(n, (int32_t)(halo + (k - (int)own.size())));
+            local_of[n] = k++;
+            hp->tile_nodes.push_back(n);
+        }
+        hp->tile_nown.push_back((int32_t)own.size());
+        hp->tile_halo_off.push_back((int32_t)halo);
+        halo += (int64_t)sh.size();
+        hp->tile_node_ptr.push_back((int32_t)hp->tile_nodes.size());
+        int kv = 0;
+        for (int32_t v : vl) {
+            vlocal_of[v] = kv++;
+            hp->tile_verts.push_back(v);
+        }
+        hp->tile_vert_ptr.push_back((int32_t)hp->tile_verts.size());
+        max_nodes = std::max(max_nodes, k);
+        max_verts = std::max(max_verts, kv);
+        for (int i = tile_cell_ptr[t]; i < tile_cell_ptr[t + 1]; ++i) {
+            const int32_t *cl = m.closure + (size_t)cells[i] * nb;
+            const int32_t *cn = m.cone + (size_t)cells[i] * nv;
+            for (int b = 0; b < nb; ++b) hp->lclosure[(size_t)b * ncells + i] = (uint16_t)local_of[cl[b]];
+            for (int b = 0; b < nv; ++b) hp->lcone[(size_t)b * ncells + i] = (uint16_t)vlocal_of[cn[b]];
+        }
+        for (int32_t n : own) local_of[n] = -1;
+        for (int32_t n : sh) local_of[n] = -1;
+        for (int32_t v : vl) vlocal_of[v] = -1;
+    }
+    if (m.cone_is_closure) {
+        hp->tile_verts = hp->tile_nodes;
+        hp->tile_vert_ptr = hp->tile_node_ptr;
+        hp->lcone = hp->lclosure;
+        max_verts = max_nodes;
+    }
+    if (halo > INT32_MAX) {
+        *err = "halo too large";
+        return 1;
+    }
+    hp->halo_slots = halo;
+    hp->max_nodes = max_nodes;
+    hp->max_verts = max_verts;
+
+    // Shared-node reduction map: nodes ascending, slots in tile order (stable sort).
+    std::stable_sort(shared_pairs.begin(), shared_pairs.end(),
+                     [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
+                         return a.first < b.first;
+                     });
+    hp->shared_node.clear();
+    hp->shared_slot.clear();
+    hp->shared_ptr.assign(1, 0);
+    for (size_t i = 0; i < shared_pairs.size(); ++i) {
+        if (i > 0 && shared_pairs[i].first != shared_pairs[i - 1].first)
+            hp->shared_ptr.push_back((int32_t)i);
+        if (i == 0 || shared_pairs[i].first != shared_pairs[i - 1].first)
+            hp->shared_node.push_back(shared_pairs[i].first);
+        hp->shared_slot.push_back(shared_pairs[i].second);
+    }
+    if (!shared_pairs.empty()) hp->shared_ptr.push_back((int32_t)shared_pairs.size());
+    return 0;
+}
+
+}  // namespace plx

@@ -1,0 +1,147 @@
+"""Python handle over the C ABI: a device-resident mesh + section + weak form.
+
+Mirrors the objects PyLith's assembly layer drives (a PETSc DM with its section, PetscDS
+weak form and auxiliary vectors; feassemble/IntegratorDomain.cc) with the device twins of
+the pointwise kernels.  Vectors are torch tensors on the GPU (device memory plumbing only).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+class PlxError(RuntimeError):
+    pass
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(L._i32p)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(L._dblp)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+        raise PlxError("vectors must be contiguous CUDA (HIP) tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stype(t):
+    if t.dtype == torch.float64:
+        return L.FP64
+    if t.dtype == torch.float32:
+        return L.FP32
+    raise PlxError(f"unsupported dtype {t.dtype}")
+
+
+class DM:
+    """Mesh + section + weak form on the device (one stream, one host thread)."""
+
+    def __init__(self, mesh, labels=None, stream=None):
+        self.lib = L.load()
+        self.mesh = mesh
+        h = ctypes.c_void_p()
+        cl, clp = _i32(mesh.closure)
+        co, cop = _i32(mesh.cone)
+        xy, xyp = _f64(mesh.coords)
+        lab = labels if labels is not None else mesh.labels
+        if lab is not None:
+            lb, lbp = _i32(lab)
+        else:
+            lbp = None
+        err = self.lib.plx_dm_create(ctypes.byref(h), mesh.cell_type, mesh.num_cells,
+                                     mesh.num_vertices, mesh.num_nodes, clp, cop, xyp, lbp)
+        if err:
+            raise PlxError(f"plx_dm_create failed with code {err}")
+        self.h = h
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        self._check(self.lib.plx_dm_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)))
+
+    def _check(self, err):
+        if err:
+            msg = self.lib.plx_dm_last_error(self.h)
+            raise PlxError(f"code {err}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.plx_dm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- section / constraints -----------------------------------------------------------
+    def sizes(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.plx_dm_get_sizes(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_constraints(self, dofs):
+        d, dp = _i32(dofs)
+        self._check(self.lib.plx_dm_set_constraints(self.h, d.size, dp))
+
+    # --- weak form ---------------------------------------------------------------------------
+    def add_residual(self, label_value, part, f0, f1):
+        self._check(self.lib.plx_dm_add_residual(self.h, label_value, part, f0, f1))
+
+    def add_jacobian(self, label_value, part, jf0, jf1, jf2, jf3):
+        self._check(self.lib.plx_dm_add_jacobian(self.h, label_value, part, jf0, jf1, jf2, jf3))
+
+    def set_auxiliary(self, label_value, part, aux, aux_sizes, per_qp=False):
+        a, ap = _f64(aux)
+        s, sp = _i32(aux_sizes)
+        self._check(self.lib.plx_dm_set_auxiliary(self.h, label_value, part, s.size, sp,
+                                                  int(per_qp), ap))
+
+    def set_constants(self, label_value, part, constants):
+        c, cp = _f64(constants)
+        self._check(self.lib.plx_dm_set_constants(self.h, label_value, part, c.size, cp))
+
+    # --- assembly ----------------------------------------------------------------------------
+    def compute_residual(self, label_value, part, locX, locF, mode=L.ADD_VALUES, t=0.0,
+                         locX_t=None):
+        self._check(self.lib.plx_compute_residual(self.h, label_value, part, _stype(locX), t,
+                                                  _ptr(locX), _ptr(locX_t), _ptr(locF), mode))
+
+    def compute_jacobian_action(self, label_value, part, locY, locF, mode=L.ADD_VALUES, t=0.0,
+                                s_tshift=0.0, locX=None, locX_t=None):
+        self._check(self.lib.plx_compute_jacobian_action(self.h, label_value, part, _stype(locY),
+                                                         t, s_tshift, _ptr(locX), _ptr(locX_t),
+                                                         _ptr(locY), _ptr(locF), mode))
+
+    def compute_jacobian_pbdiag(self, label_value, part, blocks, t=0.0, s_tshift=0.0, locX=None):
+        self._check(self.lib.plx_compute_jacobian_pbdiag(self.h, label_value, part, t, s_tshift,
+                                                         _ptr(locX), _ptr(blocks)))
+
+    # --- field scatters ------------------------------------------------------------------------
+    def local_to_global(self, loc, glob, mode=L.ADD_VALUES):
+        self._check(self.lib.plx_local_to_global(self.h, _stype(loc), _ptr(loc), _ptr(glob), mode))
+
+    def global_to_local(self, glob, loc):
+        self._check(self.lib.plx_global_to_local(self.h, _stype(loc), _ptr(glob), _ptr(loc)))
+
+    def insert_boundary_values(self, values, loc):
+        self._check(self.lib.plx_insert_boundary_values(self.h, _stype(loc), _ptr(values), _ptr(loc)))
+
+    def dot(self, x, y):
+        r = ctypes.c_double()
+        self._check(self.lib.plx_vec_dot(self.h, _stype(x), x.numel(), _ptr(x), _ptr(y),
+                                         ctypes.byref(r)))
+        return r.value
+
+    def norm2(self, x):
+        r = ctypes.c_double()
+        self._check(self.lib.plx_vec_norm2(self.h, _stype(x), x.numel(), _ptr(x), ctypes.byref(r)))
+        return r.value

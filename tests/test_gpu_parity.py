@@ -1,0 +1,195 @@
+"""Parity of the HIP residual / Jacobian-action path (through the C ABI) with the CPU oracle.
+
+Bar: FP64 within 1e-12 relative (||r_gpu - r_cpu||_inf / ||r_cpu||_inf); FP32 within 2e-5.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from pylith_amd import meshgen as mg  # noqa: E402
+from pylith_amd import meshio_ascii  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+F1, JF3 = 1, 2
+TOL64, TOL32 = 1e-12, 2e-5
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def make_dm(m, aux, sizes, label=0):
+    from pylith_amd.dm import DM
+    dm = DM(m)
+    dm.add_residual(label, 0, 0, F1)
+    dm.add_jacobian(label, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(label, 0, aux, sizes)
+    return dm
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def check_residual(oracle, m, u, label=0, aux=None, dtypes=(torch.float64, torch.float32)):
+    if aux is None:
+        aux, sizes = mg.elastic_aux(m.num_cells)
+    else:
+        aux, sizes = aux
+    P = oracle.Problem(m, aux, sizes)
+    ref = P.residual(u, F1, cells=np.nonzero(m.labels == label)[0] if label else None)
+    dm = make_dm(m, aux, sizes, label)
+    print(f"[parity] cells={m.num_cells} nodes={m.num_nodes} ct={m.cell_type}", flush=True)
+    for dt in dtypes:
+        x = dev(u, dt)
+        r = torch.zeros_like(x)
+        dm.compute_residual(label, 0, x, r, mode=1)
+        torch.cuda.synchronize()
+        e = relerr(r.double().cpu().numpy(), ref)
+        tol = TOL64 if dt == torch.float64 else TOL32
+        assert e <= tol, (dt, e)
+        # ADD mode accumulates on top of existing values
+        r2 = torch.full_like(x, 1.0)
+        dm.compute_residual(label, 0, x, r2, mode=0)
+        torch.cuda.synchronize()
+        assert relerr(r2.double().cpu().numpy() - 1.0, ref) <= 10 * tol
+        # Jacobian action (linear elasticity: J v = F(v))
+        jv = torch.zeros_like(x)
+        dm.compute_jacobian_action(label, 0, x, jv, mode=1)
+        torch.cuda.synchronize()
+        assert relerr(jv.double().cpu().numpy(), ref) <= tol
+    return dm, ref
+
+
+def test_q1_config1_32cube(oracle):
+    m = mg.hex_box(32, (0, 0, 0), (6000.0, 6000.0, 6000.0))
+    u = mg.random_field(m.num_nodes, 20261202)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    u[dofs] = vals
+    check_residual(oracle, m, u)
+
+
+def test_q1_nonaffine_and_unordered(oracle):
+    m = mg.hex_box((9, 7, 5), (0, 0, 0), (900.0, 700.0, 500.0), ordered=False)
+    rng = np.random.default_rng(11)
+    x = m.coords.copy()
+    inner = np.all((x > 1) & (x < np.array([899, 699, 499])), axis=1)
+    x[inner] += rng.uniform(-25, 25, (inner.sum(), 3))
+    m.coords = m.node_coords = x
+    u = mg.random_field(m.num_nodes, 4)
+    check_residual(oracle, m, u)
+
+
+def test_q1_reference_fixture_mesh_and_label(oracle):
+    m, groups = meshio_ascii.read(os.path.join(GOLDEN, "hex.mesh"))
+    u = mg.random_field(m.num_nodes, 9)
+    aux, sizes = mg.elastic_aux(m.num_cells, 2500.0, 3000.0, math.sqrt(3.0) * 3000.0)
+    check_residual(oracle, m, u, label=24, aux=(aux, sizes))
+
+
+def test_q1_two_materials(oracle):
+    m = mg.hex_box(8, (0, 0, 0), (800.0, 800.0, 800.0), brick=4)
+    m.labels = (m.coords[m.cone[:, 0], 2] >= 400).astype(np.int32) + 1
+    u = mg.random_field(m.num_nodes, 12)
+    aux1, sizes = mg.elastic_aux(m.num_cells, vs=3000.0, vp=5292.0)
+    aux2, _ = mg.elastic_aux(m.num_cells, vs=2000.0, vp=3600.0)
+    P1, P2 = oracle.Problem(m, aux1, sizes), oracle.Problem(m, aux2, sizes)
+    ref = P1.residual(u, F1, cells=np.nonzero(m.labels == 1)[0])
+    ref = P2.residual(u, F1, cells=np.nonzero(m.labels == 2)[0], out=ref)
+    from pylith_amd.dm import DM
+    dm = DM(m)
+    for lab, aux in ((1, aux1), (2, aux2)):
+        dm.add_residual(lab, 0, 0, F1)
+        dm.set_auxiliary(lab, 0, aux, sizes)
+    x = dev(u)
+    r = torch.zeros_like(x)
+    dm.compute_residual(1, 0, x, r, mode=0)
+    dm.compute_residual(2, 0, x, r, mode=0)
+    torch.cuda.synchronize()
+    assert relerr(r.cpu().numpy(), ref) <= TOL64
+
+
+def test_single_cell_and_ragged_tiles(oracle):
+    for n in ((1, 1, 1), (13, 3, 2)):
+        m = mg.hex_box(n, (0, 0, 0), (1.0 * n[0], 1.0 * n[1], 1.0 * n[2]))
+        check_residual(oracle, m, mg.random_field(m.num_nodes, 1))
+    t = mg.tet_p2_box((1, 1, 1), (0, 0, 0), (1, 1, 1), brick=1)
+    check_residual(oracle, t, mg.random_field(t.num_nodes, 2))
+
+
+def test_p2_tets_small(oracle):
+    m = mg.tet_p2_box(8, (0, 0, 0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=3, vs_sigma=0.2, vp_vs=1.732)
+    u = mg.sinusoid_field(m.node_coords, 17)
+    check_residual(oracle, m, u, aux=(aux, sizes))
+
+
+def test_p2_tets_config2_64cube(oracle):
+    m = mg.tet_p2_box(64, (0, 0, 0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=20261202, vs_sigma=0.2, vp_vs=1.732)
+    u = mg.sinusoid_field(m.node_coords, 20261202)
+    check_residual(oracle, m, u, aux=(aux, sizes))
+
+
+def test_deterministic_bitwise(oracle):
+    m = mg.tet_p2_box(8, (0, 0, 0), (1.0, 1.0, 1.0))
+    aux, sizes = mg.elastic_aux(m.num_cells)
+    dm = make_dm(m, aux, sizes)
+    x = dev(mg.random_field(m.num_nodes, 3))
+    outs = []
+    for _ in range(3):
+        r = torch.zeros_like(x)
+        dm.compute_residual(0, 0, x, r, mode=1)
+        outs.append(r.cpu().numpy())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_pbdiag_matches_oracle(oracle):
+    for m in (mg.hex_box(4, (0, 0, 0), (4.0, 4.0, 4.0), brick=2),
+              mg.tet_p2_box(2, (0, 0, 0), (1.0, 1.0, 1.0), brick=1)):
+        aux, sizes = mg.elastic_aux(m.num_cells)
+        ref = oracle.Problem(m, aux, sizes).pbdiag(JF3, m.num_nodes)
+        dm = make_dm(m, aux, sizes)
+        b = torch.zeros(m.num_nodes * 9, dtype=torch.float64, device="cuda")
+        dm.compute_jacobian_pbdiag(0, 0, b)
+        torch.cuda.synchronize()
+        assert relerr(b.cpu().numpy().reshape(-1, 3, 3), ref) <= TOL64
+
+
+def test_scatters_and_reductions(oracle):
+    m = mg.hex_box(4, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+    aux, sizes = mg.elastic_aux(m.num_cells)
+    dm = make_dm(m, aux, sizes)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    dm.set_constraints(dofs)
+    nloc, nglob = dm.sizes()
+    assert nloc == 3 * m.num_nodes and nglob == nloc - dofs.size
+    loc = np.random.default_rng(0).standard_normal(nloc)
+    g = torch.zeros(nglob, dtype=torch.float64, device="cuda")
+    dm.local_to_global(dev(loc), g, mode=0)
+    ref = oracle.local_to_global(loc, dofs)
+    assert np.array_equal(g.cpu().numpy(), ref)
+    back = torch.zeros(nloc, dtype=torch.float64, device="cuda")
+    dm.global_to_local(g, back)
+    dm.insert_boundary_values(dev(vals), back)
+    exp = loc.copy()
+    exp[dofs] = vals
+    assert np.array_equal(back.cpu().numpy(), exp)
+    a = np.random.default_rng(1).standard_normal(100003)
+    b = np.random.default_rng(2).standard_normal(100003)
+    assert np.isclose(dm.dot(dev(a), dev(b)), a @ b, rtol=1e-13)
+    assert np.isclose(dm.norm2(dev(a, torch.float32)), np.linalg.norm(a.astype(np.float32)), rtol=1e-6)
+
+
+def test_bad_geometry_rejected():
+    from pylith_amd.dm import DM, PlxError
+    m = mg.hex_box(1, (0, 0, 0), (1.0, 1.0, 1.0))
+    m.closure = m.closure[:, [1, 0, 3, 2, 5, 4, 7, 6]].copy()  # mirrored: negative determinant
+    m.cone = m.closure.copy()
+    with pytest.raises(PlxError):
+        DM(m)
