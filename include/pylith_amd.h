@@ -134,6 +134,14 @@ int plx_vec_dot(plx_dm dm, int scalar_type, int64_t n, const void *x, const void
                 double *result);
 int plx_vec_norm2(plx_dm dm, int scalar_type, int64_t n, const void *x, double *result);
 
+/* ---------------------------------------------------------------------------------------------
+ * Event timers (the device counterpart of pylith::petsc::EventLogger, petsc/EventLogger.icc:23-47):
+ * when on, every kernel launch of the handle is bracketed by hipEvents on its stream.
+ * Events: "plx:apply_tiles", "plx:reduce_halo".  get_event_stats synchronises the stream. */
+int plx_dm_set_event_logging(plx_dm dm, int on);
+int plx_dm_get_event_stats(plx_dm dm, const char *name, int *count, double *total_ms);
+int plx_dm_reset_event_stats(plx_dm dm);
+
 #ifdef __cplusplus
 }
 #endif

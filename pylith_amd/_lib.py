@@ -46,6 +46,9 @@ SIGNATURES = {
     "plx_insert_boundary_values": (_c_int, [_vp, _c_int, _vp, _vp]),
     "plx_vec_dot": (_c_int, [_vp, _c_int, ctypes.c_int64, _vp, _vp, _dblp]),
     "plx_vec_norm2": (_c_int, [_vp, _c_int, ctypes.c_int64, _vp, _dblp]),
+    "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
+    "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
+    "plx_dm_reset_event_stats": (_c_int, [_vp]),
 }
 
 _lib = None

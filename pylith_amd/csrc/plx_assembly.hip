@@ -224,6 +224,8 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode)
     A.halo = kp->d_halo;
     A.mode = mode;
     const size_t lds = (size_t)kp->max_verts * 3 * sizeof(double) + (size_t)kp->max_nodes * 6 * sizeof(Real);
+    {
+    ScopedEvent ev(dm, "plx:apply_tiles");
     if (dm->cell_type == PLX_HEX_Q1) {
         hipLaunchKernelGGL((apply_tiles<Real, PLX_HEX_Q1, 512>), dim3(kp->num_tiles), dim3(512), lds,
                            dm->stream, A);
@@ -231,8 +233,10 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode)
         hipLaunchKernelGGL((apply_tiles<Real, PLX_TET_P2, 384>), dim3(kp->num_tiles), dim3(384), lds,
                            dm->stream, A);
     }
+    }
     PLX_HIP(dm, hipGetLastError());
     if (kp->num_shared > 0) {
+        ScopedEvent ev(dm, "plx:reduce_halo");
         hipLaunchKernelGGL(reduce_halo<Real>, dim3((kp->num_shared + 255) / 256), dim3(256), 0,
                            dm->stream, kp->num_shared, kp->d_shared_node, kp->d_shared_ptr,
                            kp->d_shared_slot, static_cast<const Real *>(kp->d_halo),

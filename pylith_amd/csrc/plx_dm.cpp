@@ -171,6 +171,7 @@ int plx_dm_create(plx_dm *out, int cell_type, int num_cells, int num_vertices, i
 
 int plx_dm_destroy(plx_dm dm) {
     if (!dm) return 0;
+    events_release(dm);
     for (KeyPlan *k : dm->keys) free_key(k);
     if (dm->d_coords) (void)hipFree(dm->d_coords);
     if (dm->d_closure) (void)hipFree(dm->d_closure);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <map>
 #include <cstdint>
 #include <type_traits>
 #include <string>
@@ -74,6 +75,16 @@ struct KeyPlan {
     std::vector<double> constants;
 };
 
+struct EventLog {
+    struct Pending {
+        std::string name;
+        hipEvent_t begin, end;
+    };
+    bool on = false;
+    std::vector<Pending> pending;
+    std::map<std::string, std::pair<int, double>> stats;  // name -> (count, total ms)
+};
+
 struct Constraints {
     std::vector<int32_t> dofs;       // constrained local dofs (caller order)
     int32_t *d_dofs = nullptr;
@@ -98,6 +109,7 @@ struct plx_dm_s {
     double *d_scalar = nullptr;      // reduction scratch
     double *d_partials = nullptr;
     int partials_len = 0;
+    plx::EventLog events;
     std::string last_error;
 };
 
@@ -114,6 +126,20 @@ int set_error(plx_dm dm, int code, const std::string &msg);
     } while (0)
 
 KeyPlan *find_key(plx_dm dm, int label_value, int part);
+
+// Records a named hipEvent pair around a launch when the handle's event logging is on.
+class ScopedEvent {
+  public:
+    ScopedEvent(plx_dm dm, const char *name);
+    ~ScopedEvent();
+
+  private:
+    plx_dm dm_;
+    const char *name_;
+    hipEvent_t a_ = nullptr;
+    bool active_ = false;
+};
+void events_release(plx_dm dm);
 
 // Kernel launchers (plx_assembly.hip).
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,

@@ -145,3 +145,15 @@ class DM:
         r = ctypes.c_double()
         self._check(self.lib.plx_vec_norm2(self.h, _stype(x), x.numel(), _ptr(x), ctypes.byref(r)))
         return r.value
+
+    # --- event timers ---------------------------------------------------------------------------
+    def set_event_logging(self, on):
+        self._check(self.lib.plx_dm_set_event_logging(self.h, int(on)))
+
+    def event_stats(self, name):
+        n, ms = ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.plx_dm_get_event_stats(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def reset_event_stats(self):
+        self._check(self.lib.plx_dm_reset_event_stats(self.h))
