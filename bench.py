@@ -8,9 +8,13 @@ Vp = 1.732 Vs, density 2500, displacement = sum of 8 seeded sinusoids of amplitu
 One step = one residual apply (INSERT mode: zeroed local residual + assembly) with the
 solution already resident in HBM.  value = dofs / time per apply (GDoF/s).
 
-Also reported (``extra``): the FP32 apply and the FP64 Jacobian action of the same workload,
-and the Q1 hex elasticity apply of the 32^3 cube (6 km, u ~ U[-1e-3, 1e-3] m with the shear
-Dirichlet values inserted) and of a 192^3 cube (working set beyond the 256 MB Infinity Cache).
+The timed kernel is the library's default path: the quadrature sum of the 14-point rule
+evaluated through the linear structure of the pointwise kernel (vertex gradients and the rule's
+P1 moment matrix; identical to the point-by-point sum up to rounding).  ``extra`` also reports
+the point-by-point evaluation of the 14 quadrature points, the FP32 apply, the FP64 Jacobian
+action, and the Q1 hex elasticity apply of the 32^3 cube (6 km, u ~ U[-1e-3, 1e-3] m with the
+shear Dirichlet values inserted) and of a 192^3 cube (working set beyond the 256 MB Infinity
+Cache).
 
 The CPU baseline is the oracle (the C restatement of the reference's cell loop with the
 pointwise kernel called through a function pointer) timed on this host's cores.
@@ -25,7 +29,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
 SEED = 20261202
 
 
@@ -41,10 +44,6 @@ def q1_bytes(nn, nc, s):
     return 2 * s * 3 * nn + 24 * nn + 32 * nc + 2 * s * nc
 
 
-# FP64 flops per P2 cell of the 14-point scheme: per quadrature point 2*3*3*10 (gradient)
-# + 2*27 (to physical) + 15 (stress) + 2*27 (pull-back) + 2*3*3*10 (test functions) = 483,
-# plus 60 for the affine geometry.
-P2_FLOPS_PER_CELL = 14 * 483 + 60
 
 
 def time_applies(fn, steps, warmup, torch):
@@ -134,7 +133,6 @@ def main():
     hms, _ = kernel_avg_ms(dm, apply64, args.steps, torch, "plx:reduce_halo")
     B = p2_bytes(mesh.num_nodes, mesh.num_vertices, mesh.num_cells, 8)
     achieved = B / (kms * 1e-3) / 1e9
-    flops = P2_FLOPS_PER_CELL * mesh.num_cells
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -160,7 +158,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "apply_tiles<double, TET_P2>",
+            "kernel": "apply_tiles<double, TET_P2, factored, lds-atomic>",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -170,12 +168,11 @@ def main():
             "kernel_avg_ms": kms,
             "kernel_launches_timed": kcount,
             "halo_reduce_avg_ms": hms,
-            "fp64_tflops": flops / (kms * 1e-3) / 1e12,
-            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
         },
     }
 
     if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py  # noqa: E402  (checker / CPU baseline only)
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         problem = oracle_py.Problem(mesh, aux, sizes)
@@ -192,6 +189,19 @@ def main():
 
     if not args.no_extra:
         extra = {}
+        from pylith_amd import _lib as L
+        print("[bench] point-by-point 14-point evaluation", file=sys.stderr, flush=True)
+        dm.set_kernel_options(L.ALGO_QUADRATURE_POINTS, L.ACCUMULATE_AUTO)
+        secq = time_applies(apply64, args.steps, args.warmup, torch)
+        kq, _ = kernel_avg_ms(dm, apply64, args.steps, torch)
+        rq = r.clone()
+        dm.set_kernel_options(L.ALGO_AUTO, L.ACCUMULATE_AUTO)
+        apply64()
+        torch.cuda.synchronize()
+        extra["p2_fp64_residual_quadrature_points"] = {
+            "gdofs": ndofs / secq / 1e9, "ms": secq * 1e3, "kernel_ms": kq,
+            "hbm_frac": B / (kq * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "rel_diff_vs_default": float((rq - r).abs().max() / r.abs().max())}
         print("[bench] FP32 and Jacobian action", file=sys.stderr, flush=True)
         x32 = x.float()
         r32 = torch.zeros_like(x32)

@@ -75,6 +75,25 @@ const char *plx_dm_last_error(plx_dm dm);
 /* Local and global vector lengths (dofs). */
 int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size);
 
+/* Element algorithm and in-workgroup accumulation of the cell kernels.
+ *   PLX_ALGO_QUADRATURE_POINTS: the device twin of the pointwise kernel is evaluated at every
+ *     quadrature point (Q1: sum-factorized 2x2x2 Gauss; P2: 14-point rule);
+ *   PLX_ALGO_FACTORED: for pointwise kernels linear in the solution gradient with per-cell
+ *     coefficients, on affine cells, the same quadrature sum is evaluated through the polynomial
+ *     structure of the gradient (Q1: the 8 Walsh-Hadamard modes with the Gauss orthogonality
+ *     weights; P2: the 4 vertex gradients with the rule's P1 moment matrix); other cells fall
+ *     back to quadrature points.  Identical results up to rounding;
+ *   PLX_ALGO_AUTO (default): FACTORED where it applies.
+ *   PLX_ACCUMULATE_COLORED: element vectors enter the tile's LDS node buffer colour by colour, a
+ *     fixed order: results are bitwise reproducible;
+ *   PLX_ACCUMULATE_LDS_ATOMIC: LDS atomic adds, no colour phases; the last bits may vary run to run;
+ *   PLX_ACCUMULATE_AUTO (default): LDS atomics for FP64 calls, colour phases for FP32 calls.
+ * min_waves_per_simd (0 = default 2): occupancy the cell kernels' register allocation must
+ * allow (2: <= 256 VGPRs, 3: <= 168, 4: <= 128); the rest spills. */
+enum plx_algorithm { PLX_ALGO_AUTO = 0, PLX_ALGO_QUADRATURE_POINTS = 1, PLX_ALGO_FACTORED = 2 };
+enum plx_accumulate { PLX_ACCUMULATE_AUTO = 0, PLX_ACCUMULATE_COLORED = 1, PLX_ACCUMULATE_LDS_ATOMIC = 2 };
+int plx_dm_set_kernel_options(plx_dm dm, int algorithm, int accumulate, int min_waves_per_simd);
+
 /* Constrained dofs (Dirichlet, ConstraintSimple/ConstraintSpatialDB; PetscDSAddBoundary at
  * feassemble/ConstraintSimple.cc:122): `dofs` are local dof indices (HOST array). */
 int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs);
@@ -133,6 +152,38 @@ int plx_insert_boundary_values(plx_dm dm, int scalar_type, const void *values, v
 int plx_vec_dot(plx_dm dm, int scalar_type, int64_t n, const void *x, const void *y,
                 double *result);
 int plx_vec_norm2(plx_dm dm, int scalar_type, int64_t n, const void *x, double *result);
+
+/* ---------------------------------------------------------------------------------------------
+ * Linear solves around the matrix-free operator A = -J of one key (SPD for elasticity with the
+ * DM's constrained dofs held fixed), point-block Jacobi preconditioned.  Replaces the KSP solve
+ * of SNESSolve in TimeDependent::solve / GreensFns::solve (problems/TimeDependent.cc:475-484,
+ * problems/GreensFns.cc:326).
+ *   PLX_KSP_CG        FP64 preconditioned CG (the plain FP64 path);
+ *   PLX_KSP_FGMRES_IR FP64 FGMRES on the true residual, each preconditioner application an FP32
+ *                     PCG solve (FP32 apply inside FP64 iterative refinement);
+ *   PLX_KSP_IR        Richardson iterative refinement with the FP32 PCG correction.
+ * x (local FP64, device) holds the initial guess with the boundary values in the constrained dofs
+ * and returns the solution; b (local FP64, device) may be NULL (zero).  Convergence:
+ * ||b - A x|| <= rtol ||b - A x0|| on the unconstrained dofs, the residual always recomputed in
+ * FP64 at the end. */
+typedef struct plx_solver_s *plx_solver;
+enum plx_ksp_type { PLX_KSP_CG = 0, PLX_KSP_FGMRES_IR = 1, PLX_KSP_IR = 2 };
+typedef struct {
+    int outer_iterations;   /* FGMRES / IR / CG iterations */
+    int inner_iterations;   /* FP32 PCG iterations, all preconditioner applications */
+    int applies_fp64;       /* FP64 operator applications */
+    int converged;
+    double initial_residual;
+    double final_residual;  /* true FP64 residual norm */
+} plx_solve_stats;
+int plx_solver_create(plx_solver *solver, plx_dm dm, int label_value, int part);
+int plx_solver_destroy(plx_solver solver);
+/* (Re)compute the point-block preconditioner (after the aux field or constraints change). */
+int plx_solver_setup(plx_solver solver);
+/* Inner FP32 PCG relative tolerance / iteration cap and the FGMRES restart length. */
+int plx_solver_set_inner(plx_solver solver, double rtol, int maxit, int restart);
+int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, double rtol,
+                     int maxit, plx_solve_stats *stats);
 
 /* ---------------------------------------------------------------------------------------------
  * Event timers (the device counterpart of pylith::petsc::EventLogger, petsc/EventLogger.icc:23-47):

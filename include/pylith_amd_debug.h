@@ -5,7 +5,8 @@
  * Jacobian-action kernels on the CPU, bounds-checking every LDS, halo and vector index, so the
  * kernels' indexing can be verified on a machine without a GPU.  Returns 0, 1 (bad
  * arguments) or 3 (an index out of bounds, or two cells of one colour touching one node).
- *   lam_mu [num_cells][2]: Lame lambda and shear modulus per cell (original cell order).
+ *   lam_mu [num_cells][2]: Lame lambda and shear modulus per cell (original cell order);
+ *   algorithm: plx_algorithm (the element path to emulate).
  */
 #ifndef PYLITH_AMD_DEBUG_H
 #define PYLITH_AMD_DEBUG_H
@@ -21,7 +22,7 @@ extern "C" {
 int plx_debug_emulate_apply(int cell_type, int num_cells, int num_vertices, int num_nodes,
                             const int32_t *closure, const int32_t *cone, const double *coords,
                             const int32_t *labels, int label_value, const double *lam_mu,
-                            int scalar_type, const void *u, void *r, int mode);
+                            int scalar_type, const void *u, void *r, int mode, int algorithm);
 
 #ifdef __cplusplus
 }

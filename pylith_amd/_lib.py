@@ -18,6 +18,9 @@ F1V_GENMAXWELL_3D = 3
 JF3VU_GENMAXWELL_3D = 4
 VISCOUS_STRAIN_GENMAXWELL_3D = 5
 TOTAL_STRAIN_3D = 6
+KSP_CG, KSP_FGMRES_IR, KSP_IR = 0, 1, 2
+ALGO_AUTO, ALGO_QUADRATURE_POINTS, ALGO_FACTORED = 0, 1, 2
+ACCUMULATE_AUTO, ACCUMULATE_COLORED, ACCUMULATE_LDS_ATOMIC = 0, 1, 2
 
 _c_int, _c_dbl, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -32,6 +35,7 @@ SIGNATURES = {
     "plx_dm_set_stream": (_c_int, [_vp, _vp]),
     "plx_dm_last_error": (ctypes.c_char_p, [_vp]),
     "plx_dm_get_sizes": (_c_int, [_vp, _i64p, _i64p]),
+    "plx_dm_set_kernel_options": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "plx_dm_set_constraints": (_c_int, [_vp, _c_int, _i32p]),
     "plx_dm_add_residual": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int]),
     "plx_dm_add_jacobian": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
@@ -46,10 +50,25 @@ SIGNATURES = {
     "plx_insert_boundary_values": (_c_int, [_vp, _c_int, _vp, _vp]),
     "plx_vec_dot": (_c_int, [_vp, _c_int, ctypes.c_int64, _vp, _vp, _dblp]),
     "plx_vec_norm2": (_c_int, [_vp, _c_int, ctypes.c_int64, _vp, _dblp]),
+    "plx_solver_create": (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int]),
+    "plx_solver_destroy": (_c_int, [_vp]),
+    "plx_solver_setup": (_c_int, [_vp]),
+    "plx_solver_set_inner": (_c_int, [_vp, _c_dbl, _c_int, _c_int]),
+    "plx_solver_solve": (_c_int, [_vp, _c_int, _vp, _vp, _c_dbl, _c_int, _vp]),
     "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
     "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
     "plx_dm_reset_event_stats": (_c_int, [_vp]),
 }
+
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("outer_iterations", _c_int), ("inner_iterations", _c_int), ("applies_fp64", _c_int),
+                ("converged", _c_int), ("initial_residual", _c_dbl), ("final_residual", _c_dbl)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
 
 _lib = None
 
@@ -74,4 +93,4 @@ def load():
 # Test hook (include/pylith_amd_debug.h): host emulation of the tile kernels.
 SIGNATURES["plx_debug_emulate_apply"] = (_c_int, [_c_int, _c_int, _c_int, _c_int, _i32p, _i32p,
                                                   _dblp, _i32p, _c_int, _dblp, _c_int, _vp, _vp,
-                                                  _c_int])
+                                                  _c_int, _c_int])

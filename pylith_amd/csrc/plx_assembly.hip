@@ -11,6 +11,8 @@
 //   4. nodes owned by the tile are written straight to the output vector; shared nodes
 //      leave as partial sums in the halo buffer, reduced by a node-ordered second kernel.
 // The result is deterministic: no global atomics, fixed order everywhere.
+#include <algorithm>
+
 #include "plx_element.hpp"
 #include "plx_internal.hpp"
 
@@ -46,9 +48,29 @@ struct ApplyArgs {
     void *r;
     void *halo;
     int mode;
+    double scale;           // multiplies the element vectors (e.g. -1 for the SPD operator -J)
+    const int32_t *tile_list;  // tiles this launch covers (nullptr: all, in order)
+    int num_list;
+    const void *records;       // packed 16-bit local closure (+ cone) per key cell
 };
 
-// Map workgroup i to a tile so that each of the 8 XCDs (round-robin dispatch) walks a
+// Cell solution values read from the tile's LDS buffer (component c, local basis b).
+template <typename Real, int NB>
+struct LdsAcc {
+    const Real *sU;
+    int lc[NB];
+    __device__ __forceinline__ Real operator()(int c, int b) const { return sU[3 * lc[b] + c]; }
+};
+template <typename Real>
+struct LdsAccQ1 {
+    const Real *sU;
+    int lc[8];
+    __device__ __forceinline__ Real operator()(int c, int l) const { return sU[3 * lc[q1_lex(l)] + c]; }
+};
+
+constexpr int kAccColored = 0, kAccAtomic = 1;
+
+// Map slot i to a tile so that each of the 8 XCDs (round-robin workgroup dispatch) walks a
 // contiguous range of tiles: neighbouring tiles then share that XCD's L2.
 __device__ __forceinline__ int xcd_tile(int i, int n) {
     const int per = n >> 3, rem = n & 7;
@@ -56,115 +78,256 @@ __device__ __forceinline__ int xcd_tile(int i, int n) {
     return x * per + min(x, rem) + j;
 }
 
-template <typename Real, int CT, int NTHREADS>
-__global__ void __launch_bounds__(NTHREADS) apply_tiles(ApplyArgs A) {
-    constexpr int NB = CT == PLX_HEX_Q1 ? 8 : 10;
-    constexpr int NV = CT == PLX_HEX_Q1 ? 8 : 4;
-    extern __shared__ __align__(16) unsigned char smem[];
+// Workgroup shape per cell type: NT threads, CPT cells per thread, PFN node slots and PFV
+// vertex slots per thread for the register prefetch of the next tile.
+template <int CT>
+struct Shape;
+template <>
+struct Shape<PLX_HEX_Q1> {
+    // RECW: uint4 words per cell record (8 x uint16 closure)
+    static constexpr int NT = 256, CPT = 2, NB = 8, NV = 8, PFN = 3, PFV = 0, RECW = 1;
+};
+template <>
+struct Shape<PLX_TET_P2> {
+    // record: 10 x uint16 closure, 4 x uint16 cone, 2 padding = 2 uint4 words
+    static constexpr int NT = 384, CPT = 1, NB = 10, NV = 4, PFN = 2, PFV = 1, RECW = 2;
+};
+
+// Entry k of a packed 16-bit cell record.
+template <int W>
+__device__ __forceinline__ int rec_get(const uint4 (&r)[W], int k) {
+    const uint4 &w = r[k >> 3];
+    const int h = (k >> 1) & 3;
+    const unsigned v = h == 0 ? w.x : h == 1 ? w.y : h == 2 ? w.z : w.w;
+    return (k & 1) ? (int)(v >> 16) : (int)(v & 0xffffu);
+}
+
+// One tile's data on its way from HBM: gathered node values and coordinates, the cells'
+// local closures, moduli and colours (registers; the next tile's loads overlap the current
+// tile's arithmetic).
+template <typename Real, int CT>
+struct Stage {
+    using S = Shape<CT>;
+    int tile, c0, nc, n0, nn, v0, nvt, nown;
+    Real pu[S::PFN][3];
+    double px[CT == PLX_HEX_Q1 ? S::PFN : S::PFV][3];
+    uint4 rec[S::CPT][S::RECW];  // packed 16-bit local closure (and cone) of each cell
+    Real lam[S::CPT], mu[S::CPT];
+    int color[S::CPT];
+};
+
+template <typename Real, int CT, int ACC>
+__device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, int tile, int tid) {
+    using S = Shape<CT>;
     const TilePlanDev &P = A.p;
-    const int tile = xcd_tile(blockIdx.x, P.num_tiles);
-    const int c0 = P.tile_cell_ptr[tile], nc = P.tile_cell_ptr[tile + 1] - c0;
-    const int n0 = P.tile_node_ptr[tile], nn = P.tile_node_ptr[tile + 1] - n0;
-    const int v0 = P.tile_vert_ptr[tile], nvt = P.tile_vert_ptr[tile + 1] - v0;
-    const int nown = P.tile_nown[tile];
-    const int tid = threadIdx.x;
-
-    double *sX = reinterpret_cast<double *>(smem);
-    Real *sU = reinterpret_cast<Real *>(sX + 3 * nvt);
-    Real *sR = sU + 3 * nn;
     const Real *u = static_cast<const Real *>(A.u);
+    T.tile = tile;
+    T.c0 = P.tile_cell_ptr[tile];
+    T.nc = P.tile_cell_ptr[tile + 1] - T.c0;
+    T.n0 = P.tile_node_ptr[tile];
+    T.nn = P.tile_node_ptr[tile + 1] - T.n0;
+    T.v0 = P.tile_vert_ptr[tile];
+    T.nvt = P.tile_vert_ptr[tile + 1] - T.v0;
+    T.nown = P.tile_nown[tile];
+#pragma unroll
+    for (int j = 0; j < S::PFN; ++j) {
+        const int k = tid + j * S::NT;
+        if (k < T.nn) {
+            const size_t g = (size_t)P.tile_nodes[T.n0 + k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) T.pu[j][c] = u[3 * g + c];
+            if constexpr (CT == PLX_HEX_Q1) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
+            }
+        }
+    }
+    if constexpr (CT != PLX_HEX_Q1) {
+#pragma unroll
+        for (int j = 0; j < S::PFV; ++j) {
+            const int k = tid + j * S::NT;
+            if (k < T.nvt) {
+                const size_t g = (size_t)P.tile_verts[T.v0 + k];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
+            }
+        }
+    }
+    const Real *mat = static_cast<const Real *>(A.mat);
+#pragma unroll
+    for (int i = 0; i < S::CPT; ++i) {
+        const int cell = tid + i * S::NT;
+        if (cell < T.nc) {
+            const int pos = T.c0 + cell;
+            const uint4 *recs = reinterpret_cast<const uint4 *>(A.records);
+#pragma unroll
+            for (int w = 0; w < S::RECW; ++w) T.rec[i][w] = recs[(size_t)pos * S::RECW + w];
+            T.lam[i] = (Real)A.scale * mat[pos];
+            T.mu[i] = (Real)A.scale * mat[P.stride + pos];
+            if (ACC == kAccColored) T.color[i] = A.color[pos];
+        }
+    }
+}
+
+// Persistent, software-pipelined tile kernel: each workgroup walks its slots; while it works
+// on one tile, the gathers of its next tile are in flight.
+template <typename Real, int CT, bool LINEAR, int ACC, int MINW>
+__global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) {
+    using S = Shape<CT>;
+    constexpr int NT = S::NT, NB = S::NB, NV = S::NV;
+    static_assert(S::PFN * NT >= 768 && S::CPT * NT >= (CT == PLX_HEX_Q1 ? 512 : 384), "tile limits");
+    static_assert(CT == PLX_HEX_Q1 || S::PFV * NT >= 192, "vertex limit");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int G = gridDim.x;
+    int slot = blockIdx.x;
+    if (slot >= A.num_list) return;
+    auto tile_of = [&](int s) {
+        const int t = xcd_tile(s, A.num_list);
+        return A.tile_list ? A.tile_list[t] : t;
+    };
+    Stage<Real, CT> cur;
+    fetch<Real, CT, ACC>(cur, A, tile_of(slot), tid);
     Real *r = static_cast<Real *>(A.r);
-
-    for (int k = tid; k < nn; k += NTHREADS) {
-        const int g = P.tile_nodes[n0 + k];
-        sU[3 * k + 0] = u[3 * (size_t)g + 0];
-        sU[3 * k + 1] = u[3 * (size_t)g + 1];
-        sU[3 * k + 2] = u[3 * (size_t)g + 2];
-        sR[3 * k + 0] = Real(0);
-        sR[3 * k + 1] = Real(0);
-        sR[3 * k + 2] = Real(0);
-    }
-    for (int k = tid; k < nvt; k += NTHREADS) {
-        const int g = P.tile_verts[v0 + k];
-        sX[3 * k + 0] = P.coords[3 * (size_t)g + 0];
-        sX[3 * k + 1] = P.coords[3 * (size_t)g + 1];
-        sX[3 * k + 2] = P.coords[3 * (size_t)g + 2];
-    }
-    __syncthreads();
-
-    const bool active = tid < nc;
-    const int pos = c0 + tid;
-    int lc[NB];
-    Real re[3][NB];
-    int color = 0;
-    if (active) {
-        const Real *mat = static_cast<const Real *>(A.mat);
-        const Real lam = mat[pos], mu = mat[P.stride + pos];
-        color = A.color[pos];
+    Real *halo = static_cast<Real *>(A.halo);
+    const bool insert = A.mode == PLX_INSERT_VALUES;
+    while (true) {
+        // LDS layout of this tile: coordinates, then nodal values, then the accumulator
+        const int nxv = CT == PLX_HEX_Q1 ? cur.nn : cur.nvt;
+        double *sX = reinterpret_cast<double *>(smem);
+        Real *sU = reinterpret_cast<Real *>(sX + 3 * nxv);
+        Real *sR = sU + 3 * cur.nn;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) lc[b] = P.lclosure[(size_t)b * P.stride + pos];
-        if constexpr (CT == PLX_HEX_Q1) {
-            double X[3][8];
-            Real U[3][8];
+        for (int j = 0; j < S::PFN; ++j) {
+            const int k = tid + j * NT;
+            if (k < cur.nn) {
 #pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                const int v = lc[q1_lex(l)];
+                for (int c = 0; c < 3; ++c) {
+                    sU[3 * k + c] = cur.pu[j][c];
+                    sR[3 * k + c] = Real(0);
+                }
+                if constexpr (CT == PLX_HEX_Q1) {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    X[d][l] = sX[3 * v + d];
-                    U[d][l] = sU[3 * v + d];
+                    for (int d = 0; d < 3; ++d) sX[3 * k + d] = cur.px[j][d];
                 }
             }
-            Real rl[3][8];
-            q1_element<Real>(X, U, lam, mu, rl);
+        }
+        if constexpr (CT != PLX_HEX_Q1) {
 #pragma unroll
-            for (int l = 0; l < 8; ++l)
+            for (int j = 0; j < S::PFV; ++j) {
+                const int k = tid + j * NT;
+                if (k < cur.nvt) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) re[c][q1_lex(l)] = rl[c][l];
-        } else {
-            double X[4][3];
-            Real U[3][10];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const int lv = P.lcone[(size_t)v * P.stride + pos];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) X[v][d] = sX[3 * lv + d];
+                    for (int d = 0; d < 3; ++d) sX[3 * k + d] = cur.px[j][d];
+                }
             }
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) U[c][b] = sU[3 * lc[b] + c];
-            p2_element<Real, DevTab>(X, U, lam, mu, re);
         }
-    }
-    // Colour-ordered accumulation: cells of one colour share no node.
-    const int ncolor = A.tile_ncolor[tile];
-    for (int col = 0; col < ncolor; ++col) {
         __syncthreads();
-        if (active && color == col) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) sR[3 * lc[b] + c] += re[c][b];
-        }
-    }
-    __syncthreads();
 
-    const bool insert = A.mode == PLX_INSERT_VALUES;
-    for (int k = tid; k < nown; k += NTHREADS) {
-        const size_t g = (size_t)P.tile_nodes[n0 + k];
+        const int next_slot = slot + G;
+        const bool has_next = next_slot < A.num_list;
+        Stage<Real, CT> nxt;
+        if (has_next) fetch<Real, CT, ACC>(nxt, A, tile_of(next_slot), tid);
+
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const Real v = sR[3 * k + c];
-            r[3 * g + c] = insert ? v : r[3 * g + c] + v;
+        for (int i = 0; i < S::CPT; ++i) {
+            const bool active = tid + i * NT < cur.nc;
+            int lc[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) lc[b] = rec_get(cur.rec[i], b);
+            Real re[3][NB];
+            if (active) {
+                const Real lam = cur.lam[i], mu = cur.mu[i];
+                if constexpr (CT == PLX_HEX_Q1) {
+                    Real rl[3][8];
+                    if constexpr (LINEAR) {
+                        // affine tiles only (the plan routes other tiles to the quadrature-point kernel)
+                        double J[9];
+                        q1_affine_jacobian(LdsAccQ1<double>{sX, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7]}}, J);
+                        LdsAccQ1<Real> acc{sU, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7]}};
+                        q1_element_affine<Real>(J, acc, lam, mu, rl);
+                    } else {
+                        double X[3][8];
+                        Real U[3][8];
+#pragma unroll
+                        for (int l = 0; l < 8; ++l) {
+                            const int v = lc[q1_lex(l)];
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) {
+                                X[d][l] = sX[3 * v + d];
+                                U[d][l] = sU[3 * v + d];
+                            }
+                        }
+                        q1_element<Real>(X, U, lam, mu, rl);
+                    }
+#pragma unroll
+                    for (int l = 0; l < 8; ++l)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) re[c][q1_lex(l)] = rl[c][l];
+                } else {
+                    double X[4][3];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const int lv = rec_get(cur.rec[i], NB + v);
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) X[v][d] = sX[3 * lv + d];
+                    }
+                    if constexpr (LINEAR) {
+                        LdsAcc<Real, 10> acc{sU, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7], lc[8], lc[9]}};
+                        p2_element_linear<Real>(X, acc, lam, mu, c_p2.wdiag, c_p2.woff, re);
+                    } else {
+                        Real U[3][10];
+#pragma unroll
+                        for (int b = 0; b < NB; ++b)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) U[c][b] = sU[3 * lc[b] + c];
+                        p2_element<Real, DevTab>(X, U, lam, mu, re);
+                    }
+                }
+            }
+            if constexpr (ACC == kAccAtomic) {
+                // LDS atomics: no colour phases; the order of the additions into a node varies
+                if (active) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) atomicAdd(&sR[3 * lc[b] + c], re[c][b]);
+                }
+            } else {
+                // cells of one colour share no node: fixed summation order (cells i = 0 of the
+                // workgroup colour by colour, then cells i = 1), reproducible bits
+                const int ncolor = A.tile_ncolor[cur.tile];
+                for (int col = 0; col < ncolor; ++col) {
+                    __syncthreads();
+                    if (active && cur.color[i] == col) {
+#pragma unroll
+                        for (int b = 0; b < NB; ++b)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) sR[3 * lc[b] + c] += re[c][b];
+                    }
+                }
+            }
         }
-    }
-    Real *halo = static_cast<Real *>(A.halo);
-    const size_t hoff = (size_t)P.tile_halo_off[tile];
-    for (int k = nown + tid; k < nn; k += NTHREADS) {
-        const size_t s = hoff + (size_t)(k - nown);
+        __syncthreads();
+
+        for (int k = tid; k < cur.nown; k += NT) {
+            const size_t g = (size_t)A.p.tile_nodes[cur.n0 + k];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) halo[3 * s + c] = sR[3 * k + c];
+            for (int c = 0; c < 3; ++c) {
+                const Real v = sR[3 * k + c];
+                r[3 * g + c] = insert ? v : r[3 * g + c] + v;
+            }
+        }
+        const size_t hoff = (size_t)A.p.tile_halo_off[cur.tile];
+        for (int k = cur.nown + tid; k < cur.nn; k += NT) {
+            const size_t s = hoff + (size_t)(k - cur.nown);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) halo[3 * s + c] = sR[3 * k + c];
+        }
+        if (!has_next) break;
+        __syncthreads();  // every read of this tile's LDS precedes the next tile's staging
+        cur = nxt;
+        slot = next_slot;
     }
 }
 
@@ -213,8 +376,9 @@ static TilePlanDev plan_dev(plx_dm dm, const KeyPlan *kp) {
 }
 
 template <typename Real>
-static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode) {
+static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode, double scale) {
     ApplyArgs A;
+    A.scale = scale;
     A.p = plan_dev(dm, kp);
     A.color = kp->d_color;
     A.tile_ncolor = kp->d_tile_ncolor;
@@ -223,16 +387,58 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode)
     A.r = r;
     A.halo = kp->d_halo;
     A.mode = mode;
+    A.records = kp->d_records;
     const size_t lds = (size_t)kp->max_verts * 3 * sizeof(double) + (size_t)kp->max_nodes * 6 * sizeof(Real);
+    const bool linear = dm->algorithm != PLX_ALGO_QUADRATURE_POINTS;
+    // AUTO: LDS atomics for FP64 (fast native ds_add_f64), colour phases for FP32
+    const bool atomic = dm->accumulate == PLX_ACCUMULATE_LDS_ATOMIC ||
+                        (dm->accumulate == PLX_ACCUMULATE_AUTO && std::is_same<Real, double>::value);
+    // persistent grid: WG_PER_CU workgroups per CU (a multiple of 8 keeps each workgroup on
+    // the XCD whose tile range it walks), never more than there are tiles
+    int ncu = 0;
+    PLX_HIP(dm, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
+    const int resident = ((std::max(ncu, 8) * dm->wg_per_cu + 7) / 8) * 8;
     {
     ScopedEvent ev(dm, "plx:apply_tiles");
+#define PLX_LAUNCH_W(CT, LIN, ACC, W, N)                                                             \
+    hipLaunchKernelGGL((apply_tiles<Real, CT, LIN, ACC, W>), dim3(std::min(N, resident)),              \
+                       dim3(Shape<CT>::NT), lds, dm->stream, A)
+#define PLX_LAUNCH(CT, LIN, ACC, LIST, N)                                                            \
+    do {                                                                                             \
+        A.tile_list = LIST;                                                                          \
+        A.num_list = N;                                                                              \
+        if (N > 0) {                                                                                 \
+            if (minw <= 2) PLX_LAUNCH_W(CT, LIN, ACC, 2, N);                                           \
+            else if (minw == 3) PLX_LAUNCH_W(CT, LIN, ACC, 3, N);                                      \
+            else PLX_LAUNCH_W(CT, LIN, ACC, 4, N);                                                     \
+        }                                                                                            \
+    } while (0)
+    const int minw = dm->min_waves;
     if (dm->cell_type == PLX_HEX_Q1) {
-        hipLaunchKernelGGL((apply_tiles<Real, PLX_HEX_Q1, 512>), dim3(kp->num_tiles), dim3(512), lds,
-                           dm->stream, A);
+        if (linear) {
+            // affine tiles: modal kernel; the rest: quadrature-point kernel
+            if (atomic) {
+                PLX_LAUNCH(PLX_HEX_Q1, true, kAccAtomic, kp->d_tiles_affine, kp->num_tiles_affine);
+                PLX_LAUNCH(PLX_HEX_Q1, false, kAccAtomic, kp->d_tiles_general, kp->num_tiles_general);
+            } else {
+                PLX_LAUNCH(PLX_HEX_Q1, true, kAccColored, kp->d_tiles_affine, kp->num_tiles_affine);
+                PLX_LAUNCH(PLX_HEX_Q1, false, kAccColored, kp->d_tiles_general, kp->num_tiles_general);
+            }
+        } else {
+            if (atomic) PLX_LAUNCH(PLX_HEX_Q1, false, kAccAtomic, nullptr, kp->num_tiles);
+            else PLX_LAUNCH(PLX_HEX_Q1, false, kAccColored, nullptr, kp->num_tiles);
+        }
     } else {
-        hipLaunchKernelGGL((apply_tiles<Real, PLX_TET_P2, 384>), dim3(kp->num_tiles), dim3(384), lds,
-                           dm->stream, A);
+        if (linear) {
+            if (atomic) PLX_LAUNCH(PLX_TET_P2, true, kAccAtomic, nullptr, kp->num_tiles);
+            else PLX_LAUNCH(PLX_TET_P2, true, kAccColored, nullptr, kp->num_tiles);
+        } else {
+            if (atomic) PLX_LAUNCH(PLX_TET_P2, false, kAccAtomic, nullptr, kp->num_tiles);
+            else PLX_LAUNCH(PLX_TET_P2, false, kAccColored, nullptr, kp->num_tiles);
+        }
     }
+#undef PLX_LAUNCH_W
+#undef PLX_LAUNCH
     }
     PLX_HIP(dm, hipGetLastError());
     if (kp->num_shared > 0) {
@@ -247,13 +453,13 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode)
 }
 
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u, void *r,
-                 int mode) {
+                 int mode, double scale) {
     (void)jacobian;  // linear isotropic elasticity: J v uses the same moduli as the residual
     int err = ensure_tables(dm);
     if (err) return err;
     if (mode != PLX_ADD_VALUES && mode != PLX_INSERT_VALUES) return set_error(dm, 1, "bad insert mode");
-    if (scalar_type == PLX_FP64) return apply_typed<double>(dm, kp, u, r, mode);
-    if (scalar_type == PLX_FP32) return apply_typed<float>(dm, kp, u, r, mode);
+    if (scalar_type == PLX_FP64) return apply_typed<double>(dm, kp, u, r, mode, scale);
+    if (scalar_type == PLX_FP32) return apply_typed<float>(dm, kp, u, r, mode, scale);
     return set_error(dm, 1, "bad scalar type");
 }
 

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "plx_element.hpp"
 #include "plx_internal.hpp"
 
 namespace plx {
@@ -32,15 +33,34 @@ static int upload(plx_dm dm, T **dst, const T *src, size_t n) {
     return 0;
 }
 
-static void free_key(KeyPlan *k) {
+static void free_plan_arrays(KeyPlan *k) {
     void *ptrs[] = {k->d_cells,       k->d_tile_cell_ptr, k->d_tile_node_ptr, k->d_tile_nown,
                     k->d_tile_nodes,  k->d_tile_vert_ptr, k->d_tile_verts,    k->d_tile_halo_off,
                     k->d_lclosure,    k->d_lcone,         k->d_shared_node,   k->d_color,
-                    k->d_tile_ncolor,   k->d_shared_ptr,
+                    k->d_tile_ncolor,   k->d_records,       k->d_shared_ptr,    k->d_tiles_affine,  k->d_tiles_general,
                     k->d_shared_slot, k->d_halo,          k->d_mat64,         k->d_mat32,
                     k->d_state};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    KeyPlan fresh;
+    fresh.label_value = k->label_value;
+    fresh.part = k->part;
+    fresh.f0 = k->f0;
+    fresh.f1 = k->f1;
+    fresh.jf0 = k->jf0;
+    fresh.jf1 = k->jf1;
+    fresh.jf2 = k->jf2;
+    fresh.jf3 = k->jf3;
+    fresh.num_aux = k->num_aux;
+    fresh.aux_sizes = std::move(k->aux_sizes);
+    fresh.aux_per_qp = k->aux_per_qp;
+    fresh.aux_host = std::move(k->aux_host);
+    fresh.constants = std::move(k->constants);
+    *k = std::move(fresh);
+}
+
+static void free_key(KeyPlan *k) {
+    free_plan_arrays(k);
     delete k;
 }
 
@@ -51,7 +71,10 @@ static int build_plan(plx_dm dm, KeyPlan *kp) {
                 dm->cell_type == PLX_HEX_Q1};
     HostPlan hp;
     std::string msg;
-    if (build_host_plan(mv, kp->label_value, tile_limits(dm->cell_type), &hp, &msg))
+    // P2 tiles keep colour order (the tets of a cube share nodes: colour order keeps a wave's
+    // atomic targets distinct); Q1 tiles keep mesh order (structured LDS access).
+    kp->sorted_by_color = dm->cell_type == PLX_TET_P2;
+    if (build_host_plan(mv, kp->label_value, tile_limits(dm->cell_type), &hp, &msg, kp->sorted_by_color))
         return set_error(dm, 1, msg);
     kp->cells = hp.cells;
     kp->num_tiles = hp.num_tiles;
@@ -72,11 +95,86 @@ static int build_plan(plx_dm dm, KeyPlan *kp) {
     if ((err = upload(dm, &kp->d_lclosure, hp.lclosure.data(), hp.lclosure.size()))) return err;
     if ((err = upload(dm, &kp->d_lcone, hp.lcone.data(), hp.lcone.size()))) return err;
     if ((err = upload(dm, &kp->d_color, hp.color.data(), hp.color.size()))) return err;
+    {
+        // packed per-cell records: Q1 8 x uint16 closure; P2 10 closure + 4 cone + 2 pad
+        const int recw = dm->cell_type == PLX_HEX_Q1 ? 8 : 16;
+        const size_t n = hp.cells.size();
+        std::vector<uint16_t> rec(n * recw, 0);
+        for (size_t i = 0; i < n; ++i) {
+            for (int b = 0; b < dm->nb; ++b) rec[i * recw + b] = hp.lclosure[(size_t)b * n + i];
+            if (dm->cell_type != PLX_HEX_Q1)
+                for (int v = 0; v < dm->nv; ++v) rec[i * recw + dm->nb + v] = hp.lcone[(size_t)v * n + i];
+        }
+        if ((err = upload(dm, &kp->d_records, rec.data(), rec.size()))) return err;
+    }
     if ((err = upload(dm, &kp->d_tile_ncolor, hp.tile_ncolor.data(), hp.tile_ncolor.size()))) return err;
     if ((err = upload(dm, &kp->d_shared_node, hp.shared_node.data(), hp.shared_node.size()))) return err;
     if ((err = upload(dm, &kp->d_shared_ptr, hp.shared_ptr.data(), hp.shared_ptr.size()))) return err;
     if ((err = upload(dm, &kp->d_shared_slot, hp.shared_slot.data(), hp.shared_slot.size()))) return err;
     if (hp.halo_slots > 0) PLX_HIP(dm, hipMalloc(&kp->d_halo, (size_t)hp.halo_slots * 3 * sizeof(double)));
+    // Q1 tiles whose cells are all parallelepipeds take the modal kernel (geometry is static).
+    if (dm->cell_type == PLX_HEX_Q1) {
+        std::vector<int32_t> aff, gen;
+        for (int t = 0; t < hp.num_tiles; ++t) {
+            bool all = true;
+            for (int i = hp.tile_cell_ptr[t]; i < hp.tile_cell_ptr[t + 1] && all; ++i) {
+                double X[3][8], TX[3][8];
+                const int32_t *cn = &dm->cone[(size_t)hp.cells[i] * 8];
+                for (int l = 0; l < 8; ++l)
+                    for (int d = 0; d < 3; ++d) X[d][l] = dm->coords[3 * (size_t)cn[q1_lex(l)] + d];
+                for (int d = 0; d < 3; ++d) wht8<double>(X[d], TX[d]);
+                all = q1_affine(TX);
+            }
+            (all ? aff : gen).push_back(t);
+        }
+        kp->num_tiles_affine = (int)aff.size();
+        kp->num_tiles_general = (int)gen.size();
+        if ((err = upload(dm, &kp->d_tiles_affine, aff.data(), aff.size()))) return err;
+        if ((err = upload(dm, &kp->d_tiles_general, gen.data(), gen.size()))) return err;
+    }
+    return 0;
+}
+
+// Moduli per key cell in tile order (SoA: lambda then mu), FP64 and FP32, from the captured aux.
+// Elastic moduli are the last two subfields (IsotropicLinearElasticity::setContext,
+// IsotropicLinearElasticity.hh:116-117) or, for generalized Maxwell, subfields numA-6 and
+// numA-5 (IsotropicLinearGenMaxwell.hh:133-134); per-qp aux must be uniform in a cell there.
+static int repack_aux(plx_dm dm, KeyPlan *kp) {
+    const int num_aux = kp->num_aux;
+    std::vector<int> off(num_aux);
+    int naux_tot = 0;
+    for (int i = 0; i < num_aux; ++i) {
+        off[i] = naux_tot;
+        naux_tot += kp->aux_sizes[i];
+    }
+    const int per = kp->aux_per_qp ? dm->nq : 1;
+    const int n = (int)kp->cells.size();
+    const bool genmax = kp->f1 == PLX_F1V_GENMAXWELL_3D || kp->jf3 == PLX_JF3VU_GENMAXWELL_3D;
+    if (genmax && num_aux < 7) return set_error(dm, 1, "generalized Maxwell needs 7 aux subfields");
+    const int i_mu = genmax ? num_aux - 6 : num_aux - 2;
+    const int i_k = genmax ? num_aux - 5 : num_aux - 1;
+    std::vector<double> m64(2 * (size_t)n);
+    std::vector<float> m32(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const double *a = kp->aux_host.data() + (size_t)kp->cells[i] * per * naux_tot;
+        const double mu = a[off[i_mu]], K = a[off[i_k]];
+        if (!(mu > 0.0) || !(K > 0.0)) return set_error(dm, 1, "non-positive shear or bulk modulus");
+        for (int q = 1; q < per; ++q)
+            if (a[q * naux_tot + off[i_mu]] != mu || a[q * naux_tot + off[i_k]] != K)
+                return set_error(dm, 1, "moduli must be uniform within a cell");
+        m64[i] = K - 2.0 / 3.0 * mu;  // lambda
+        m64[n + i] = mu;
+        m32[i] = (float)m64[i];
+        m32[n + i] = (float)mu;
+    }
+    if (kp->d_mat64) (void)hipFree(kp->d_mat64);
+    if (kp->d_mat32) (void)hipFree(kp->d_mat32);
+    kp->d_mat64 = nullptr;
+    kp->d_mat32 = nullptr;
+    int err = 0;
+    if ((err = upload(dm, &kp->d_mat64, m64.data(), m64.size()))) return err;
+    if ((err = upload(dm, &kp->d_mat32, m32.data(), m32.size()))) return err;
+    kp->has_aux = true;
     return 0;
 }
 
@@ -192,6 +290,21 @@ int plx_dm_set_stream(plx_dm dm, void *stream) {
 
 const char *plx_dm_last_error(plx_dm dm) { return dm ? dm->last_error.c_str() : "null handle"; }
 
+int plx_dm_set_kernel_options(plx_dm dm, int algorithm, int accumulate, int min_waves_per_simd) {
+    if (!dm) return 1;
+    if (algorithm != PLX_ALGO_AUTO && algorithm != PLX_ALGO_QUADRATURE_POINTS &&
+        algorithm != PLX_ALGO_FACTORED)
+        return set_error(dm, 1, "unknown algorithm");
+    if (accumulate != PLX_ACCUMULATE_AUTO && accumulate != PLX_ACCUMULATE_COLORED &&
+        accumulate != PLX_ACCUMULATE_LDS_ATOMIC)
+        return set_error(dm, 1, "unknown accumulation mode");
+    if (min_waves_per_simd < 0 || min_waves_per_simd > 4) return set_error(dm, 1, "min waves must be 0..4");
+    dm->algorithm = algorithm;
+    dm->accumulate = accumulate;
+    dm->min_waves = min_waves_per_simd <= 2 ? 2 : min_waves_per_simd;
+    return 0;
+}
+
 int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size) {
     if (!dm) return 1;
     if (local_size) *local_size = dm->local_size;
@@ -263,47 +376,16 @@ int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux, cons
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp) return set_error(dm, 1, "unknown key");
     int naux_tot = 0;
-    std::vector<int> off(num_aux);
     for (int i = 0; i < num_aux; ++i) {
         if (aux_sizes[i] <= 0) return set_error(dm, 1, "bad aux subfield size");
-        off[i] = naux_tot;
         naux_tot += aux_sizes[i];
     }
     const int per = aux_per_qp ? dm->nq : 1;
-    const int n = (int)kp->cells.size();
-    // Elastic moduli: the last two subfields (IsotropicLinearElasticity::setContext,
-    // IsotropicLinearElasticity.hh:116-117) or, for generalized Maxwell, subfields numA-6 and
-    // numA-5 (IsotropicLinearGenMaxwell.hh:133-134).  Per-cell values; per-qp aux must be
-    // uniform within a cell for these subfields.
-    const bool genmax = kp->f1 == PLX_F1V_GENMAXWELL_3D || kp->jf3 == PLX_JF3VU_GENMAXWELL_3D;
-    if (genmax && num_aux < 7) return set_error(dm, 1, "generalized Maxwell needs 7 aux subfields");
-    const int i_mu = genmax ? num_aux - 6 : num_aux - 2;
-    const int i_k = genmax ? num_aux - 5 : num_aux - 1;
-    std::vector<double> m64(2 * (size_t)n);
-    std::vector<float> m32(2 * (size_t)n);
-    for (int i = 0; i < n; ++i) {
-        const double *a = aux + (size_t)kp->cells[i] * per * naux_tot;
-        const double mu = a[off[i_mu]], K = a[off[i_k]];
-        if (!(mu > 0.0) || !(K > 0.0)) return set_error(dm, 1, "non-positive shear or bulk modulus");
-        for (int q = 1; q < per; ++q)
-            if (a[q * naux_tot + off[i_mu]] != mu || a[q * naux_tot + off[i_k]] != K)
-                return set_error(dm, 1, "moduli must be uniform within a cell");
-        m64[i] = K - 2.0 / 3.0 * mu;  // lambda
-        m64[n + i] = mu;
-        m32[i] = (float)m64[i];
-        m32[n + i] = (float)mu;
-    }
-    if (kp->d_mat64) (void)hipFree(kp->d_mat64);
-    if (kp->d_mat32) (void)hipFree(kp->d_mat32);
-    int err = 0;
-    if ((err = upload(dm, &kp->d_mat64, m64.data(), m64.size()))) return err;
-    if ((err = upload(dm, &kp->d_mat32, m32.data(), m32.size()))) return err;
     kp->num_aux = num_aux;
     kp->aux_sizes.assign(aux_sizes, aux_sizes + num_aux);
     kp->aux_per_qp = aux_per_qp;
     kp->aux_host.assign(aux, aux + (size_t)dm->num_cells * per * naux_tot);
-    kp->has_aux = true;
-    return 0;
+    return repack_aux(dm, kp);
 }
 
 int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, double t,

@@ -11,6 +11,8 @@ constexpr int kP2Quad = 14;
 struct P2Tables {
     double w[kP2Quad];
     double dphi[kP2Quad][10][3];
+    double lam[kP2Quad][4];  // barycentric coordinates of the points
+    double wdiag, woff;      // P1 moments of the rule: sum_q w_q l_c l_d = wdiag (c == d), woff (c != d)
 };
 
 // Reference tables of the P2 tetrahedron at the 14-point degree-5 rule (two S31 orbits and
@@ -35,6 +37,12 @@ inline void p2_fill_tables(P2Tables *t) {
         t->w[q] = W3;
     }
     const double dl[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    t->wdiag = t->woff = 0.0;
+    for (q = 0; q < kP2Quad; ++q) {
+        for (int v = 0; v < 4; ++v) t->lam[q][v] = lam[q][v];
+        t->wdiag += t->w[q] * lam[q][0] * lam[q][0];
+        t->woff += t->w[q] * lam[q][0] * lam[q][1];
+    }
     for (q = 0; q < kP2Quad; ++q) {
         for (int v = 0; v < 4; ++v)
             for (int d = 0; d < 3; ++d) t->dphi[q][v][d] = (4.0 * lam[q][v] - 1.0) * dl[v][d];
@@ -276,6 +284,286 @@ __host__ __device__ inline void p2_element(const double X[4][3], const Real U[3]
                 re[c][b] += P[3 * c + 0] * (Real)Tab::dphi(q, b, 0) +
                             P[3 * c + 1] * (Real)Tab::dphi(q, b, 1) +
                             P[3 * c + 2] * (Real)Tab::dphi(q, b, 2);
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// Factorized element paths for pointwise kernels that are linear in the solution gradient with
+// a per-cell constant coefficient (isotropic linear elasticity: f1 = -C : grad u and its Jf3),
+// on affine cells.  With P(q) = L(G(q)) linear, the quadrature sum
+//     r_i = sum_q w_q P(q) : dphi_i(q)
+// is evaluated exactly through the structure of the basis: the pointwise map is applied to the
+// polynomial components of the gradient instead of to its values at the points.  The result is
+// the same quadrature sum (identical up to rounding), at a fraction of the arithmetic.
+// --------------------------------------------------------------------------------------------
+
+// Solution values of one cell read on demand (component c, local basis b).
+template <typename Real>
+struct ArrayAcc3 {
+    const Real (*U)[10];
+    __host__ __device__ Real operator()(int c, int b) const { return U[c][b]; }
+};
+template <typename Real>
+struct ArrayAcc3Q1 {
+    const Real (*U)[8];
+    __host__ __device__ Real operator()(int c, int b) const { return U[c][b]; }
+};
+struct ArrayAccX {
+    const double (*X)[8];
+    __host__ __device__ double operator()(int d, int l) const { return X[d][l]; }
+};
+
+// Walsh-Hadamard transform of 8 lexicographic corner values: T[b] for b = bxi + 2 beta + 4 bzeta,
+// T[b] = sum_l F[l] prod(sign of the odd coordinates of corner l).
+template <typename Real>
+__host__ __device__ inline void wht8(const Real F[8], Real T[8]) {
+    Real a[8];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {  // xi
+        a[2 * p] = F[2 * p] + F[2 * p + 1];
+        a[2 * p + 1] = F[2 * p + 1] - F[2 * p];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)  // eta
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            T[i + 4 * k] = a[i + 4 * k] + a[i + 2 + 4 * k];
+            T[i + 2 + 4 * k] = a[i + 2 + 4 * k] - a[i + 4 * k];
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // zeta
+        a[i] = T[i] + T[i + 4];
+        a[i + 4] = T[i + 4] - T[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) T[i] = a[i];
+}
+
+// Inverse structure: nodal values r[l] = sum_{b != 0} R[b] prod(signs of corner l in b).
+template <typename Real>
+__host__ __device__ inline void iwht8(const Real R[8], Real r[8]) {
+    Real a[8], t[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // zeta
+        a[i] = R[i] - R[i + 4];
+        a[i + 4] = R[i] + R[i + 4];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)  // eta
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            t[i + 4 * k] = a[i + 4 * k] - a[i + 2 + 4 * k];
+            t[i + 2 + 4 * k] = a[i + 4 * k] + a[i + 2 + 4 * k];
+        }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {  // xi
+        r[2 * p] = t[2 * p] - t[2 * p + 1];
+        r[2 * p + 1] = t[2 * p] + t[2 * p + 1];
+    }
+}
+
+// Affine test of a Q1 cell from the coordinate transform: the bilinear and trilinear modes of
+// the corner coordinates vanish (relative to the cell size).
+__host__ __device__ inline bool q1_affine(const double TX[3][8]) {
+    double size = 0.0, higher = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        size = fmax(size, fmax(fabs(TX[d][1]), fmax(fabs(TX[d][2]), fabs(TX[d][4]))));
+        higher = fmax(higher, fmax(fmax(fabs(TX[d][3]), fabs(TX[d][5])), fmax(fabs(TX[d][6]), fabs(TX[d][7]))));
+    }
+    return higher <= 1e-13 * size;
+}
+
+// Q1 affine element: modal form of the 2x2x2 Gauss sum (Gauss orthogonality of the modes:
+// sum_q w_q m(q) m'(q) = 8 delta nu_m, nu = 1, 1/3, 1/9 for constant, linear, bilinear modes).
+// re[c][l] in lexicographic corner order.  TX: WHT of the corner coordinates (affine cell).
+// Constant Jacobian of an affine Q1 cell (a parallelepiped: every mode above the linear ones
+// vanishes) from the edge vectors at lexicographic corner 0: J[:, k] = (x(e_k) - x(0)) / 2 with
+// e = 1 (xi), 2 (eta), 4 (zeta).  The plan routes only cells that pass q1_affine here.
+template <typename XAcc>
+__host__ __device__ inline void q1_affine_jacobian(const XAcc &X, double J[9]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double x0 = X(d, 0);
+        J[3 * d + 0] = 0.5 * (X(d, 1) - x0);
+        J[3 * d + 1] = 0.5 * (X(d, 2) - x0);
+        J[3 * d + 2] = 0.5 * (X(d, 4) - x0);
+    }
+}
+
+template <typename Real, typename UAcc>
+__host__ __device__ inline void q1_element_affine(const double J[9], const UAcc &U, Real lam, Real mu,
+                                                  Real re[3][8]) {
+    double Jid[9];
+    const double det = inv3(J, Jid);
+    Real Ji[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ji[i] = (Real)Jid[i];
+    // modal coefficients of u times 8: T[c][b]
+    Real T[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Real u[8];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) u[l] = U(c, l);
+        wht8<Real>(u, T[c]);
+    }
+    // Gradient modes (times 8), then fluxes scaled by det * nu_m / 8:
+    //   const: [T1, T2, T4];  xi: [0, T3, T5];  eta: [T3, 0, T6];  zeta: [T5, T6, 0];
+    //   eta zeta: [T7, 0, 0]; xi zeta: [0, T7, 0]; xi eta: [0, 0, T7]  (T3 = xi eta, T5 = xi zeta, T6 = eta zeta)
+    Real R[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) R[c][b] = Real(0);
+    const Real s0 = (Real)(0.125 * det), s1 = (Real)(0.125 * det / 3.0), s2 = (Real)(0.125 * det / 9.0);
+    Real G[9], P[9];
+    // constant mode -> R_xi (b=1), R_eta (b=2), R_zeta (b=4): column k of P
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        G[3 * c + 0] = T[c][1];
+        G[3 * c + 1] = T[c][2];
+        G[3 * c + 2] = T[c][4];
+    }
+    iso_flux<Real>(G, Ji, lam, mu, s0, P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        R[c][1] += P[3 * c + 0];
+        R[c][2] += P[3 * c + 1];
+        R[c][4] += P[3 * c + 2];
+    }
+    // xi mode: P_xi, columns eta -> R(xi eta) b=3, zeta -> R(xi zeta) b=5
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        G[3 * c + 0] = Real(0);
+        G[3 * c + 1] = T[c][3];
+        G[3 * c + 2] = T[c][5];
+    }
+    iso_flux<Real>(G, Ji, lam, mu, s1, P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        R[c][3] += P[3 * c + 1];
+        R[c][5] += P[3 * c + 2];
+    }
+    // eta mode: columns xi -> b=3, zeta -> R(eta zeta) b=6
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        G[3 * c + 0] = T[c][3];
+        G[3 * c + 1] = Real(0);
+        G[3 * c + 2] = T[c][6];
+    }
+    iso_flux<Real>(G, Ji, lam, mu, s1, P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        R[c][3] += P[3 * c + 0];
+        R[c][6] += P[3 * c + 2];
+    }
+    // zeta mode: columns xi -> b=5, eta -> b=6
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        G[3 * c + 0] = T[c][5];
+        G[3 * c + 1] = T[c][6];
+        G[3 * c + 2] = Real(0);
+    }
+    iso_flux<Real>(G, Ji, lam, mu, s1, P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        R[c][5] += P[3 * c + 0];
+        R[c][6] += P[3 * c + 1];
+    }
+    // bilinear modes -> R(xi eta zeta) b=7: eta zeta (column xi), xi zeta (column eta), xi eta (column zeta)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G[3 * c + 0] = k == 0 ? T[c][7] : Real(0);
+            G[3 * c + 1] = k == 1 ? T[c][7] : Real(0);
+            G[3 * c + 2] = k == 2 ? T[c][7] : Real(0);
+        }
+        iso_flux<Real>(G, Ji, lam, mu, s2, P);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[c][7] += P[3 * c + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) iwht8<Real>(R[c], re[c]);
+}
+
+// P2 affine element, factorized through the P1 structure of the gradient: G(q) = sum_c l_c(q) G_c
+// with G_c the reference gradient at vertex c, so P(q) = sum_c l_c(q) P_c and
+//   r_i = sum_d M_d : dphi_i(vertex d),  M_d = sum_c W_dc P_c,  W_dc = sum_q w_q l_d(q) l_c(q)
+// (W from the 14-point rule: wdiag on the diagonal, woff off it).
+template <typename Real, typename UAcc>
+__host__ __device__ inline void p2_element_linear(const double X[4][3], const UAcc &U, Real lam,
+                                                  Real mu, double wdiag, double woff, Real re[3][10]) {
+    double J[9], Jid[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) J[3 * d + k] = X[k + 1][d] - X[0][d];
+    const double det = inv3(J, Jid);
+    Real Ji[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ji[i] = (Real)Jid[i];
+    // vertex reference gradients, one solution component at a time: S = (u1-u0, u2-u0, u3-u0),
+    // G_v = 4 d_v - S with d_0 = (u01-u0, u02-u0, u03-u0), d_1 = (u1-u01, u12-u01, u13-u01),
+    // d_2 = (u12-u02, u2-u02, u23-u02), d_3 = (u13-u03, u23-u03, u3-u03)
+    Real P[4][9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Real u[10];
+#pragma unroll
+        for (int b = 0; b < 10; ++b) u[b] = U(c, b);
+        const Real S0 = u[1] - u[0], S1 = u[2] - u[0], S2 = u[3] - u[0];
+        P[0][3 * c + 0] = Real(4) * (u[4] - u[0]) - S0;
+        P[0][3 * c + 1] = Real(4) * (u[5] - u[0]) - S1;
+        P[0][3 * c + 2] = Real(4) * (u[6] - u[0]) - S2;
+        P[1][3 * c + 0] = Real(4) * (u[1] - u[4]) - S0;
+        P[1][3 * c + 1] = Real(4) * (u[7] - u[4]) - S1;
+        P[1][3 * c + 2] = Real(4) * (u[8] - u[4]) - S2;
+        P[2][3 * c + 0] = Real(4) * (u[7] - u[5]) - S0;
+        P[2][3 * c + 1] = Real(4) * (u[2] - u[5]) - S1;
+        P[2][3 * c + 2] = Real(4) * (u[9] - u[5]) - S2;
+        P[3][3 * c + 0] = Real(4) * (u[8] - u[6]) - S0;
+        P[3][3 * c + 1] = Real(4) * (u[9] - u[6]) - S1;
+        P[3][3 * c + 2] = Real(4) * (u[3] - u[6]) - S2;
+    }
+    // vertex fluxes, in place
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        Real G[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) G[i] = P[v][i];
+        iso_flux<Real>(G, Ji, lam, mu, (Real)det, P[v]);
+    }
+    // M_d = wdiag' P_d + woff SP with wdiag' = wdiag - woff; sum_d M_d = (wdiag' + 4 woff) SP
+    Real SP[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) SP[i] = P[0][i] + P[1][i] + P[2][i] + P[3][i];
+    const Real a = (Real)(wdiag - woff), b = (Real)woff;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[v][i] = a * P[v][i] + b * SP[i];
+    const Real sa = a + Real(4) * b;
+    // contraction with e_k: k >= 1 -> column k-1; k = 0 -> minus the row sum
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Real Me0[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Me0[v] = -(P[v][3 * c] + P[v][3 * c + 1] + P[v][3 * c + 2]);
+        const Real SMe0 = -sa * (SP[3 * c] + SP[3 * c + 1] + SP[3 * c + 2]);
+        // vertices: r_a = (4 M_a - sum M) . e_a
+        re[c][0] = Real(4) * Me0[0] - SMe0;
+        re[c][1] = Real(4) * P[1][3 * c + 0] - sa * SP[3 * c + 0];
+        re[c][2] = Real(4) * P[2][3 * c + 1] - sa * SP[3 * c + 1];
+        re[c][3] = Real(4) * P[3][3 * c + 2] - sa * SP[3 * c + 2];
+        // edges: r_ab = 4 (M_a . e_b + M_b . e_a)
+        re[c][4] = Real(4) * (P[0][3 * c + 0] + Me0[1]);          // (0,1)
+        re[c][5] = Real(4) * (P[0][3 * c + 1] + Me0[2]);          // (0,2)
+        re[c][6] = Real(4) * (P[0][3 * c + 2] + Me0[3]);          // (0,3)
+        re[c][7] = Real(4) * (P[1][3 * c + 1] + P[2][3 * c + 0]); // (1,2)
+        re[c][8] = Real(4) * (P[1][3 * c + 2] + P[3][3 * c + 0]); // (1,3)
+        re[c][9] = Real(4) * (P[2][3 * c + 2] + P[3][3 * c + 1]); // (2,3)
     }
 }
 

@@ -38,7 +38,7 @@ int xcd_tile(int i, int n) {
 
 template <typename Real>
 int emulate(const plx::MeshView &mv, const double *coords, const plx::HostPlan &hp,
-            const double *lam_mu, const Real *u, Real *r, int mode, int cell_type) {
+            const double *lam_mu, const Real *u, Real *r, int mode, int cell_type, bool linear) {
     const int nb = mv.nb, nv = mv.nv;
     const int stride = (int)hp.cells.size();
     const plx::TileLimits lim = plx::tile_limits(cell_type);
@@ -88,7 +88,18 @@ int emulate(const plx::MeshView &mv, const double *coords, const plx::HostPlan &
                         U[d][l] = sU[3 * v + d];
                     }
                 }
-                plx::q1_element<Real>(X, U, lam, mu, rl);
+                bool done = false;
+                if (linear) {
+                    double TX[3][8];
+                    for (int d = 0; d < 3; ++d) plx::wht8<double>(X[d], TX[d]);
+                    if (plx::q1_affine(TX)) {
+                        double J[9];
+                        plx::q1_affine_jacobian(plx::ArrayAccX{X}, J);
+                        plx::q1_element_affine<Real>(J, plx::ArrayAcc3Q1<Real>{U}, lam, mu, rl);
+                        done = true;
+                    }
+                }
+                if (!done) plx::q1_element<Real>(X, U, lam, mu, rl);
                 for (int l = 0; l < 8; ++l)
                     for (int c = 0; c < 3; ++c) out[c * nb + plx::q1_lex(l)] = rl[c][l];
             } else {
@@ -101,7 +112,10 @@ int emulate(const plx::MeshView &mv, const double *coords, const plx::HostPlan &
                 }
                 for (int b = 0; b < nb; ++b)
                     for (int c = 0; c < 3; ++c) U[c][b] = sU[3 * lc[b] + c];
-                plx::p2_element<Real, HostTab>(X, U, lam, mu, rl);
+                if (linear)
+                    plx::p2_element_linear<Real>(X, plx::ArrayAcc3<Real>{U}, lam, mu, HostTab::t().wdiag, HostTab::t().woff, rl);
+                else
+                    plx::p2_element<Real, HostTab>(X, U, lam, mu, rl);
                 for (int b = 0; b < nb; ++b)
                     for (int c = 0; c < 3; ++c) out[c * nb + b] = rl[c][b];
             }
@@ -150,7 +164,7 @@ extern "C" int plx_debug_emulate_apply(int cell_type, int num_cells, int num_ver
                                        int num_nodes, const int32_t *closure, const int32_t *cone,
                                        const double *coords, const int32_t *labels,
                                        int label_value, const double *lam_mu, int scalar_type,
-                                       const void *u, void *r, int mode) {
+                                       const void *u, void *r, int mode, int algorithm) {
     if ((cell_type != PLX_HEX_Q1 && cell_type != PLX_TET_P2) || !closure || !cone || !coords ||
         !lam_mu || !u || !r)
         return 1;
@@ -164,8 +178,10 @@ extern "C" int plx_debug_emulate_apply(int cell_type, int num_cells, int num_ver
     std::string msg;
     if (plx::build_host_plan(mv, label_value, plx::tile_limits(cell_type), &hp, &msg)) return 1;
     if (scalar_type == PLX_FP64)
-        return emulate<double>(mv, coords, hp, lam_mu, (const double *)u, (double *)r, mode, cell_type);
+        return emulate<double>(mv, coords, hp, lam_mu, (const double *)u, (double *)r, mode, cell_type,
+                               algorithm != PLX_ALGO_QUADRATURE_POINTS);
     if (scalar_type == PLX_FP32)
-        return emulate<float>(mv, coords, hp, lam_mu, (const float *)u, (float *)r, mode, cell_type);
+        return emulate<float>(mv, coords, hp, lam_mu, (const float *)u, (float *)r, mode, cell_type,
+                              algorithm != PLX_ALGO_QUADRATURE_POINTS);
     return 1;
 }

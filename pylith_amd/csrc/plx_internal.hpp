@@ -57,9 +57,13 @@ struct KeyPlan {
     int32_t *d_tile_nodes = nullptr, *d_tile_vert_ptr = nullptr, *d_tile_verts = nullptr;
     int32_t *d_tile_halo_off = nullptr;
     uint16_t *d_lclosure = nullptr, *d_lcone = nullptr;
+    uint16_t *d_records = nullptr;   // [ncells][8 or 16] packed local closure (+ cone) per cell
     uint8_t *d_color = nullptr;      // [ncells] colour within the tile (cells sorted by colour)
     int32_t *d_tile_ncolor = nullptr;
     int max_colors = 0;
+    bool sorted_by_color = true;
+    int32_t *d_tiles_affine = nullptr, *d_tiles_general = nullptr;  // Q1: tiles of affine cells / others
+    int num_tiles_affine = 0, num_tiles_general = 0;
     int32_t *d_shared_node = nullptr, *d_shared_ptr = nullptr, *d_shared_slot = nullptr;
     void *d_halo = nullptr;          // halo partial sums, sized for FP64
     // auxiliary field, repacked per key cell (SoA, tile order)
@@ -110,6 +114,10 @@ struct plx_dm_s {
     double *d_partials = nullptr;
     int partials_len = 0;
     plx::EventLog events;
+    int algorithm = PLX_ALGO_AUTO;
+    int accumulate = PLX_ACCUMULATE_AUTO;
+    int min_waves = 2;
+    int wg_per_cu = 2;               // resident workgroups per CU of the persistent cell kernels
     std::string last_error;
 };
 
@@ -143,7 +151,7 @@ void events_release(plx_dm dm);
 
 // Kernel launchers (plx_assembly.hip).
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
-                 void *r, int mode);
+                 void *r, int mode, double scale = 1.0);
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
 int launch_state_update(plx_dm dm, KeyPlan *kp, const double *u);
 int validate_geometry(plx_dm dm, int *num_bad);

@@ -12,7 +12,7 @@
 namespace plx {
 
 int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
-                    std::string *err) {
+                    std::string *err, bool sort_by_color) {
     const int nb = m.nb, nv = m.nv;
     std::vector<int32_t> &cells = hp->cells;
     cells.clear();
@@ -104,10 +104,11 @@ int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, H
                 const int32_t *cl = m.closure + (size_t)cells[i] * nb;
                 for (int b = 0; b < nb; ++b) used[2 * (size_t)cl[b]] = used[2 * (size_t)cl[b] + 1] = 0;
             }
-            std::stable_sort(order.begin(), order.end(),
-                             [](const std::pair<int, int32_t> &a, const std::pair<int, int32_t> &b) {
-                                 return a.first < b.first;
-                             });
+            if (sort_by_color)
+                std::stable_sort(order.begin(), order.end(),
+                                 [](const std::pair<int, int32_t> &a, const std::pair<int, int32_t> &b) {
+                                     return a.first < b.first;
+                                 });
             for (int i = b0; i < b1; ++i) {
                 hp->color[i] = (uint8_t)order[i - b0].first;
                 cells[i] = order[i - b0].second;

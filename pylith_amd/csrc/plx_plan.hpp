@@ -43,7 +43,10 @@ struct HostPlan {
     std::vector<int32_t> shared_node, shared_ptr, shared_slot;
 };
 
+// sort_by_color: order each tile's cells by colour (colour-phase accumulation activates
+// contiguous threads); otherwise cells keep their mesh order inside the tile (neighbouring
+// threads then touch neighbouring nodes: structured LDS access for atomic accumulation).
 int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
-                    std::string *err);
+                    std::string *err, bool sort_by_color = true);
 
 }  // namespace plx

@@ -1,0 +1,566 @@
+// plx_solver.hip -- Krylov solves around the matrix-free operator, on the device.
+//
+// The reference solves J du = -F with PETSc KSP (GMRES + GAMG, vpbjacobi fine smoother;
+// materials/Elasticity.cc:307-318, problems/GreensFns.cc:236-256).  Here the operator is the
+// matrix-free Jacobian action A = -J (symmetric positive definite for elasticity once the
+// constrained dofs are fixed), preconditioned by the inverses of its 3x3 point blocks.
+//
+//   * PLX_KSP_CG:        plain PCG in FP64 (the FP64 reference path);
+//   * PLX_KSP_FGMRES_IR: FGMRES in FP64 on the true residual, each preconditioner application
+//                        an FP32 PCG solve (iterative refinement in a flexible Krylov loop);
+//   * PLX_KSP_IR:        plain iterative refinement (Richardson) with the FP32 PCG correction.
+// Vectors are local vectors; constrained dofs are held fixed (the operator is masked there).
+// Scalars of the PCG recurrences stay on the device, so an iteration needs no host round trip.
+#include <vector>
+
+#include "plx_element.hpp"
+#include "plx_internal.hpp"
+
+namespace plx {
+
+constexpr int kRedBlocks = 512;
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_RZNEW, S_COUNT };
+
+template <typename Real>
+__global__ void __launch_bounds__(256) mask_kernel(int n, const int32_t *__restrict__ dofs, Real *v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[dofs[i]] = Real(0);
+}
+
+// Block-reduce helper (256 threads): returns the block sum in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *s) {
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    return s[0];
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) dot_partials(int64_t n, const Real *__restrict__ a,
+                                                    const Real *__restrict__ b, double *partials) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        acc += (double)a[i] * (double)b[i];
+    const double t = block_sum(acc, s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// scal[S_PQ] = sum(partials); scal[S_ALPHA] = scal[S_RZ] / scal[S_PQ]
+__global__ void __launch_bounds__(256) finish_alpha(const double *__restrict__ partials, double *scal) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) acc += partials[i];
+    const double t = block_sum(acc, s);
+    if (threadIdx.x == 0) {
+        scal[S_PQ] = t;
+        scal[S_ALPHA] = t != 0.0 ? scal[S_RZ] / t : 0.0;
+    }
+}
+
+// x += alpha p; r -= alpha q; z = Binv r (per node); partial sums of r.z and r.r
+template <typename Real>
+__global__ void __launch_bounds__(256) cg_update(int nnodes, const Real *__restrict__ Binv, Real *x,
+                                                 Real *r, const Real *__restrict__ p,
+                                                 const Real *__restrict__ q, Real *z,
+                                                 const double *__restrict__ scal, double *part_rz,
+                                                 double *part_rr) {
+    __shared__ double s[256];
+    const Real alpha = (Real)scal[S_ALPHA];
+    double rz = 0.0, rr = 0.0;
+    for (int node = blockIdx.x * 256 + threadIdx.x; node < nnodes; node += gridDim.x * 256) {
+        const size_t o = 3 * (size_t)node;
+        Real rv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[o + c] += alpha * p[o + c];
+            rv[c] = r[o + c] - alpha * q[o + c];
+            r[o + c] = rv[c];
+        }
+        const Real *B = Binv + 9 * (size_t)node;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const Real zc = B[3 * c] * rv[0] + B[3 * c + 1] * rv[1] + B[3 * c + 2] * rv[2];
+            z[o + c] = zc;
+            rz += (double)rv[c] * (double)zc;
+            rr += (double)rv[c] * (double)rv[c];
+        }
+    }
+    const double a = block_sum(rz, s);
+    __syncthreads();
+    const double b = block_sum(rr, s);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = a;
+        part_rr[blockIdx.x] = b;
+    }
+}
+
+// beta = rz_new / rz; rz = rz_new; rr
+__global__ void __launch_bounds__(256) finish_beta(const double *__restrict__ part_rz,
+                                                   const double *__restrict__ part_rr, double *scal) {
+    __shared__ double s[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) {
+        a += part_rz[i];
+        b += part_rr[i];
+    }
+    const double rz = block_sum(a, s);
+    __syncthreads();
+    const double rr = block_sum(b, s);
+    if (threadIdx.x == 0) {
+        const double old = scal[S_RZ];
+        scal[S_BETA] = old != 0.0 ? rz / old : 0.0;
+        scal[S_RZ] = rz;
+        scal[S_RR] = rr;
+    }
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) p_update(int64_t n, const Real *__restrict__ z, Real *p,
+                                                const double *__restrict__ scal) {
+    const Real beta = (Real)scal[S_BETA];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = z[i] + beta * p[i];
+}
+
+// z = Binv r, p = z, partials of r.z and r.r (start of PCG)
+template <typename Real>
+__global__ void __launch_bounds__(256) pc_start(int nnodes, const Real *__restrict__ Binv,
+                                                const Real *__restrict__ r, Real *z, Real *p,
+                                                double *part_rz, double *part_rr) {
+    __shared__ double s[256];
+    double rz = 0.0, rr = 0.0;
+    for (int node = blockIdx.x * 256 + threadIdx.x; node < nnodes; node += gridDim.x * 256) {
+        const size_t o = 3 * (size_t)node;
+        const Real *B = Binv + 9 * (size_t)node;
+        const Real r0 = r[o], r1 = r[o + 1], r2 = r[o + 2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const Real zc = B[3 * c] * r0 + B[3 * c + 1] * r1 + B[3 * c + 2] * r2;
+            z[o + c] = zc;
+            p[o + c] = zc;
+            const Real rc = c == 0 ? r0 : c == 1 ? r1 : r2;
+            rz += (double)rc * (double)zc;
+            rr += (double)rc * (double)rc;
+        }
+    }
+    const double a = block_sum(rz, s);
+    __syncthreads();
+    const double b = block_sum(rr, s);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = a;
+        part_rr[blockIdx.x] = b;
+    }
+}
+
+__global__ void __launch_bounds__(256) finish_start(const double *__restrict__ part_rz,
+                                                    const double *__restrict__ part_rr, double *scal) {
+    __shared__ double s[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 256) {
+        a += part_rz[i];
+        b += part_rr[i];
+    }
+    const double rz = block_sum(a, s);
+    __syncthreads();
+    const double rr = block_sum(b, s);
+    if (threadIdx.x == 0) {
+        scal[S_RZ] = rz;
+        scal[S_RR] = rr;
+        scal[S_BETA] = 0.0;
+    }
+}
+
+template <typename Src, typename Dst>
+__global__ void __launch_bounds__(256) convert_kernel(int64_t n, const Src *__restrict__ a, Dst *b,
+                                                      double scale) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        b[i] = (Dst)((double)a[i] * scale);
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) fill_kernel(int64_t n, Real *a, Real v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        a[i] = v;
+}
+
+// y = a*x + b*y (FP64)
+__global__ void __launch_bounds__(256) axpby_kernel(int64_t n, double a, const double *__restrict__ x,
+                                                    double b, double *y) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        y[i] = a * x[i] + b * y[i];
+}
+
+// Invert the 3x3 point blocks of A = -J; rows/columns of constrained dofs become identity.
+__global__ void __launch_bounds__(256) invert_blocks(int nnodes, const double *__restrict__ Jblocks,
+                                                     const uint8_t *__restrict__ conmask,
+                                                     double *Binv64, float *Binv32) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nnodes) return;
+    double A[9];
+    for (int i = 0; i < 9; ++i) A[i] = -Jblocks[9 * (size_t)node + i];
+    for (int c = 0; c < 3; ++c)
+        if (conmask[3 * (size_t)node + c]) {
+            for (int k = 0; k < 3; ++k) A[3 * c + k] = A[3 * k + c] = 0.0;
+            A[4 * c] = 1.0;
+        }
+    double Ai[9];
+    const double det = inv3(A, Ai);
+    for (int i = 0; i < 9; ++i) {
+        const double v = det != 0.0 ? Ai[i] : (i % 4 == 0 ? 1.0 : 0.0);
+        Binv64[9 * (size_t)node + i] = v;
+        Binv32[9 * (size_t)node + i] = (float)v;
+    }
+}
+
+}  // namespace plx
+
+using namespace plx;
+
+struct plx_solver_s {
+    plx_dm dm = nullptr;
+    KeyPlan *kp = nullptr;
+    int64_t n = 0;
+    int nnodes = 0;
+    double *Binv64 = nullptr;
+    float *Binv32 = nullptr;
+    // FP32 inner PCG work vectors
+    float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr;
+    // FP64 PCG work vectors
+    double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *z64 = nullptr, *t64 = nullptr;
+    // FGMRES bases
+    int restart = 30;
+    std::vector<double *> V, Z;
+    double *scal = nullptr, *part_a = nullptr, *part_b = nullptr;
+    double inner_rtol = 1e-4;
+    int inner_maxit = 2000;
+    plx_solve_stats last{};
+};
+
+namespace plx {
+
+static inline dim3 red_grid() { return dim3(kRedBlocks); }
+
+template <typename Real>
+static int masked_apply(plx_solver s, const Real *x, Real *y) {
+    plx_dm dm = s->dm;
+    const int st = std::is_same<Real, double>::value ? PLX_FP64 : PLX_FP32;
+    int err = launch_apply(dm, s->kp, st, true, x, y, PLX_INSERT_VALUES, -1.0);
+    if (err) return err;
+    const int nc = (int)dm->con.dofs.size();
+    if (nc > 0) {
+        hipLaunchKernelGGL(mask_kernel<Real>, dim3((nc + 255) / 256), dim3(256), 0, dm->stream, nc,
+                           dm->con.d_dofs, y);
+        PLX_HIP(dm, hipGetLastError());
+    }
+    return 0;
+}
+
+static int read_scalar(plx_solver s, int idx, double *v) {
+    PLX_HIP(s->dm, hipMemcpyAsync(v, s->scal + idx, sizeof(double), hipMemcpyDeviceToHost, s->dm->stream));
+    PLX_HIP(s->dm, hipStreamSynchronize(s->dm->stream));
+    return 0;
+}
+
+static int dot64(plx_solver s, const double *a, const double *b, double *out) {
+    plx_dm dm = s->dm;
+    hipLaunchKernelGGL(dot_partials<double>, red_grid(), dim3(256), 0, dm->stream, s->n, a, b, s->part_a);
+    hipLaunchKernelGGL(finish_start, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_a, s->scal);
+    PLX_HIP(dm, hipGetLastError());
+    return read_scalar(s, S_RZ, out);
+}
+
+// PCG on A x = b (x, b, work vectors of type Real; x must hold the initial guess, masked).
+// Stops when ||r|| <= rtol ||b|| or maxit; checks convergence every `check` iterations.
+template <typename Real>
+static int pcg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, const Real *Binv,
+               double rtol, int maxit, int *its, double *rnorm_out) {
+    plx_dm dm = s->dm;
+    // r = b - A x  (caller stored b in r; x == 0 guess skips the apply)
+    hipLaunchKernelGGL(pc_start<Real>, red_grid(), dim3(256), 0, dm->stream, s->nnodes, Binv, r, z, p,
+                       s->part_a, s->part_b);
+    hipLaunchKernelGGL(finish_start, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+    PLX_HIP(dm, hipGetLastError());
+    double rr0 = 0.0;
+    int err = read_scalar(s, S_RR, &rr0);
+    if (err) return err;
+    const double target = rtol * rtol * rr0;
+    int it = 0;
+    double rr = rr0;
+    const int check = 10;
+    while (it < maxit && rr > target && rr0 > 0.0) {
+        for (int k = 0; k < check && it < maxit; ++k, ++it) {
+            if ((err = masked_apply<Real>(s, p, q))) return err;
+            hipLaunchKernelGGL(dot_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, p, q, s->part_a);
+            hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
+            hipLaunchKernelGGL(cg_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->nnodes, Binv, x, r,
+                               p, q, z, s->scal, s->part_a, s->part_b);
+            hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+            hipLaunchKernelGGL(p_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, z, p, s->scal);
+        }
+        PLX_HIP(dm, hipGetLastError());
+        if ((err = read_scalar(s, S_RR, &rr))) return err;
+    }
+    *its = it;
+    *rnorm_out = rr0 > 0.0 ? std::sqrt(rr / rr0) : 0.0;
+    return 0;
+}
+
+// True FP64 residual r = b - A x on unconstrained dofs (x carries the boundary values).
+static int true_residual(plx_solver s, const double *b, const double *x, double *r) {
+    plx_dm dm = s->dm;
+    int err = masked_apply<double>(s, x, r);
+    if (err) return err;
+    if (b)
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, b, -1.0, r);
+    else
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 0.0, r, -1.0, r);
+    PLX_HIP(dm, hipGetLastError());
+    const int nc = (int)dm->con.dofs.size();
+    if (nc > 0) {
+        hipLaunchKernelGGL(mask_kernel<double>, dim3((nc + 255) / 256), dim3(256), 0, dm->stream, nc,
+                           dm->con.d_dofs, r);
+        PLX_HIP(dm, hipGetLastError());
+    }
+    return 0;
+}
+
+// Inner preconditioner: out (FP64) ~= A^{-1} in (FP64) by FP32 PCG from a zero guess.
+static int inner_solve(plx_solver s, const double *in, double *out, int *its) {
+    plx_dm dm = s->dm;
+    double nrm = 0.0;
+    int err = dot64(s, in, in, &nrm);
+    if (err) return err;
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) {
+        hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, s->n, out, 0.0);
+        *its = 0;
+        return 0;
+    }
+    hipLaunchKernelGGL((convert_kernel<double, float>), red_grid(), dim3(256), 0, dm->stream, s->n, in,
+                       s->r32, 1.0 / nrm);
+    hipLaunchKernelGGL(fill_kernel<float>, red_grid(), dim3(256), 0, dm->stream, s->n, s->x32, 0.0f);
+    PLX_HIP(dm, hipGetLastError());
+    double rn = 0.0;
+    if ((err = pcg<float>(s, s->x32, s->r32, s->p32, s->q32, s->z32, s->Binv32, s->inner_rtol,
+                          s->inner_maxit, its, &rn)))
+        return err;
+    hipLaunchKernelGGL((convert_kernel<float, double>), red_grid(), dim3(256), 0, dm->stream, s->n, s->x32,
+                       out, nrm);
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
+}  // namespace plx
+
+extern "C" {
+
+int plx_solver_create(plx_solver *out, plx_dm dm, int label_value, int part) {
+    if (!out || !dm) return 1;
+    *out = nullptr;
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp || !kp->jf3 || !kp->has_aux) return set_error(dm, 1, "solver needs a key with Jacobian kernels and aux");
+    plx_solver s = new plx_solver_s();
+    s->dm = dm;
+    s->kp = kp;
+    s->n = dm->local_size;
+    s->nnodes = dm->num_nodes;
+    const size_t n = (size_t)s->n;
+    bool ok = hipMalloc(&s->Binv64, 9 * (size_t)s->nnodes * sizeof(double)) == hipSuccess &&
+              hipMalloc(&s->Binv32, 9 * (size_t)s->nnodes * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->scal, 16 * sizeof(double)) == hipSuccess &&
+              hipMalloc(&s->part_a, kRedBlocks * sizeof(double)) == hipSuccess &&
+              hipMalloc(&s->part_b, kRedBlocks * sizeof(double)) == hipSuccess;
+    for (float **v : {&s->x32, &s->r32, &s->p32, &s->q32, &s->z32})
+        ok = ok && hipMalloc(v, n * sizeof(float)) == hipSuccess;
+    for (double **v : {&s->r64, &s->p64, &s->q64, &s->z64, &s->t64})
+        ok = ok && hipMalloc(v, n * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        plx_solver_destroy(s);
+        return set_error(dm, 76, "solver allocation failed");
+    }
+    *out = s;
+    return plx_solver_setup(s);
+}
+
+int plx_solver_destroy(plx_solver s) {
+    if (!s) return 0;
+    void *ptrs[] = {s->Binv64, s->Binv32, s->scal, s->part_a, s->part_b, s->x32, s->r32, s->p32,
+                    s->q32,    s->z32,    s->r64,  s->p64,    s->q64,    s->z64, s->t64};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (double *p : s->V) (void)hipFree(p);
+    for (double *p : s->Z) (void)hipFree(p);
+    delete s;
+    return 0;
+}
+
+int plx_solver_setup(plx_solver s) {
+    if (!s) return 1;
+    plx_dm dm = s->dm;
+    double *blocks = nullptr;
+    uint8_t *mask = nullptr;
+    PLX_HIP(dm, hipMalloc(&blocks, 9 * (size_t)s->nnodes * sizeof(double)));
+    PLX_HIP(dm, hipMemsetAsync(blocks, 0, 9 * (size_t)s->nnodes * sizeof(double), dm->stream));
+    int err = launch_pbdiag(dm, s->kp, blocks);
+    if (err) return err;
+    std::vector<uint8_t> hmask(dm->local_size, 0);
+    for (int32_t d : dm->con.dofs) hmask[d] = 1;
+    PLX_HIP(dm, hipMalloc(&mask, hmask.size()));
+    PLX_HIP(dm, hipMemcpyAsync(mask, hmask.data(), hmask.size(), hipMemcpyHostToDevice, dm->stream));
+    hipLaunchKernelGGL(invert_blocks, dim3((s->nnodes + 255) / 256), dim3(256), 0, dm->stream, s->nnodes,
+                       blocks, mask, s->Binv64, s->Binv32);
+    PLX_HIP(dm, hipGetLastError());
+    PLX_HIP(dm, hipStreamSynchronize(dm->stream));
+    (void)hipFree(blocks);
+    (void)hipFree(mask);
+    return 0;
+}
+
+int plx_solver_set_inner(plx_solver s, double rtol, int maxit, int restart) {
+    if (!s || rtol <= 0.0 || maxit <= 0 || restart <= 0 || restart > 200) return 1;
+    s->inner_rtol = rtol;
+    s->inner_maxit = maxit;
+    s->restart = restart;
+    return 0;
+}
+
+int plx_solver_solve(plx_solver s, int method, const double *b, double *x, double rtol, int maxit,
+                     plx_solve_stats *stats) {
+    if (!s || !x || rtol <= 0.0 || maxit <= 0) return 1;
+    plx_dm dm = s->dm;
+    plx_solve_stats st{};
+    int err = 0;
+    const int nc = (int)dm->con.dofs.size();
+    // r = b - A x (true FP64 residual)
+    if ((err = true_residual(s, b, x, s->r64))) return err;
+    double rr0 = 0.0;
+    if ((err = dot64(s, s->r64, s->r64, &rr0))) return err;
+    const double r0 = std::sqrt(rr0);
+    st.initial_residual = r0;
+    if (method == PLX_KSP_CG) {
+        // PCG in FP64 on the correction: A d = r, x += d
+        double *d = s->t64;
+        hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, s->n, d, 0.0);
+        int its = 0;
+        double rn = 0.0;
+        if ((err = pcg<double>(s, d, s->r64, s->p64, s->q64, s->z64, s->Binv64, rtol, maxit, &its, &rn)))
+            return err;
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, d, 1.0, x);
+        PLX_HIP(dm, hipGetLastError());
+        st.outer_iterations = its;
+        st.applies_fp64 = its + 1;
+    } else if (method == PLX_KSP_IR) {
+        double rr = rr0;
+        while (st.outer_iterations < maxit && r0 > 0.0 && std::sqrt(rr) > rtol * r0) {
+            int its = 0;
+            if ((err = inner_solve(s, s->r64, s->t64, &its))) return err;
+            st.inner_iterations += its;
+            hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, s->t64, 1.0, x);
+            if ((err = true_residual(s, b, x, s->r64))) return err;
+            if ((err = dot64(s, s->r64, s->r64, &rr))) return err;
+            st.outer_iterations++;
+            st.applies_fp64++;
+        }
+        st.applies_fp64++;
+    } else if (method == PLX_KSP_FGMRES_IR) {
+        const int m = s->restart;
+        while ((int)s->V.size() < m + 1) {
+            double *v = nullptr;
+            PLX_HIP(dm, hipMalloc(&v, (size_t)s->n * sizeof(double)));
+            s->V.push_back(v);
+        }
+        while ((int)s->Z.size() < m) {
+            double *v = nullptr;
+            PLX_HIP(dm, hipMalloc(&v, (size_t)s->n * sizeof(double)));
+            s->Z.push_back(v);
+        }
+        double beta = r0;
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1);
+        bool done = r0 == 0.0;
+        while (!done && st.outer_iterations < maxit) {
+            // V0 = r / beta
+            hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0 / beta, s->r64,
+                               0.0, s->V[0]);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int j = 0;
+            for (; j < m && st.outer_iterations < maxit; ++j) {
+                int its = 0;
+                if ((err = inner_solve(s, s->V[j], s->Z[j], &its))) return err;
+                st.inner_iterations += its;
+                if (nc > 0)
+                    hipLaunchKernelGGL(mask_kernel<double>, dim3((nc + 255) / 256), dim3(256), 0, dm->stream,
+                                       nc, dm->con.d_dofs, s->Z[j]);
+                if ((err = masked_apply<double>(s, s->Z[j], s->V[j + 1]))) return err;
+                st.applies_fp64++;
+                // modified Gram-Schmidt
+                for (int i = 0; i <= j; ++i) {
+                    double h = 0.0;
+                    if ((err = dot64(s, s->V[i], s->V[j + 1], &h))) return err;
+                    H[(size_t)i * m + j] = h;
+                    hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, -h, s->V[i],
+                                       1.0, s->V[j + 1]);
+                }
+                double hn = 0.0;
+                if ((err = dot64(s, s->V[j + 1], s->V[j + 1], &hn))) return err;
+                hn = std::sqrt(hn);
+                H[(size_t)(j + 1) * m + j] = hn;
+                if (hn > 0.0)
+                    hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0 / hn,
+                                       s->V[j + 1], 0.0, s->V[j + 1]);
+                // Givens rotations
+                for (int i = 0; i < j; ++i) {
+                    const double a = H[(size_t)i * m + j], c = H[(size_t)(i + 1) * m + j];
+                    H[(size_t)i * m + j] = cs[i] * a + sn[i] * c;
+                    H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+                }
+                const double a = H[(size_t)j * m + j], c = H[(size_t)(j + 1) * m + j];
+                const double rho = std::sqrt(a * a + c * c);
+                cs[j] = rho > 0 ? a / rho : 1.0;
+                sn[j] = rho > 0 ? c / rho : 0.0;
+                H[(size_t)j * m + j] = rho;
+                H[(size_t)(j + 1) * m + j] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                st.outer_iterations++;
+                if (std::fabs(g[j + 1]) <= rtol * r0 || hn == 0.0) {
+                    ++j;
+                    break;
+                }
+            }
+            // x += Z y, H y = g (upper triangular j x j)
+            std::vector<double> y(j, 0.0);
+            for (int i = j - 1; i >= 0; --i) {
+                double acc = g[i];
+                for (int k = i + 1; k < j; ++k) acc -= H[(size_t)i * m + k] * y[k];
+                y[i] = acc / H[(size_t)i * m + i];
+            }
+            for (int i = 0; i < j; ++i)
+                hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, y[i], s->Z[i], 1.0, x);
+            PLX_HIP(dm, hipGetLastError());
+            // restart from the true residual
+            if ((err = true_residual(s, b, x, s->r64))) return err;
+            double rr = 0.0;
+            if ((err = dot64(s, s->r64, s->r64, &rr))) return err;
+            beta = std::sqrt(rr);
+            st.applies_fp64++;
+            done = beta <= rtol * r0;
+        }
+    } else {
+        return set_error(dm, 1, "unknown solver method");
+    }
+    if ((err = true_residual(s, b, x, s->r64))) return err;
+    double rr = 0.0;
+    if ((err = dot64(s, s->r64, s->r64, &rr))) return err;
+    st.final_residual = std::sqrt(rr);
+    st.converged = st.final_residual <= rtol * r0 * (method == PLX_KSP_CG ? 10.0 : 1.0) ? 1 : 0;
+    s->last = st;
+    if (stats) *stats = st;
+    return 0;
+}
+
+}  // extern "C"

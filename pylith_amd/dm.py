@@ -88,6 +88,9 @@ class DM:
         self._check(self.lib.plx_dm_get_sizes(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_kernel_options(self, algorithm=L.ALGO_AUTO, accumulate=L.ACCUMULATE_AUTO, min_waves=0):
+        self._check(self.lib.plx_dm_set_kernel_options(self.h, algorithm, accumulate, min_waves))
+
     def set_constraints(self, dofs):
         d, dp = _i32(dofs)
         self._check(self.lib.plx_dm_set_constraints(self.h, d.size, dp))
@@ -157,3 +160,39 @@ class DM:
 
     def reset_event_stats(self):
         self._check(self.lib.plx_dm_reset_event_stats(self.h))
+
+
+class Solver:
+    """Matrix-free Krylov solver of one key's operator A = -J (plx_solver_* in the C ABI)."""
+
+    def __init__(self, dm, label_value=0, part=0):
+        self.dm = dm
+        self.lib = dm.lib
+        h = ctypes.c_void_p()
+        dm._check(self.lib.plx_solver_create(ctypes.byref(h), dm.h, label_value, part))
+        self.h = h
+
+    def set_inner(self, rtol=1e-4, maxit=2000, restart=30):
+        self.dm._check(self.lib.plx_solver_set_inner(self.h, rtol, maxit, restart))
+
+    def setup(self):
+        self.dm._check(self.lib.plx_solver_setup(self.h))
+
+    def solve(self, x, b=None, method=L.KSP_FGMRES_IR, rtol=1e-10, maxit=10000):
+        if x.dtype != torch.float64:
+            raise PlxError("solutions are FP64 local vectors")
+        st = L.SolveStats()
+        self.dm._check(self.lib.plx_solver_solve(self.h, method, _ptr(b), _ptr(x), rtol, maxit,
+                                                 ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.plx_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

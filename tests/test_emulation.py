@@ -14,7 +14,7 @@ from pylith_amd import meshio_ascii
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-def emulate(m, aux, u, label=0, scalar=L.FP64, mode=L.INSERT_VALUES, r0=None):
+def emulate(m, aux, u, label=0, scalar=L.FP64, mode=L.INSERT_VALUES, r0=None, algo=L.ALGO_AUTO):
     lib = L.load()
     mu, K = aux[:, 1], aux[:, 2]
     lam_mu = np.ascontiguousarray(np.stack([K - 2 * mu / 3, mu], 1))
@@ -28,7 +28,7 @@ def emulate(m, aux, u, label=0, scalar=L.FP64, mode=L.INSERT_VALUES, r0=None):
                                       cl.ctypes.data_as(L._i32p), co.ctypes.data_as(L._i32p),
                                       xy.ctypes.data_as(L._dblp), lb.ctypes.data_as(L._i32p), label,
                                       lam_mu.ctypes.data_as(L._dblp), scalar,
-                                      ctypes.c_void_p(uu.ctypes.data), ctypes.c_void_p(r.ctypes.data), mode)
+                                      ctypes.c_void_p(uu.ctypes.data), ctypes.c_void_p(r.ctypes.data), mode, algo)
     assert err == 0, f"emulation error {err}"
     return r
 
@@ -55,15 +55,16 @@ def cases():
     }
 
 
+@pytest.mark.parametrize("algo", [L.ALGO_QUADRATURE_POINTS, L.ALGO_FACTORED])
 @pytest.mark.parametrize("name", list(cases()))
-def test_emulated_kernels_match_oracle(oracle, name):
+def test_emulated_kernels_match_oracle(oracle, name, algo):
     m, label = cases()[name]
     aux, sizes = mg.elastic_aux(m.num_cells, seed=5, vs_sigma=0.2, vp_vs=1.8)
     u = mg.random_field(m.num_nodes, 3)
     ref = oracle.Problem(m, aux, sizes).residual(u, 1)
-    r = emulate(m, aux, u, label)
+    r = emulate(m, aux, u, label, algo=algo)
     assert relerr(r, ref) <= 1e-12
-    r32 = emulate(m, aux, u, label, scalar=L.FP32)
+    r32 = emulate(m, aux, u, label, scalar=L.FP32, algo=algo)
     assert relerr(r32.astype(np.float64), ref) <= 2e-5
-    radd = emulate(m, aux, u, label, mode=L.ADD_VALUES, r0=np.ones_like(u))
+    radd = emulate(m, aux, u, label, mode=L.ADD_VALUES, r0=np.ones_like(u), algo=algo)
     assert relerr(radd - 1.0, ref) <= 1e-11

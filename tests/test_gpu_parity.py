@@ -45,14 +45,17 @@ def check_residual(oracle, m, u, label=0, aux=None, dtypes=(torch.float64, torch
     ref = P.residual(u, F1, cells=np.nonzero(m.labels == label)[0] if label else None)
     dm = make_dm(m, aux, sizes, label)
     print(f"[parity] cells={m.num_cells} nodes={m.num_nodes} ct={m.cell_type}", flush=True)
-    for dt in dtypes:
+    from pylith_amd import _lib as L
+    for algo, acc, dt in [(a, c, d) for a in (L.ALGO_QUADRATURE_POINTS, L.ALGO_FACTORED)
+                          for c in (L.ACCUMULATE_COLORED, L.ACCUMULATE_LDS_ATOMIC) for d in dtypes]:
+        dm.set_kernel_options(algo, acc, 0)
         x = dev(u, dt)
         r = torch.zeros_like(x)
         dm.compute_residual(label, 0, x, r, mode=1)
         torch.cuda.synchronize()
         e = relerr(r.double().cpu().numpy(), ref)
         tol = TOL64 if dt == torch.float64 else TOL32
-        assert e <= tol, (dt, e)
+        assert e <= tol, (dt, algo, acc, e)
         # ADD mode accumulates on top of existing values
         r2 = torch.full_like(x, 1.0)
         dm.compute_residual(label, 0, x, r2, mode=0)
@@ -140,6 +143,8 @@ def test_deterministic_bitwise(oracle):
     m = mg.tet_p2_box(8, (0, 0, 0), (1.0, 1.0, 1.0))
     aux, sizes = mg.elastic_aux(m.num_cells)
     dm = make_dm(m, aux, sizes)
+    from pylith_amd import _lib as L
+    dm.set_kernel_options(L.ALGO_AUTO, L.ACCUMULATE_COLORED)
     x = dev(mg.random_field(m.num_nodes, 3))
     outs = []
     for _ in range(3):
