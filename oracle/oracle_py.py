@@ -218,3 +218,45 @@ def tangential_directions(r1, r2, n):
     r1, r2, n = (np.ascontiguousarray(a, np.float64) for a in (r1, r2, n))
     lib().oracle_tangential_directions(_p(t1, "d"), _p(t2, "d"), _p(r1, "d"), _p(r2, "d"), _p(n, "d"))
     return t1, t2
+
+
+class Cohesive:
+    """Cohesive cells of one fault on the CPU (prescribed slip, FaultCohesiveKin kernels)."""
+
+    def __init__(self, mesh, fault, slip, lag_offset):
+        self.closure_neg = np.ascontiguousarray(fault.closure_neg, np.int32)
+        self.closure_pos = np.ascontiguousarray(fault.closure_pos, np.int32)
+        self.closure_lag = np.ascontiguousarray(fault.closure_lag, np.int32)
+        self.face_cone = np.ascontiguousarray(fault.face_cone, np.int32)
+        self.coords = np.ascontiguousarray(mesh.coords, np.float64)
+        self.slip = np.ascontiguousarray(slip, np.float64)
+        s = OracleCohesive()
+        s.num_cells = self.closure_neg.shape[0]
+        s.closure_neg = _p(self.closure_neg, "i")
+        s.closure_pos = _p(self.closure_pos, "i")
+        s.closure_lag = _p(self.closure_lag, "i")
+        s.face_cone = _p(self.face_cone, "i")
+        s.coords = _p(self.coords, "d")
+        s.lag_offset = lag_offset
+        s.slip = _p(self.slip, "d")
+        for i in range(3):
+            s.ref_dir_1[i] = fault.ref_dir_1[i]
+            s.ref_dir_2[i] = fault.ref_dir_2[i]
+        self.s = s
+
+    def residual(self, locX, out=None, t=0.0):
+        locX = np.ascontiguousarray(locX, np.float64)
+        out = np.zeros_like(locX) if out is None else out
+        err = lib().oracle_cohesive_residual(ctypes.byref(self.s), ctypes.c_double(t), _p(locX, "d"),
+                                             _p(out, "d"))
+        if err:
+            raise RuntimeError(f"oracle cohesive residual error {err}")
+        return out
+
+    def jacobian_action(self, locY, out=None):
+        locY = np.ascontiguousarray(locY, np.float64)
+        out = np.zeros_like(locY) if out is None else out
+        err = lib().oracle_cohesive_jacobian_action(ctypes.byref(self.s), _p(locY, "d"), _p(out, "d"))
+        if err:
+            raise RuntimeError(f"oracle cohesive jacobian error {err}")
+        return out

@@ -217,3 +217,110 @@ def shear_bcs(node_coords, lo_x, hi_x, shear=1.0, tol=1e-9):
     vals = np.concatenate([np.zeros(3 * neg.size), np.full(pos.size, shear)])
     order = np.argsort(dofs, kind="stable")
     return dofs[order], vals[order]
+
+
+# ------------------------------------------------------------------------------------------
+# Faults: cohesive cells on a through-going mesh surface (faults/TopologyOps.cc inserts them
+# in PyLith; here the structured generator splits the nodes of the plane i = i_fault).
+# ------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class Fault:
+    closure_neg: np.ndarray   # [ncoh, 4] node ids on the negative side (face vertex order)
+    closure_pos: np.ndarray   # [ncoh, 4] node ids on the positive side
+    closure_lag: np.ndarray   # [ncoh, 4] Lagrange-multiplier node ids (0 .. num_lag - 1)
+    face_cone: np.ndarray     # [ncoh, 4] vertex ids (negative side) for the face geometry
+    neg_node: np.ndarray      # [num_lag] negative-side node of each Lagrange node
+    pos_node: np.ndarray      # [num_lag] positive-side node of each Lagrange node
+    lag_coords: np.ndarray    # [num_lag, 3]
+    lag_jk: np.ndarray        # [num_lag, 2] (j, k) grid position on the fault
+    colors: np.ndarray        # [ncoh] 4-colouring of the cohesive cells (no shared nodes)
+    ref_dir_1: tuple = (0.0, 0.0, 1.0)   # FaultCohesive.py defaults
+    ref_dir_2: tuple = (0.0, 1.0, 0.0)
+
+    @property
+    def num_lag(self):
+        return self.neg_node.size
+
+    @property
+    def num_cohesive(self):
+        return self.closure_neg.shape[0]
+
+
+def hex_box_with_fault(n, lo, hi, i_fault, brick=8, shear=None):
+    """Q1 box with a through-going fault on the vertex plane i = i_fault (0 < i_fault < nx).
+    Nodes on the plane are split: the negative side (-x, cells i < i_fault) keeps the original
+    ids, the positive side gets ids num_vertices + (fault vertex index).  Returns (mesh, fault).
+    The face vertex order makes d x/d xi (along +y) cross d x/d eta (along +z) point from the
+    negative to the positive side."""
+    n = (n, n, n) if np.isscalar(n) else tuple(n)
+    nx, ny, nz = n
+    if not 0 < i_fault < nx:
+        raise ValueError("the fault plane must be interior")
+    m = hex_box(n, lo, hi, brick=brick, shear=shear)
+    vx, vy, vz = nx + 1, ny + 1, nz + 1
+    vrank = _brick_order((vx, vy, vz), brick)
+    J, K = np.meshgrid(np.arange(vy), np.arange(vz), indexing="ij")
+    J, K = J.ravel(order="F"), K.ravel(order="F")
+    fault_v = vrank[i_fault + vx * (J + vy * K)]            # fault vertex ids (j fastest)
+    nf = fault_v.size
+    nv = m.num_vertices
+    pos_id = nv + np.arange(nf)
+    # cells on the positive side: cell index i >= i_fault
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci = ci.ravel(order="F")
+    crank = _brick_order(n, brick)
+    cell_i = np.empty(ci.size, np.int64)
+    cell_i[crank] = ci
+    remap = np.arange(nv + nf)
+    remap[fault_v] = pos_id
+    closure = m.closure.copy()
+    posc = cell_i >= i_fault
+    closure[posc] = remap[closure[posc]].astype(np.int32)
+    coords = np.vstack([m.coords, m.coords[fault_v]])
+    # cohesive cells on faces (j, k)
+    fj, fk = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    fj, fk = fj.ravel(order="F"), fk.ravel(order="F")
+    lag = lambda j, k: j + vy * k                           # Lagrange node = fault vertex index
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    cl_lag = np.stack([lag(fj + a, fk + b) for a, b in corners], 1).astype(np.int32)
+    cl_neg = fault_v[cl_lag].astype(np.int32)
+    cl_pos = pos_id[cl_lag].astype(np.int32)
+    colors = ((fj % 2) + 2 * (fk % 2)).astype(np.int32)
+    mesh = Mesh(HEX_Q1, closure, closure.copy(), coords, coords, n,
+                np.zeros(closure.shape[0], np.int32), m.colors)
+    fault = Fault(cl_neg, cl_pos, cl_lag, cl_neg.copy(), fault_v.astype(np.int32),
+                  pos_id.astype(np.int32), m.coords[fault_v], np.stack([J, K], 1), colors)
+    return mesh, fault
+
+
+def fault_normal_frame(fault, coords):
+    """Unit normal of the (planar) fault and the tangent directions of
+    BoundaryDirections::tangential_directions (ref dirs of the fault)."""
+    X = coords[fault.face_cone[0]]
+    dxi = 0.25 * (-X[0] + X[1] + X[2] - X[3])
+    deta = 0.25 * (-X[0] - X[1] + X[2] + X[3])
+    nrm = np.cross(dxi, deta)
+    nrm /= np.linalg.norm(nrm)
+    r1, r2 = np.asarray(fault.ref_dir_1, float), np.asarray(fault.ref_dir_2, float)
+    r = r1 if abs(r1 @ nrm) <= 0.999 else r2
+    t1 = np.cross(r, nrm)
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(nrm, t1)
+    t2 /= np.linalg.norm(t2)
+    if r2[0] * nrm[0] + r2[1] * nrm[1] + r2[1] * nrm[1] < 0.0:  # as written at BoundaryDirections.hh:76
+        t2 = -t2
+    return nrm, t1, t2
+
+
+def tapered_patch_slip(fault, center, half_width, slip, component=1, taper=None):
+    """Slip [num_lag, 3] (opening, left-lateral, reverse): `slip` on the square patch of the
+    fault (in fault-plane coordinates (y, z)) |y - yc| <= a, |z - zc| <= a, tapering linearly to
+    zero over `taper` outside it."""
+    taper = half_width if taper is None else taper
+    y, z = fault.lag_coords[:, 1], fault.lag_coords[:, 2]
+    dy = np.maximum(np.abs(y - center[0]) - half_width, 0.0)
+    dz = np.maximum(np.abs(z - center[1]) - half_width, 0.0)
+    w = np.clip(1.0 - np.maximum(dy, dz) / taper, 0.0, 1.0)
+    s = np.zeros((fault.num_lag, 3))
+    s[:, component] = slip * w
+    return s
