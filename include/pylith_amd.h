@@ -146,6 +146,35 @@ int plx_global_to_local(plx_dm dm, int scalar_type, const void *glob, void *loc)
 int plx_insert_boundary_values(plx_dm dm, int scalar_type, const void *values, void *loc);
 
 /* ---------------------------------------------------------------------------------------------
+ * Fault with prescribed slip: the cohesive cells of FaultCohesiveKin integrated by
+ * IntegratorInterface (faults/FaultCohesiveKin.cc:330-461, feassemble/IntegratorInterface.cc:
+ * 603-722).  Q1 hex meshes.  Per cohesive cell [num_cohesive][4], in the face's cyclic vertex
+ * order: closure_neg / closure_pos the split displacement nodes of the negative / positive side,
+ * closure_lag the fault (Lagrange-multiplier) node, face_cone the vertex ids of the face
+ * geometry (into the mesh coords).  Adds 3*num_lagrange dofs after the displacement field;
+ * constraints set before are kept.  ref_dir_1/2 are the reference directions of the fault
+ * frame (FaultCohesive::setRefDir1/2, BoundaryDirections.hh:50-82).  HOST arrays, copied. */
+int plx_dm_add_fault(plx_dm dm, int num_cohesive, const int32_t *closure_neg,
+                     const int32_t *closure_pos, const int32_t *closure_lag,
+                     const int32_t *face_cone, int num_lagrange, const double ref_dir_1[3],
+                     const double ref_dir_2[3]);
+/* Slip per fault node [num_lagrange][3] in the fault frame (opening = normal, left-lateral =
+ * t1, reverse = t2), the "slip" auxiliary subfield that FaultCohesiveKin::_updateSlip writes
+ * (FaultCohesiveKin.cc:264-324).  _device takes a device array (stream-ordered copy). */
+int plx_dm_set_fault_slip(plx_dm dm, const double *slip);
+int plx_dm_set_fault_slip_device(plx_dm dm, const double *slip);
+int plx_dm_get_fault_info(plx_dm dm, int64_t *lagrange_offset, int *num_lagrange);
+/* DMPlexComputeResidualHybridByKey (IntegratorInterface.cc:653) with the kernels f0u_neg,
+ * f0u_pos, f0l_slip: locF (+)= the cohesive cells' residual, FP64.  INSERT_VALUES zeroes the
+ * Lagrange-multiplier block first and adds into the displacement block (the domain residual
+ * owns those entries). */
+int plx_compute_residual_cohesive(plx_dm dm, double t, const double *locX, double *locF,
+                                  int mode);
+/* DMPlexComputeJacobianHybridByKey's blocks Jf0ul_neg, Jf0ul_pos, Jf0lu
+ * (IntegratorInterface.cc:714) applied to locY, FP64 (same mode rule). */
+int plx_compute_jacobian_action_cohesive(plx_dm dm, const double *locY, double *locF, int mode);
+
+/* ---------------------------------------------------------------------------------------------
  * Reductions and BLAS-1 on device vectors, FP64 accumulation (deterministic two-pass tree).
  * Results of dot/norm are written to a device scalar and copied to *result (synchronises the
  * handle's stream). */

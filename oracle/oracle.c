@@ -781,6 +781,7 @@ int oracle_update_state(const oracle_mesh *m, int kernel_id, int subfield, doubl
 /* ------------------------------------------------------------------------------------------ */
 
 static const double quad_ref[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+#define FACE_GAUSS 0.57735026918962576451 /* 1/sqrt(3): 2x2 Gauss points on [-1,1]^2, weight 1 */
 
 /* Face geometry at a 2x2 Gauss point: basis psi[4], area element, unit normal
  * (d x/d xi  cross  d x/d eta, which the mesh orients from the negative to the positive side). */
@@ -805,7 +806,7 @@ static void face_geometry(const double X[4][3], const double xi[2], double psi[4
 int oracle_cohesive_residual(const oracle_cohesive *C, double t, const double *locX,
                              double *locF) {
     if (!C || !locX || !locF) return 1;
-    const double g = 1.0 / sqrt(3.0);
+    const double g = FACE_GAUSS;
     const double constants[6] = {C->ref_dir_1[0], C->ref_dir_1[1], C->ref_dir_1[2],
                                  C->ref_dir_2[0], C->ref_dir_2[1], C->ref_dir_2[2]};
     const int sOff[2] = {0, 6}, sOff_x[2] = {0, 0}, aOff[1] = {0}, aOff_x[1] = {0};
@@ -857,7 +858,7 @@ int oracle_cohesive_residual(const oracle_cohesive *C, double t, const double *l
  * (FaultCohesiveKin.hh:298-417), integrated with the face basis. */
 int oracle_cohesive_jacobian_action(const oracle_cohesive *C, const double *locY, double *locF) {
     if (!C || !locY || !locF) return 1;
-    const double g = 1.0 / sqrt(3.0);
+    const double g = FACE_GAUSS;
     for (int c = 0; c < C->num_cells; ++c) {
         const int32_t *neg = C->closure_neg + 4 * (size_t)c, *pos = C->closure_pos + 4 * (size_t)c;
         const int32_t *lag = C->closure_lag + 4 * (size_t)c, *fc = C->face_cone + 4 * (size_t)c;

@@ -55,6 +55,12 @@ SIGNATURES = {
     "plx_solver_setup": (_c_int, [_vp]),
     "plx_solver_set_inner": (_c_int, [_vp, _c_dbl, _c_int, _c_int]),
     "plx_solver_solve": (_c_int, [_vp, _c_int, _vp, _vp, _c_dbl, _c_int, _vp]),
+    "plx_dm_add_fault": (_c_int, [_vp, _c_int, _i32p, _i32p, _i32p, _i32p, _c_int, _dblp, _dblp]),
+    "plx_dm_set_fault_slip": (_c_int, [_vp, _dblp]),
+    "plx_dm_set_fault_slip_device": (_c_int, [_vp, _vp]),
+    "plx_dm_get_fault_info": (_c_int, [_vp, _i64p, ctypes.POINTER(_c_int)]),
+    "plx_compute_residual_cohesive": (_c_int, [_vp, _c_dbl, _vp, _vp, _c_int]),
+    "plx_compute_jacobian_action_cohesive": (_c_int, [_vp, _vp, _vp, _c_int]),
     "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
     "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
     "plx_dm_reset_event_stats": (_c_int, [_vp]),

@@ -256,7 +256,11 @@ int plx_dm_create(plx_dm *out, int cell_type, int num_cells, int num_vertices, i
         plx_dm_destroy(dm);
         return 76;
     }
-    dm->con.global_size = dm->local_size;
+    // no constrained dofs yet: the global vector is the local one (identity g2l map)
+    if (plx_dm_set_constraints(dm, 0, nullptr)) {
+        plx_dm_destroy(dm);
+        return 76;
+    }
     int bad = 0;
     const int verr = validate_geometry(dm, &bad);
     if (verr || bad) {
@@ -271,6 +275,7 @@ int plx_dm_destroy(plx_dm dm) {
     if (!dm) return 0;
     events_release(dm);
     for (KeyPlan *k : dm->keys) free_key(k);
+    if (dm->fault) free_fault(dm->fault);
     if (dm->d_coords) (void)hipFree(dm->d_coords);
     if (dm->d_closure) (void)hipFree(dm->d_closure);
     if (dm->d_cone) (void)hipFree(dm->d_cone);

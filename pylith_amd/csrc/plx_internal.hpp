@@ -96,6 +96,26 @@ struct Constraints {
     int64_t global_size = 0;
 };
 
+// Cohesive cells of one fault with prescribed slip (FaultCohesiveKin).  Each cohesive cell is a
+// quadrilateral face carrying a negative-side, a positive-side and a Lagrange-multiplier node at
+// each of its four vertices.  The Lagrange-multiplier field follows the displacement field in
+// the local vector: dof lag_offset + 3*l + k.
+struct FaultData {
+    int num_cohesive = 0, num_lag = 0;
+    int64_t lag_offset = 0;
+    double ref_dir_1[3] = {0, 0, 1}, ref_dir_2[3] = {0, 1, 0};
+    std::vector<int32_t> color_ptr;        // cells of colour k: color_cells[color_ptr[k]..]
+    std::vector<int32_t> neg_node, pos_node;  // [num_lag] split-node pair of each fault vertex
+    int32_t *d_neg = nullptr, *d_pos = nullptr, *d_lag = nullptr, *d_cone = nullptr;  // [ncoh][4]
+    int32_t *d_color_cells = nullptr;
+    int32_t *d_neg_node = nullptr, *d_pos_node = nullptr;
+    double *d_slip = nullptr;              // [num_lag][3] in the fault frame (normal, t1, t2)
+    // scalar fault-face mass matrix M_ab = integral psi_a psi_b over the fault (CSR over Lagrange
+    // nodes, the same 2x2 Gauss rule as the cohesive kernels) and its inverse diagonal
+    int32_t *d_mass_ptr = nullptr, *d_mass_col = nullptr;
+    double *d_mass_val = nullptr, *d_mass_dinv = nullptr;
+};
+
 }  // namespace plx
 
 struct plx_dm_s {
@@ -110,6 +130,7 @@ struct plx_dm_s {
     hipStream_t stream = nullptr;
     std::vector<plx::KeyPlan *> keys;
     plx::Constraints con;
+    plx::FaultData *fault = nullptr;
     double *d_scalar = nullptr;      // reduction scratch
     double *d_partials = nullptr;
     int partials_len = 0;
@@ -155,6 +176,17 @@ int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const v
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
 int launch_state_update(plx_dm dm, KeyPlan *kp, const double *u);
 int validate_geometry(plx_dm dm, int *num_bad);
+
+// Cohesive cells and split-node maps (plx_fault.hip).
+int launch_cohesive(plx_dm dm, bool jacobian, const double *x, double *f, int mode);
+enum { kFold = 0, kUnfold = 1, kClearPos = 2 };
+int launch_fold(plx_dm dm, int scalar_type, void *v, int op);
+int launch_gather_pos(plx_dm dm, const double *v, double *out);
+int launch_sync_pos(plx_dm dm, const double *jump, double *v);
+int launch_fault_mass(plx_dm dm, const double *x, double *y);
+int launch_fault_jacobi(plx_dm dm, const double *r, double *z);
+int launch_fault_slip_rhs(plx_dm dm, double *lag_out);
+void free_fault(FaultData *f);
 
 // Vector kernels (plx_vector.hip).
 int launch_l2g(plx_dm dm, int scalar_type, const void *loc, void *glob, int mode);

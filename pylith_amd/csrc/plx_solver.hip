@@ -11,6 +11,14 @@
 //   * PLX_KSP_IR:        plain iterative refinement (Richardson) with the FP32 PCG correction.
 // Vectors are local vectors; constrained dofs are held fixed (the operator is masked there).
 // Scalars of the PCG recurrences stay on the device, so an iteration needs no host round trip.
+//
+// A fault with prescribed slip (cohesive cells) turns the system into a saddle point
+// [K  B^T; B  0] [u; lambda] = [b_u; g - b_lambda].  It is solved by eliminating the constraint:
+// B u = M (u+ - u-) = g fixes the jump u+ - u- = M^-1 (g - b_lambda) across every split node pair
+// (M the fault-face mass matrix, one small PCG), so u = U w + u_jump with U copying each negative
+// node's value to its positive twin.  U^T K U is symmetric positive definite and is what the
+// Krylov methods above see (corrections w carry zeros at the positive nodes; "fold" is U^T).
+// Finally the positive-side rows give M lambda = (A u)_+ - b_+.
 #include <vector>
 
 #include "plx_element.hpp"
@@ -216,6 +224,14 @@ __global__ void __launch_bounds__(256) invert_blocks(int nnodes, const double *_
     }
 }
 
+// blocks[neg] += blocks[pos] for every split node pair (3x3 blocks)
+__global__ void __launch_bounds__(256) merge_blocks(int n, const int32_t *__restrict__ negn,
+                                                   const int32_t *__restrict__ posn, double *blocks) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 9; ++k) blocks[9 * (size_t)negn[i] + k] += blocks[9 * (size_t)posn[i] + k];
+}
+
 }  // namespace plx
 
 using namespace plx;
@@ -238,18 +254,26 @@ struct plx_solver_s {
     double inner_rtol = 1e-4;
     int inner_maxit = 2000;
     plx_solve_stats last{};
+    // fault (null-space method): jump across split nodes and Lagrange-space work vectors
+    bool fault = false;
+    double *jump = nullptr, *lr = nullptr, *lp = nullptr, *lq = nullptr, *lz = nullptr, *lx = nullptr;
 };
 
 namespace plx {
 
 static inline dim3 red_grid() { return dim3(kRedBlocks); }
 
+// y = A x on the unconstrained dofs.  With a fault x is a merged vector (zeros at positive split
+// nodes) and y = U^T A U x; x's positive entries are filled for the apply and cleared again.
 template <typename Real>
-static int masked_apply(plx_solver s, const Real *x, Real *y) {
+static int masked_apply(plx_solver s, Real *x, Real *y) {
     plx_dm dm = s->dm;
     const int st = std::is_same<Real, double>::value ? PLX_FP64 : PLX_FP32;
-    int err = launch_apply(dm, s->kp, st, true, x, y, PLX_INSERT_VALUES, -1.0);
-    if (err) return err;
+    int err = 0;
+    if (s->fault && (err = launch_fold(dm, st, x, kUnfold))) return err;
+    if ((err = launch_apply(dm, s->kp, st, true, x, y, PLX_INSERT_VALUES, -1.0))) return err;
+    if (s->fault && ((err = launch_fold(dm, st, x, kClearPos)) || (err = launch_fold(dm, st, y, kFold))))
+        return err;
     const int nc = (int)dm->con.dofs.size();
     if (nc > 0) {
         hipLaunchKernelGGL(mask_kernel<Real>, dim3((nc + 255) / 256), dim3(256), 0, dm->stream, nc,
@@ -263,6 +287,14 @@ static int read_scalar(plx_solver s, int idx, double *v) {
     PLX_HIP(s->dm, hipMemcpyAsync(v, s->scal + idx, sizeof(double), hipMemcpyDeviceToHost, s->dm->stream));
     PLX_HIP(s->dm, hipStreamSynchronize(s->dm->stream));
     return 0;
+}
+
+static int dot64n(plx_solver s, int64_t n, const double *a, const double *b, double *out) {
+    plx_dm dm = s->dm;
+    hipLaunchKernelGGL(dot_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, a, b, s->part_a);
+    hipLaunchKernelGGL(finish_start, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_a, s->scal);
+    PLX_HIP(dm, hipGetLastError());
+    return read_scalar(s, S_RZ, out);
 }
 
 static int dot64(plx_solver s, const double *a, const double *b, double *out) {
@@ -309,16 +341,18 @@ static int pcg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, const 
     return 0;
 }
 
-// True FP64 residual r = b - A x on unconstrained dofs (x carries the boundary values).
+// True FP64 residual r = b - A x on unconstrained dofs (x carries the boundary values; with a
+// fault x is the full displacement, positive nodes included, and r is folded).
 static int true_residual(plx_solver s, const double *b, const double *x, double *r) {
     plx_dm dm = s->dm;
-    int err = masked_apply<double>(s, x, r);
+    int err = launch_apply(dm, s->kp, PLX_FP64, true, x, r, PLX_INSERT_VALUES, -1.0);
     if (err) return err;
     if (b)
         hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, b, -1.0, r);
     else
         hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 0.0, r, -1.0, r);
     PLX_HIP(dm, hipGetLastError());
+    if (s->fault && (err = launch_fold(dm, PLX_FP64, r, kFold))) return err;
     const int nc = (int)dm->con.dofs.size();
     if (nc > 0) {
         hipLaunchKernelGGL(mask_kernel<double>, dim3((nc + 255) / 256), dim3(256), 0, dm->stream, nc,
@@ -326,6 +360,70 @@ static int true_residual(plx_solver s, const double *b, const double *x, double 
         PLX_HIP(dm, hipGetLastError());
     }
     return 0;
+}
+
+// out = M^-1 rhs on the fault (Lagrange space, three components), Jacobi-preconditioned CG in
+// FP64 to round-off (M is the well-conditioned fault-face mass matrix).
+static int fault_mass_solve(plx_solver s, const double *rhs, double *out) {
+    plx_dm dm = s->dm;
+    const int64_t n = 3 * (int64_t)dm->fault->num_lag;
+    int err = 0;
+    hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, n, out, 0.0);
+    hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, rhs, 0.0, s->lr);
+    if ((err = launch_fault_jacobi(dm, s->lr, s->lz))) return err;
+    hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, s->lz, 0.0, s->lp);
+    double rz = 0.0, bb = 0.0;
+    if ((err = dot64n(s, n, s->lr, s->lz, &rz)) || (err = dot64n(s, n, rhs, rhs, &bb))) return err;
+    for (int it = 0; it < 500 && bb > 0.0; ++it) {
+        double pq = 0.0, rr = 0.0, rz_new = 0.0;
+        if ((err = launch_fault_mass(dm, s->lp, s->lq)) || (err = dot64n(s, n, s->lp, s->lq, &pq))) return err;
+        if (pq <= 0.0) break;
+        const double alpha = rz / pq;
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, alpha, s->lp, 1.0, out);
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, -alpha, s->lq, 1.0, s->lr);
+        if ((err = dot64n(s, n, s->lr, s->lr, &rr))) return err;
+        if (rr <= 1e-30 * bb) break;
+        if ((err = launch_fault_jacobi(dm, s->lr, s->lz)) || (err = dot64n(s, n, s->lr, s->lz, &rz_new)))
+            return err;
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, s->lz, rz_new / rz, s->lp);
+        rz = rz_new;
+    }
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
+// Jump across the split nodes: M jump = g - b_lambda, g = integral psi d (the slip forcing).
+static int fault_jump(plx_solver s, const double *b) {
+    plx_dm dm = s->dm;
+    const int64_t n = 3 * (int64_t)dm->fault->num_lag;
+    int err = launch_fault_slip_rhs(dm, s->lx);  // lx = -g
+    if (err) return err;
+    if (b)
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, -1.0,
+                           b + dm->fault->lag_offset, -1.0, s->lx);
+    else
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 0.0, s->lx, -1.0, s->lx);
+    PLX_HIP(dm, hipGetLastError());
+    return fault_mass_solve(s, s->lx, s->jump);
+}
+
+// Lagrange multipliers from the positive-side rows: M lambda = (A u)_+ - b_+.
+static int fault_lambda(plx_solver s, const double *b, double *x) {
+    plx_dm dm = s->dm;
+    const int64_t n = 3 * (int64_t)dm->fault->num_lag;
+    int err = launch_apply(dm, s->kp, PLX_FP64, true, x, s->t64, PLX_INSERT_VALUES, -1.0);
+    if (err || (err = launch_gather_pos(dm, s->t64, s->lx))) return err;
+    if (b) {
+        if ((err = launch_gather_pos(dm, b, s->lq))) return err;
+        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, -1.0, s->lq, 1.0, s->lx);
+        PLX_HIP(dm, hipGetLastError());
+    }
+    return fault_mass_solve(s, s->lx, x + dm->fault->lag_offset);
+}
+
+// Re-impose u+ = u- + jump after an update of the negative-side values.
+static int sync_x(plx_solver s, double *x) {
+    return s->fault ? launch_sync_pos(s->dm, s->jump, x) : 0;
 }
 
 // Inner preconditioner: out (FP64) ~= A^{-1} in (FP64) by FP32 PCG from a zero guess.
@@ -366,9 +464,24 @@ int plx_solver_create(plx_solver *out, plx_dm dm, int label_value, int part) {
     plx_solver s = new plx_solver_s();
     s->dm = dm;
     s->kp = kp;
-    s->n = dm->local_size;
+    s->n = 3 * (int64_t)dm->num_nodes;  // the displacement block; Lagrange dofs are eliminated
     s->nnodes = dm->num_nodes;
     const size_t n = (size_t)s->n;
+    for (int32_t d : dm->con.dofs)
+        if (d >= s->n) {
+            delete s;
+            return set_error(dm, 1, "constrained Lagrange-multiplier dofs are not supported");
+        }
+    if (dm->fault) {
+        s->fault = true;
+        std::vector<char> is_pos(dm->num_nodes, 0);
+        for (int32_t p : dm->fault->pos_node) is_pos[p] = 1;
+        for (int32_t d : dm->con.dofs)
+            if (is_pos[d / 3]) {
+                delete s;
+                return set_error(dm, 1, "constrain the negative-side twin of a split fault node, not the positive one");
+            }
+    }
     bool ok = hipMalloc(&s->Binv64, 9 * (size_t)s->nnodes * sizeof(double)) == hipSuccess &&
               hipMalloc(&s->Binv32, 9 * (size_t)s->nnodes * sizeof(float)) == hipSuccess &&
               hipMalloc(&s->scal, 16 * sizeof(double)) == hipSuccess &&
@@ -378,6 +491,11 @@ int plx_solver_create(plx_solver *out, plx_dm dm, int label_value, int part) {
         ok = ok && hipMalloc(v, n * sizeof(float)) == hipSuccess;
     for (double **v : {&s->r64, &s->p64, &s->q64, &s->z64, &s->t64})
         ok = ok && hipMalloc(v, n * sizeof(double)) == hipSuccess;
+    if (s->fault) {
+        const size_t nl = 3 * (size_t)dm->fault->num_lag;
+        for (double **v : {&s->jump, &s->lr, &s->lp, &s->lq, &s->lz, &s->lx})
+            ok = ok && hipMalloc(v, nl * sizeof(double)) == hipSuccess;
+    }
     if (!ok) {
         plx_solver_destroy(s);
         return set_error(dm, 76, "solver allocation failed");
@@ -391,6 +509,8 @@ int plx_solver_destroy(plx_solver s) {
     void *ptrs[] = {s->Binv64, s->Binv32, s->scal, s->part_a, s->part_b, s->x32, s->r32, s->p32,
                     s->q32,    s->z32,    s->r64,  s->p64,    s->q64,    s->z64, s->t64};
     for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (double *p : {s->jump, s->lr, s->lp, s->lq, s->lz, s->lx})
         if (p) (void)hipFree(p);
     for (double *p : s->V) (void)hipFree(p);
     for (double *p : s->Z) (void)hipFree(p);
@@ -407,8 +527,18 @@ int plx_solver_setup(plx_solver s) {
     PLX_HIP(dm, hipMemsetAsync(blocks, 0, 9 * (size_t)s->nnodes * sizeof(double), dm->stream));
     int err = launch_pbdiag(dm, s->kp, blocks);
     if (err) return err;
-    std::vector<uint8_t> hmask(dm->local_size, 0);
+    std::vector<uint8_t> hmask(s->n, 0);
     for (int32_t d : dm->con.dofs) hmask[d] = 1;
+    if (s->fault) {
+        // merged point blocks: the negative node carries both twins' diagonal blocks (the two
+        // sides share no cell); positive nodes are outside the merged space (identity)
+        for (int32_t p : dm->fault->pos_node)
+            for (int c = 0; c < 3; ++c) hmask[3 * (size_t)p + c] = 1;
+        const int nl = dm->fault->num_lag;
+        hipLaunchKernelGGL(merge_blocks, dim3((nl + 255) / 256), dim3(256), 0, dm->stream, nl,
+                           dm->fault->d_neg_node, dm->fault->d_pos_node, blocks);
+        PLX_HIP(dm, hipGetLastError());
+    }
     PLX_HIP(dm, hipMalloc(&mask, hmask.size()));
     PLX_HIP(dm, hipMemcpyAsync(mask, hmask.data(), hmask.size(), hipMemcpyHostToDevice, dm->stream));
     hipLaunchKernelGGL(invert_blocks, dim3((s->nnodes + 255) / 256), dim3(256), 0, dm->stream, s->nnodes,
@@ -435,6 +565,8 @@ int plx_solver_solve(plx_solver s, int method, const double *b, double *x, doubl
     plx_solve_stats st{};
     int err = 0;
     const int nc = (int)dm->con.dofs.size();
+    // fault: fix the jump across the split nodes, then x = U w + u_jump throughout
+    if (s->fault && ((err = fault_jump(s, b)) || (err = sync_x(s, x)))) return err;
     // r = b - A x (true FP64 residual)
     if ((err = true_residual(s, b, x, s->r64))) return err;
     double rr0 = 0.0;
@@ -451,6 +583,7 @@ int plx_solver_solve(plx_solver s, int method, const double *b, double *x, doubl
             return err;
         hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, d, 1.0, x);
         PLX_HIP(dm, hipGetLastError());
+        if ((err = sync_x(s, x))) return err;
         st.outer_iterations = its;
         st.applies_fp64 = its + 1;
     } else if (method == PLX_KSP_IR) {
@@ -460,6 +593,7 @@ int plx_solver_solve(plx_solver s, int method, const double *b, double *x, doubl
             if ((err = inner_solve(s, s->r64, s->t64, &its))) return err;
             st.inner_iterations += its;
             hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, s->t64, 1.0, x);
+            if ((err = sync_x(s, x))) return err;
             if ((err = true_residual(s, b, x, s->r64))) return err;
             if ((err = dot64(s, s->r64, s->r64, &rr))) return err;
             st.outer_iterations++;
@@ -542,6 +676,7 @@ int plx_solver_solve(plx_solver s, int method, const double *b, double *x, doubl
             for (int i = 0; i < j; ++i)
                 hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, y[i], s->Z[i], 1.0, x);
             PLX_HIP(dm, hipGetLastError());
+            if ((err = sync_x(s, x))) return err;
             // restart from the true residual
             if ((err = true_residual(s, b, x, s->r64))) return err;
             double rr = 0.0;
@@ -557,6 +692,7 @@ int plx_solver_solve(plx_solver s, int method, const double *b, double *x, doubl
     double rr = 0.0;
     if ((err = dot64(s, s->r64, s->r64, &rr))) return err;
     st.final_residual = std::sqrt(rr);
+    if (s->fault && (err = fault_lambda(s, b, x))) return err;
     st.converged = st.final_residual <= rtol * r0 * (method == PLX_KSP_CG ? 10.0 : 1.0) ? 1 : 0;
     s->last = st;
     if (stats) *stats = st;

@@ -128,6 +128,41 @@ class DM:
         self._check(self.lib.plx_compute_jacobian_pbdiag(self.h, label_value, part, t, s_tshift,
                                                          _ptr(locX), _ptr(blocks)))
 
+    # --- fault (cohesive cells, prescribed slip) ----------------------------------------------
+    def add_fault(self, fault):
+        """Cohesive cells of `fault` (meshgen.Fault layout); the Lagrange-multiplier dofs follow
+        the displacement dofs in the local vector."""
+        arrs = [_i32(getattr(fault, k)) for k in ("closure_neg", "closure_pos", "closure_lag", "face_cone")]
+        r1, r1p = _f64(fault.ref_dir_1)
+        r2, r2p = _f64(fault.ref_dir_2)
+        self._check(self.lib.plx_dm_add_fault(self.h, int(arrs[0][0].shape[0]), *[a[1] for a in arrs],
+                                              int(fault.num_lag), r1p, r2p))
+
+    def fault_info(self):
+        off, n = ctypes.c_int64(), ctypes.c_int()
+        self._check(self.lib.plx_dm_get_fault_info(self.h, ctypes.byref(off), ctypes.byref(n)))
+        return off.value, n.value
+
+    def set_fault_slip(self, slip):
+        """Slip [num_lag, 3] (opening, left-lateral, reverse): a host array or a CUDA tensor."""
+        if isinstance(slip, torch.Tensor) and slip.is_cuda:
+            if slip.dtype != torch.float64:
+                raise PlxError("slip must be float64")
+            self._check(self.lib.plx_dm_set_fault_slip_device(self.h, _ptr(slip.contiguous())))
+        else:
+            s, sp = _f64(slip)
+            self._check(self.lib.plx_dm_set_fault_slip(self.h, sp))
+
+    def compute_residual_cohesive(self, locX, locF, mode=L.ADD_VALUES, t=0.0):
+        if locX.dtype != torch.float64 or locF.dtype != torch.float64:
+            raise PlxError("cohesive cells are computed in float64")
+        self._check(self.lib.plx_compute_residual_cohesive(self.h, t, _ptr(locX), _ptr(locF), mode))
+
+    def compute_jacobian_action_cohesive(self, locY, locF, mode=L.ADD_VALUES):
+        if locY.dtype != torch.float64 or locF.dtype != torch.float64:
+            raise PlxError("cohesive cells are computed in float64")
+        self._check(self.lib.plx_compute_jacobian_action_cohesive(self.h, _ptr(locY), _ptr(locF), mode))
+
     # --- field scatters ------------------------------------------------------------------------
     def local_to_global(self, loc, glob, mode=L.ADD_VALUES):
         self._check(self.lib.plx_local_to_global(self.h, _stype(loc), _ptr(loc), _ptr(glob), mode))

@@ -1,0 +1,169 @@
+"""Fault with prescribed slip on the device: parity of the cohesive-cell residual / Jacobian action
+with the oracle (FP64, 1e-12 relative), and the constraint-eliminating solve against a dense
+direct solve of the full saddle-point system and against the two-block rigid-motion solution
+(tests/fullscale/linearelasticity/faults-3d/twoblocks_soln.py)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from pylith_amd import _lib as L  # noqa: E402
+from pylith_amd import meshgen as mg  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+F1, JF3 = 1, 2
+TOL64 = 1e-12
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def faulted(n, i_fault, shear=None, lo=(-2000.0, 0.0, 0.0), hi=(2000.0, 3000.0, 3000.0), brick=2):
+    m, f = mg.hex_box_with_fault(n, lo, hi, i_fault, brick=brick, shear=shear)
+    aux, sizes = mg.elastic_aux(m.num_cells)
+    from pylith_amd.dm import DM
+    dm = DM(m)
+    dm.add_residual(0, 0, 0, F1)
+    dm.add_jacobian(0, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(0, 0, aux, sizes)
+    dm.add_fault(f)
+    return m, f, aux, sizes, dm
+
+
+def test_cohesive_residual_and_jacobian_parity(oracle):
+    m, f, aux, sizes, dm = faulted((6, 5, 4), 3, shear=0.3)
+    nu = 3 * m.num_nodes
+    off, nl = dm.fault_info()
+    assert off == nu and nl == f.num_lag and dm.sizes()[0] == nu + 3 * nl
+    print(f"[fault parity] nodes={m.num_nodes} lag={nl} cohesive={f.num_cohesive}", flush=True)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(nu + 3 * nl)
+    x[nu:] *= 1e7
+    slip = rng.standard_normal((nl, 3))
+    C = oracle.Cohesive(m, f, slip, nu)
+    ref = C.residual(x)
+    dm.set_fault_slip(slip)
+    out = torch.zeros(nu + 3 * nl, dtype=torch.float64, device="cuda")
+    dm.compute_residual_cohesive(dev(x), out, mode=L.INSERT_VALUES)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert relerr(got[:nu], ref[:nu]) <= TOL64
+    assert relerr(got[nu:], ref[nu:]) <= TOL64
+    # ADD mode on top of existing values; slip from a device array
+    dm.set_fault_slip(dev(slip))
+    out2 = torch.full_like(out, 2.0)
+    dm.compute_residual_cohesive(dev(x), out2, mode=L.ADD_VALUES)
+    torch.cuda.synchronize()
+    assert relerr(out2.cpu().numpy() - 2.0, ref) <= 10 * TOL64
+    # Jacobian action (no slip term)
+    jref = C.jacobian_action(x)
+    jv = torch.zeros_like(out)
+    dm.compute_jacobian_action_cohesive(dev(x), jv, mode=L.INSERT_VALUES)
+    torch.cuda.synchronize()
+    assert relerr(jv.cpu().numpy()[nu:], jref[nu:]) <= TOL64
+    assert relerr(jv.cpu().numpy()[:nu], jref[:nu]) <= TOL64
+    # the domain residual stays on the displacement block: total = domain + cohesive
+    P = oracle.Problem(m, aux, sizes)
+    tot = torch.zeros_like(out)
+    dm.compute_residual(0, 0, dev(x), tot, mode=L.INSERT_VALUES)
+    dm.compute_residual_cohesive(dev(x), tot, mode=L.ADD_VALUES)
+    torch.cuda.synchronize()
+    tref = ref.copy()
+    tref[:nu] += P.residual(x[:nu], F1)
+    assert relerr(tot.cpu().numpy(), tref) <= TOL64
+
+
+def dense_saddle_solution(oracle, m, f, aux, sizes, slip, dofs, vals):
+    """Direct solve of the full linear system F(x) = 0 (domain + cohesive rows) with the
+    constrained dofs fixed, assembled column by column from the oracle."""
+    nu = 3 * m.num_nodes
+    n = nu + 3 * f.num_lag
+    P = oracle.Problem(m, aux, sizes)
+    C0 = oracle.Cohesive(m, f, np.zeros((f.num_lag, 3)), nu)
+    J = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        col = C0.jacobian_action(e)
+        col[:nu] += P.residual(e[:nu], F1)
+        J[:, i] = col
+    F0 = oracle.Cohesive(m, f, slip, nu).residual(np.zeros(n))
+    free = np.setdiff1d(np.arange(n), dofs)
+    x = np.zeros(n)
+    x[dofs] = vals
+    rhs = -F0 - J @ x
+    x[free] = np.linalg.solve(J[np.ix_(free, free)], rhs[free])
+    return x
+
+
+@pytest.mark.parametrize("method", [L.KSP_CG, L.KSP_FGMRES_IR, L.KSP_IR])
+def test_fault_solve_matches_dense_saddle_point(oracle, method):
+    from pylith_amd.dm import Solver
+    m, f, aux, sizes, dm = faulted((4, 3, 3), 2, shear=0.25)
+    nu = 3 * m.num_nodes
+    # -x face fixed, +x face (positive side of the fault) u_y = 0; faces are sheared planes
+    x0 = m.node_coords[:, 0] - 0.25 * m.node_coords[:, 2]
+    neg = np.nonzero(np.isclose(x0, -2000.0))[0]
+    pos = np.nonzero(np.isclose(x0, 2000.0))[0]
+    dofs = np.sort(np.concatenate([3 * neg, 3 * neg + 1, 3 * neg + 2, 3 * pos + 1])).astype(np.int32)
+    vals = np.zeros(dofs.size)
+    dm.set_constraints(dofs)
+    slip = mg.tapered_patch_slip(f, (1500.0, 1500.0), 600.0, 1.5, component=1, taper=800.0)
+    slip[:, 2] = 0.5 * slip[:, 1]
+    dm.set_fault_slip(slip)
+    ref = dense_saddle_solution(oracle, m, f, aux, sizes, slip, dofs, vals)
+    print(f"[fault solve] method={method} n={ref.size}", flush=True)
+    x = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
+    dm.insert_boundary_values(dev(vals), x)
+    st = Solver(dm).solve(x, method=method, rtol=1e-12, maxit=2000)
+    torch.cuda.synchronize()
+    got = x.cpu().numpy()
+    assert st["converged"], st
+    assert relerr(got[:nu], ref[:nu]) <= 1e-9
+    assert relerr(got[nu:], ref[nu:]) <= 1e-8
+    # the full residual (domain + cohesive rows) vanishes at the device solution
+    P = oracle.Problem(m, aux, sizes)
+    F = oracle.Cohesive(m, f, slip, nu).residual(got)
+    F[:nu] += P.residual(got[:nu], F1)
+    F[dofs] = 0.0
+    scale = np.abs(P.residual(got[:nu], F1)).max()  # the fault-node forces
+    assert np.abs(F[:nu]).max() <= 1e-8 * scale
+
+
+def test_two_blocks_rigid_motion(oracle):
+    from pylith_amd.dm import Solver
+    m, f, aux, sizes, dm = faulted((8, 6, 6), 4, lo=(-4000.0, 0.0, 0.0), hi=(4000.0, 6000.0, 6000.0))
+    nu = 3 * m.num_nodes
+    nrm, t1, t2 = mg.fault_normal_frame(f, m.coords)
+    slip = np.zeros((f.num_lag, 3))
+    slip[:, 1] = 2.0
+    d = 2.0 * t1
+    x_face = m.node_coords[:, 0]
+    fixed = np.nonzero(np.isclose(x_face, -4000.0))[0]
+    moved = np.nonzero(np.isclose(x_face, 4000.0))[0]
+    dofs = np.concatenate([3 * fixed[:, None] + np.arange(3), 3 * moved[:, None] + np.arange(3)]).ravel()
+    vals = np.concatenate([np.zeros((fixed.size, 3)), np.tile(d, (moved.size, 1))]).ravel()
+    order = np.argsort(dofs)
+    dofs, vals = dofs[order], vals[order]
+    dm.set_constraints(dofs)
+    dm.set_fault_slip(slip)
+    x = torch.zeros(nu + 3 * f.num_lag, dtype=torch.float64, device="cuda")
+    dm.insert_boundary_values(dev(vals), x)
+    st = Solver(dm).solve(x, method=L.KSP_FGMRES_IR, rtol=1e-12, maxit=200)
+    torch.cuda.synchronize()
+    got = x.cpu().numpy()
+    u = got[:nu].reshape(-1, 3)
+    pos_side = np.zeros(m.num_nodes, bool)
+    for c in range(m.num_cells):
+        if m.coords[m.cone[c]].mean(0)[0] > 0:
+            pos_side[m.closure[c]] = True
+    exact = np.where(pos_side[:, None], d[None, :], 0.0)
+    assert st["converged"], st
+    assert np.abs(u - exact).max() <= 1e-9 * 2.0
+    # zero traction on the fault: lambda vanishes relative to mu * slip / h
+    assert np.abs(got[nu:]).max() <= 1e-9 * 2.25e10 * 2.0 / 1000.0
