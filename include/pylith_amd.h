@@ -108,13 +108,28 @@ int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, 
 /* DMSetAuxiliaryVec (IntegratorDomain.cc:447).  aux is a HOST array laid out like PyLith's
  * auxiliary local vector: per cell (aux_per_qp = 0) or per cell and quadrature point
  * (aux_per_qp = 1), subfields concatenated in order with the given component counts.
- * The values are captured (copied to the device) at this call. */
+ * The values are captured (copied to the device) at this call.
+ * Isotropic linear elasticity: [..., shear_modulus, bulk_modulus] (IsotropicLinearElasticity.hh:
+ * 116-117).  Generalized Maxwell (Q1, per quadrature point): [..., shear_modulus, bulk_modulus,
+ * maxwell_time(3), shear_modulus_ratio(3), viscous_strain(18), total_strain(6)]
+ * (IsotropicLinearGenMaxwell.hh:109-165); moduli, times and ratios uniform within a cell. */
 int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux,
                          const int32_t *aux_sizes, int aux_per_qp, const double *aux);
 
-/* PetscDSSetConstants via Integrator::_setKernelConstants (Integrator.cc:264-288). */
+/* PetscDSSetConstants via Integrator::_setKernelConstants (Integrator.cc:264-288).
+ * Generalized Maxwell: constants[0] = dt (IsotropicLinearGenMaxwell.cc:211-212); setting it
+ * recomputes the Jacobian's effective shear modulus and the state stress of the step. */
 int plx_dm_set_constants(plx_dm dm, int label_value, int part, int num_constants,
                          const double *constants);
+
+/* State variables: IntegratorDomain::updateStateVars -> _updateStateVars
+ * (IntegratorDomain.cc:555-590, DMProjectFieldLocal of the viscous and total strain kernels,
+ * materials/IsotropicLinearGenMaxwell.cc:219-244) at the solution locX (FP64, device) of the
+ * step just solved.  No-op for keys without state variables. */
+int plx_update_state_vars(plx_dm dm, int label_value, int part, double t, const double *locX);
+/* Copy of the key's auxiliary field in the layout given to plx_dm_set_auxiliary, with the
+ * current state variables (HOST array; synchronises the stream). */
+int plx_dm_get_auxiliary(plx_dm dm, int label_value, int part, double *aux);
 
 /* ---------------------------------------------------------------------------------------------
  * DMPlexComputeResidualByKey(dm, key, cellsIS, t, locX, locX_t, t, locF)

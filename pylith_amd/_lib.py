@@ -61,6 +61,8 @@ SIGNATURES = {
     "plx_dm_get_fault_info": (_c_int, [_vp, _i64p, ctypes.POINTER(_c_int)]),
     "plx_compute_residual_cohesive": (_c_int, [_vp, _c_dbl, _vp, _vp, _c_int]),
     "plx_compute_jacobian_action_cohesive": (_c_int, [_vp, _vp, _vp, _c_int]),
+    "plx_update_state_vars": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _vp]),
+    "plx_dm_get_auxiliary": (_c_int, [_vp, _c_int, _c_int, _dblp]),
     "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
     "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
     "plx_dm_reset_event_stats": (_c_int, [_vp]),

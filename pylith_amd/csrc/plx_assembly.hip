@@ -52,6 +52,7 @@ struct ApplyArgs {
     const int32_t *tile_list;  // tiles this launch covers (nullptr: all, in order)
     int num_list;
     const void *records;       // packed 16-bit local closure (+ cone) per key cell
+    const double *prestress;   // Q1: per-point stress added to sigma, [6][8][stride] (nullptr: none)
 };
 
 // Cell solution values read from the tile's LDS buffer (component c, local basis b).
@@ -258,7 +259,8 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
                                 U[d][l] = sU[3 * v + d];
                             }
                         }
-                        q1_element<Real>(X, U, lam, mu, rl);
+                        const double *pre = A.prestress ? A.prestress + cur.c0 + tid + i * NT : nullptr;
+                        q1_element<Real>(X, U, lam, mu, rl, pre, (size_t)A.p.stride, (Real)A.scale);
                     }
 #pragma unroll
                     for (int l = 0; l < 8; ++l)
@@ -376,9 +378,11 @@ static TilePlanDev plan_dev(plx_dm dm, const KeyPlan *kp) {
 }
 
 template <typename Real>
-static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode, double scale) {
+static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode, double scale,
+                       const double *prestress) {
     ApplyArgs A;
     A.scale = scale;
+    A.prestress = prestress;
     A.p = plan_dev(dm, kp);
     A.color = kp->d_color;
     A.tile_ncolor = kp->d_tile_ncolor;
@@ -389,7 +393,8 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     A.mode = mode;
     A.records = kp->d_records;
     const size_t lds = (size_t)kp->max_verts * 3 * sizeof(double) + (size_t)kp->max_nodes * 6 * sizeof(Real);
-    const bool linear = dm->algorithm != PLX_ALGO_QUADRATURE_POINTS;
+    // a per-point prestress needs the quadrature-point path
+    const bool linear = dm->algorithm != PLX_ALGO_QUADRATURE_POINTS && !prestress;
     // AUTO: LDS atomics for FP64 (fast native ds_add_f64), colour phases for FP32
     const bool atomic = dm->accumulate == PLX_ACCUMULATE_LDS_ATOMIC ||
                         (dm->accumulate == PLX_ACCUMULATE_AUTO && std::is_same<Real, double>::value);
@@ -452,14 +457,18 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     return 0;
 }
 
+// Residual and Jacobian action share the element operator: for isotropic linear elasticity
+// J v = F(v); for generalized Maxwell F(u) = J u + (the state term), with J the elastic operator
+// at the effective shear modulus (the residual adds the per-point state stress).
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u, void *r,
                  int mode, double scale) {
-    (void)jacobian;  // linear isotropic elasticity: J v uses the same moduli as the residual
     int err = ensure_tables(dm);
     if (err) return err;
     if (mode != PLX_ADD_VALUES && mode != PLX_INSERT_VALUES) return set_error(dm, 1, "bad insert mode");
-    if (scalar_type == PLX_FP64) return apply_typed<double>(dm, kp, u, r, mode, scale);
-    if (scalar_type == PLX_FP32) return apply_typed<float>(dm, kp, u, r, mode, scale);
+    const double *pre = (!jacobian && kp->genmax) ? kp->d_prestress : nullptr;
+    if (pre && scalar_type != PLX_FP64) return set_error(dm, 1, "the viscoelastic residual is FP64");
+    if (scalar_type == PLX_FP64) return apply_typed<double>(dm, kp, u, r, mode, scale, pre);
+    if (scalar_type == PLX_FP32) return apply_typed<float>(dm, kp, u, r, mode, scale, pre);
     return set_error(dm, 1, "bad scalar type");
 }
 

@@ -39,7 +39,7 @@ static void free_plan_arrays(KeyPlan *k) {
                     k->d_lclosure,    k->d_lcone,         k->d_shared_node,   k->d_color,
                     k->d_tile_ncolor,   k->d_records,       k->d_shared_ptr,    k->d_tiles_affine,  k->d_tiles_general,
                     k->d_shared_slot, k->d_halo,          k->d_mat64,         k->d_mat32,
-                    k->d_state};
+                    k->d_state,       k->d_gm_mat,        k->d_prestress};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     KeyPlan fresh;
@@ -150,9 +150,10 @@ static int repack_aux(plx_dm dm, KeyPlan *kp) {
     const int per = kp->aux_per_qp ? dm->nq : 1;
     const int n = (int)kp->cells.size();
     const bool genmax = kp->f1 == PLX_F1V_GENMAXWELL_3D || kp->jf3 == PLX_JF3VU_GENMAXWELL_3D;
-    if (genmax && num_aux < 7) return set_error(dm, 1, "generalized Maxwell needs 7 aux subfields");
-    const int i_mu = genmax ? num_aux - 6 : num_aux - 2;
-    const int i_k = genmax ? num_aux - 5 : num_aux - 1;
+    if (genmax) return genmax_capture(dm, kp);
+    kp->genmax = false;
+    const int i_mu = num_aux - 2;
+    const int i_k = num_aux - 1;
     std::vector<double> m64(2 * (size_t)n);
     std::vector<float> m32(2 * (size_t)n);
     for (int i = 0; i < n; ++i) {
@@ -372,7 +373,7 @@ int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const doub
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp) return set_error(dm, 1, "unknown key");
     kp->constants.assign(constants, constants + n);
-    return 0;
+    return genmax_refresh(dm, kp);
 }
 
 int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux, const int32_t *aux_sizes,
@@ -401,8 +402,10 @@ int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, 
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp || !kp->f1) return set_error(dm, 1, "no residual kernels registered for key");
     if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
-    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D)
+    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D && kp->f1 != PLX_F1V_GENMAXWELL_3D)
         return set_error(dm, 1, "residual kernel not supported by this entry point");
+    if ((kp->f1 == PLX_F1V_GENMAXWELL_3D) != kp->genmax)
+        return set_error(dm, 1, "residual kernel and auxiliary layout disagree");
     return launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
 }
 
@@ -417,8 +420,8 @@ int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp || !kp->jf3) return set_error(dm, 1, "no Jacobian kernels registered for key");
     if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
-    if (kp->jf3 != PLX_JF3VU_ISOLINEAR_ELASTICITY_3D)
-        return set_error(dm, 1, "Jacobian kernel not supported by this entry point");
+    if ((kp->jf3 == PLX_JF3VU_GENMAXWELL_3D) != kp->genmax)
+        return set_error(dm, 1, "Jacobian kernel and auxiliary layout disagree");
     return launch_apply(dm, kp, scalar_type, true, locY, locF, mode);
 }
 

@@ -107,6 +107,33 @@ __host__ __device__ inline void iso_flux(const Real G[9], const Real Ji[9], Real
                            s[3 * c + 2] * Ji[3 * k + 2];
 }
 
+// iso_flux plus a given stress ps (vector order xx, yy, zz, xy, yz, xz) added to sigma.
+template <typename Real>
+__host__ __device__ inline void iso_flux_pre(const Real G[9], const Real Ji[9], Real lam, Real mu,
+                                             Real scale, const Real ps[6], Real P[9]) {
+    Real g[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            g[3 * c + d] = G[3 * c + 0] * Ji[0 * 3 + d] + G[3 * c + 1] * Ji[1 * 3 + d] +
+                           G[3 * c + 2] * Ji[2 * 3 + d];
+    const Real tr = g[0] + g[4] + g[8];
+    Real s[9];
+    s[0] = -scale * (lam * tr + Real(2) * mu * g[0] + ps[0]);
+    s[4] = -scale * (lam * tr + Real(2) * mu * g[4] + ps[1]);
+    s[8] = -scale * (lam * tr + Real(2) * mu * g[8] + ps[2]);
+    s[1] = s[3] = -scale * (mu * (g[1] + g[3]) + ps[3]);
+    s[5] = s[7] = -scale * (mu * (g[5] + g[7]) + ps[4]);
+    s[2] = s[6] = -scale * (mu * (g[2] + g[6]) + ps[5]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            P[3 * c + k] = s[3 * c + 0] * Ji[3 * k + 0] + s[3 * c + 1] * Ji[3 * k + 1] +
+                           s[3 * c + 2] * Ji[3 * k + 2];
+}
+
 // --------------------------------------------------------------------------------------------
 // Q1 hexahedron, 2x2x2 Gauss, sum factorization.  Lexicographic corner index l = i + 2j + 4k
 // maps to PyLith's vertex order through kLex.
@@ -191,9 +218,13 @@ __host__ __device__ inline void q1_ref_grad_t(const Real SX[2][2], const Real SY
 }
 
 // Element vector of one Q1 cell (lexicographic corners): re[c][l].
+// pre (optional): a stress added to sigma at each quadrature point q = qi + 2 qj + 4 qk, six
+// components (xx, yy, zz, xy, yz, xz), component k of point q at pre[(8 k + q) * pstride], scaled
+// by prescale -- the state-dependent part of a viscoelastic stress.
 template <typename Real>
 __host__ __device__ inline void q1_element(const double X[3][8], const Real U[3][8], Real lam,
-                                           Real mu, Real re[3][8]) {
+                                           Real mu, Real re[3][8], const double *pre = nullptr,
+                                           size_t pstride = 0, Real prescale = Real(0)) {
     double jX[3][2][2], jY[3][2][2], jZ[3][2][2];
 #pragma unroll
     for (int d = 0; d < 3; ++d) q1_ref_grad<double>(X[d], jX[d], jY[d], jZ[d]);
@@ -230,7 +261,15 @@ __host__ __device__ inline void q1_element(const double X[3][8], const Real U[3]
                     G[3 * c + 1] = uY[c][qi][qk];
                     G[3 * c + 2] = uZ[c][qi][qj];
                 }
-                iso_flux<Real>(G, Ji, lam, mu, (Real)det, P);
+                if (pre) {
+                    const int q = qi + 2 * qj + 4 * qk;
+                    Real ps[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) ps[k] = prescale * (Real)pre[(size_t)(8 * k + q) * pstride];
+                    iso_flux_pre<Real>(G, Ji, lam, mu, (Real)det, ps, P);
+                } else {
+                    iso_flux<Real>(G, Ji, lam, mu, (Real)det, P);
+                }
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     SX[c][qj][qk] += P[3 * c + 0];

@@ -73,8 +73,12 @@ struct KeyPlan {
     bool has_aux = false;
     double *d_mat64 = nullptr;       // [2][stride]: lambda, mu (elastic part)
     float *d_mat32 = nullptr;        // same in FP32
-    double *d_state = nullptr;       // per-qp rheology state (GenMaxwell), SoA [comp][stride*nq]
-    int state_ncomp = 0;
+    // generalized Maxwell (Q1, aux per quadrature point): per-cell material, per-point state and
+    // the per-point state stress of the current time step, SoA over key cells (tile order)
+    bool genmax = false;
+    double *d_gm_mat = nullptr;      // [8][stride]: mu, K, maxwell_time[3], shear_modulus_ratio[3]
+    double *d_state = nullptr;       // [24][8][stride]: viscous_strain[3][6], total_strain[6]
+    double *d_prestress = nullptr;   // [6][8][stride]
     std::vector<double> aux_host;    // captured aux (original layout) for state kernels
     std::vector<double> constants;
 };
@@ -174,7 +178,11 @@ void events_release(plx_dm dm);
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
                  void *r, int mode, double scale = 1.0);
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
-int launch_state_update(plx_dm dm, KeyPlan *kp, const double *u);
+// Generalized Maxwell state (plx_genmax.hip).
+int launch_genmax_prestress(plx_dm dm, KeyPlan *kp);
+int launch_genmax_update(plx_dm dm, KeyPlan *kp, const double *u);
+int genmax_capture(plx_dm dm, KeyPlan *kp);
+int genmax_refresh(plx_dm dm, KeyPlan *kp);
 int validate_geometry(plx_dm dm, int *num_bad);
 
 // Cohesive cells and split-node maps (plx_fault.hip).

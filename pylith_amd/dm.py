@@ -112,6 +112,17 @@ class DM:
         c, cp = _f64(constants)
         self._check(self.lib.plx_dm_set_constants(self.h, label_value, part, c.size, cp))
 
+    def update_state_vars(self, label_value, part, locX, t=0.0):
+        if locX.dtype != torch.float64:
+            raise PlxError("state variables are updated from a float64 solution")
+        self._check(self.lib.plx_update_state_vars(self.h, label_value, part, t, _ptr(locX)))
+
+    def get_auxiliary(self, label_value, part, like):
+        """Current auxiliary field (with state variables) in the layout of `like` (host array)."""
+        out = np.zeros_like(np.ascontiguousarray(like, np.float64))
+        self._check(self.lib.plx_dm_get_auxiliary(self.h, label_value, part, out.ctypes.data_as(L._dblp)))
+        return out
+
     # --- assembly ----------------------------------------------------------------------------
     def compute_residual(self, label_value, part, locX, locF, mode=L.ADD_VALUES, t=0.0,
                          locX_t=None):

@@ -183,6 +183,41 @@ def elastic_aux(num_cells, density=2500.0, vs=3000.0, vp=5292.0, seed=None, vs_s
     return np.ascontiguousarray(np.stack([rho, mu, K], 1)), np.array([1, 1, 1], np.int32)
 
 
+YEAR = 365.25 * 24.0 * 3600.0
+GENMAX_SIZES = np.array([1, 1, 1, 3, 3, 18, 6], np.int32)
+
+
+def genmax_aux(num_cells, nq=8, density=2500.0, vs=3000.0, vp=5292.0,
+               viscosity=(9.46728e17, 4.73364e17, 1.893456e18), ratios=(0.25, 0.25, 0.25),
+               seed=None, log_eta_range=None, viscous_strain=None, total_strain=None):
+    """Generalized Maxwell aux field per cell and quadrature point, [density, shear_modulus,
+    bulk_modulus, maxwell_time(3), shear_modulus_ratio(3), viscous_strain(18), total_strain(6)]
+    (IsotropicLinearGenMaxwell.hh:109-165; maxwell_time = viscosity / (shear_modulus * ratio),
+    materials/Query.cc).  ``log_eta_range`` = (lo, hi) draws each cell's three viscosities
+    log-uniformly (seeded); viscous/total strain give the initial state (6 or 18 / 6 values,
+    broadcast)."""
+    rho = np.full(num_cells, density)
+    mu, K = moduli(rho, np.full(num_cells, vs, float), np.full(num_cells, vp, float))
+    r = np.broadcast_to(np.asarray(ratios, float), (num_cells, 3))
+    if log_eta_range is not None:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        lo, hi = np.log10(log_eta_range[0]), np.log10(log_eta_range[1])
+        eta = 10.0 ** rng.uniform(lo, hi, (num_cells, 3))
+    else:
+        eta = np.broadcast_to(np.asarray(viscosity, float), (num_cells, 3))
+    tau = eta / (mu[:, None] * r)
+    cell = np.zeros((num_cells, 33))
+    cell[:, 0], cell[:, 1], cell[:, 2] = rho, mu, K
+    cell[:, 3:6], cell[:, 6:9] = tau, r
+    if viscous_strain is not None:
+        vs_ = np.asarray(viscous_strain, float)
+        cell[:, 9:27] = np.tile(vs_, 3) if vs_.size == 6 else vs_
+    if total_strain is not None:
+        cell[:, 27:33] = np.asarray(total_strain, float)
+    aux = np.repeat(cell, nq, axis=0)
+    return np.ascontiguousarray(aux), GENMAX_SIZES.copy()
+
+
 # ------------------------------------------------------------------------------------------
 # Fields and boundary conditions.
 # ------------------------------------------------------------------------------------------

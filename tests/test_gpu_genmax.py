@@ -1,0 +1,142 @@
+"""Generalized Maxwell viscoelasticity on the device: parity of the residual, the Jacobian action
+and the state update with the oracle over several time steps (FP64 1e-12 relative), and the
+axial-strain relaxation problem of tests/fullscale/viscoelasticity/nofaults-3d
+(axialstrain_genmaxwell_soln.py) solved step by step."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from pylith_amd import _lib as L  # noqa: E402
+from pylith_amd import meshgen as mg  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+F1_GM, JF3_GM, VISC, TOTAL = 3, 4, 5, 6
+TOL64, TOL32 = 1e-12, 2e-5
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def gm_update(P, u):
+    new = P.update_state(u, VISC, 5)
+    tot = P.update_state(u, TOTAL, 6)
+    new[:, 27:33] = tot[:, 27:33]
+    return new
+
+
+def gm_dm(m, aux, sizes, dt):
+    from pylith_amd.dm import DM
+    dm = DM(m)
+    dm.add_residual(0, 0, 0, F1_GM)
+    dm.add_jacobian(0, 0, 0, 0, 0, JF3_GM)
+    dm.set_auxiliary(0, 0, aux, sizes, per_qp=True)
+    dm.set_constants(0, 0, [dt])
+    return dm
+
+
+@pytest.mark.parametrize("shape", ["affine", "perturbed"])
+def test_genmax_residual_jacobian_and_state_parity(oracle, shape):
+    m = mg.hex_box((6, 5, 4), (0, 0, 0), (6000.0, 5000.0, 4000.0), brick=4, shear=0.2)
+    if shape == "perturbed":
+        rng = np.random.default_rng(21)
+        x = m.coords.copy()
+        x += rng.uniform(-60, 60, x.shape)
+        m.coords = m.node_coords = x
+    rng = np.random.default_rng(7)
+    aux, sizes = mg.genmax_aux(m.num_cells, 8, seed=5, log_eta_range=(1e17, 1e19), ratios=(0.3, 0.3, 0.3))
+    aux[:, 9:33] = rng.standard_normal((aux.shape[0], 24)) * 1e-4  # state varies per point
+    dt = 0.1 * mg.YEAR
+    dm = gm_dm(m, aux, sizes, dt)
+    print(f"[genmax parity] shape={shape} cells={m.num_cells}", flush=True)
+    for step in range(3):
+        u = mg.random_field(m.num_nodes, 100 + step)
+        P = oracle.Problem(m, aux, sizes, per_qp=True, constants=[dt])
+        ref = P.residual(u, F1_GM)
+        r = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+        dm.compute_residual(0, 0, dev(u), r, mode=L.INSERT_VALUES)
+        torch.cuda.synchronize()
+        assert relerr(r.cpu().numpy(), ref) <= TOL64, step
+        jref = P.jacobian_action(u, JF3_GM)
+        for dt_, tol in ((torch.float64, TOL64), (torch.float32, TOL32)):
+            jv = torch.zeros(3 * m.num_nodes, dtype=dt_, device="cuda")
+            dm.compute_jacobian_action(0, 0, dev(u, dt_), jv, mode=L.INSERT_VALUES)
+            torch.cuda.synchronize()
+            assert relerr(jv.double().cpu().numpy(), jref) <= tol, (step, dt_)
+        # state update at u, then compare the whole aux field
+        dm.update_state_vars(0, 0, dev(u))
+        got = dm.get_auxiliary(0, 0, aux)
+        aux = gm_update(P, u)
+        assert relerr(got[:, 9:27], aux[:, 9:27]) <= TOL64, step
+        assert relerr(got[:, 27:33], aux[:, 27:33]) <= TOL64, step
+        assert np.array_equal(got[:, :9], aux[:, :9])
+    # a new time step size: effective moduli and state stress follow
+    dt = 0.05 * mg.YEAR
+    dm.set_constants(0, 0, [dt])
+    u = mg.random_field(m.num_nodes, 999)
+    P = oracle.Problem(m, aux, sizes, per_qp=True, constants=[dt])
+    r = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+    dm.compute_residual(0, 0, dev(u), r, mode=L.INSERT_VALUES)
+    torch.cuda.synchronize()
+    assert relerr(r.cpu().numpy(), P.residual(u, F1_GM)) <= TOL64
+
+
+def test_axial_strain_relaxation_solve(oracle):
+    """20 steps of dt = 0.025 yr: each step solves F(u) = 0 with the Dirichlet values of the
+    reference test and updates the state; the solution stays the uniform axial strain and the
+    boundary reactions follow the analytic relaxation of sigma_zz."""
+    from pylith_amd.dm import Solver
+    rho, vs, vp = 2500.0, 3464.1016, 6000.0
+    eta, ratio = (9.46728e17, 4.73364e17, 1.893456e18), (0.25, 0.25, 0.25)
+    e0 = 1.0 / 8000.0
+    m = mg.hex_box(4, (-4000.0, -4000.0, -8000.0), (4000.0, 4000.0, 0.0), brick=4)
+    eps0 = np.array([0, 0, e0, 0, 0, 0.0])
+    dev0 = eps0 - np.array([1, 1, 1, 0, 0, 0]) * e0 / 3.0
+    aux, sizes = mg.genmax_aux(m.num_cells, 8, rho, vs, vp, eta, ratio, viscous_strain=dev0,
+                               total_strain=eps0)
+    dt = 0.025 * mg.YEAR
+    dm = gm_dm(m, aux, sizes, dt)
+    X = m.node_coords
+    faces = [(np.isclose(X[:, 0], -4000) | np.isclose(X[:, 0], 4000), 0, lambda z: 0.0 * z),
+             (np.isclose(X[:, 1], -4000) | np.isclose(X[:, 1], 4000), 1, lambda z: 0.0 * z),
+             (np.isclose(X[:, 2], -8000), 2, lambda z: 0.0 * z),
+             (np.isclose(X[:, 2], 0.0), 2, lambda z: 0.0 * z + 1.0)]
+    bc = {}
+    for sel, comp, f in faces:
+        for node in np.nonzero(sel)[0]:
+            bc[3 * node + comp] = f(X[node, 2])
+    dofs = np.array(sorted(bc), np.int32)
+    vals = np.array([bc[d] for d in dofs])
+    dm.set_constraints(dofs)
+    solver = Solver(dm)
+    exact = np.zeros(3 * m.num_nodes)
+    exact[2::3] = e0 * (X[:, 2] + 8000.0)
+    mu, K = mg.moduli(rho, vs, vp)
+    lam = K - 2.0 / 3.0 * mu
+    tau = np.array(eta) / (mu * np.array(ratio))
+    top = np.isclose(X[:, 2], 0.0)
+    n = 3 * m.num_nodes
+    for step in range(1, 21):
+        zero = torch.zeros(n, dtype=torch.float64, device="cuda")
+        b = torch.zeros_like(zero)
+        dm.compute_residual(0, 0, zero, b, mode=L.INSERT_VALUES)  # the state term: F(0)
+        x = torch.zeros_like(zero)
+        dm.insert_boundary_values(dev(vals), x)
+        st = solver.solve(x, b=b, method=L.KSP_CG, rtol=1e-13, maxit=500)
+        assert st["converged"], (step, st)
+        got = x.cpu().numpy()
+        assert np.abs(got - exact).max() <= 1e-10, step
+        r = torch.zeros_like(zero)
+        dm.compute_residual(0, 0, x, r, mode=L.INSERT_VALUES)
+        torch.cuda.synchronize()
+        t = step * dt
+        fac = 1.0 - sum(ratio) + sum(ratio[i] * np.exp(-t / tau[i]) for i in range(3))
+        szz = 2.0 * mu * (2.0 * e0 / 3.0) * fac + e0 * (3.0 * lam + 2.0 * mu) / 3.0
+        reac = r.cpu().numpy().reshape(-1, 3)[top, 2].sum()
+        assert np.isclose(reac, -szz * 8000.0 * 8000.0, rtol=1e-9), step
+        dm.update_state_vars(0, 0, x)
