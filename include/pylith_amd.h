@@ -230,6 +230,36 @@ int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, 
                      int maxit, plx_solve_stats *stats);
 
 /* ---------------------------------------------------------------------------------------------
+ * Batched Green's functions: GreensFns (problems/GreensFns.cc:236-331) with the impulses of
+ * FaultCohesiveImpulses (faults/FaultCohesiveImpulses.cc:263-300) -- one linear solve per
+ * fault-slip impulse, all sharing the operator.  The operator of the key (Q1, every cell, planar
+ * fault) is assembled once at create, with the DM's constrained dofs; impulses are solved k at
+ * a time (k in {8, 16, 32, 64}, autotuned when 0) by FP64 iterative refinement around an FP32
+ * block PCG, each block solved to ||b - A x|| <= rtol ||b|| per impulse in FP64.
+ *   slip [num_impulses][num_lagrange][3] (device, fault frame), u [num_impulses][local size]
+ *   (device): the displacement of each impulse (Lagrange-multiplier entries set to zero). */
+typedef struct plx_gf_s *plx_gf;
+typedef struct {
+    int block;                     /* block width used */
+    int num_blocks;
+    int outer_iterations;          /* FP64 refinement steps, summed over blocks */
+    int inner_iterations;          /* FP32 block PCG iterations, summed over blocks */
+    int converged;
+    double max_relative_residual;  /* worst ||b - A x|| / ||b|| over the impulses, FP64 */
+} plx_gf_stats;
+int plx_gf_create(plx_gf *gf, plx_dm dm, int label_value, int part);
+int plx_gf_destroy(plx_gf gf);
+int plx_gf_set_options(plx_gf gf, int block, double inner_rtol, int inner_maxit, int use_graph);
+/* Time probe iterations for k in {8, 16, 32, 64}; keeps the fastest per impulse. */
+int plx_gf_autotune(plx_gf gf, int *best_block);
+/* Microseconds per impulse per FP32 iteration measured by the last autotune, for k = 8..64. */
+int plx_gf_probe_times(plx_gf gf, double *us_per_rhs_iteration);
+/* Y = A X for a block of k columns ([dof][k] layout, displacement dofs; synchronous). */
+int plx_gf_apply(plx_gf gf, int k, int scalar_type, const void *X, void *Y);
+int plx_gf_solve(plx_gf gf, int num_impulses, const double *slip, double *u, double rtol, int maxit,
+                 plx_gf_stats *stats);
+
+/* ---------------------------------------------------------------------------------------------
  * Event timers (the device counterpart of pylith::petsc::EventLogger, petsc/EventLogger.icc:23-47):
  * when on, every kernel launch of the handle is bracketed by hipEvents on its stream.
  * Events: "plx:apply_tiles", "plx:reduce_halo".  get_event_stats synchronises the stream. */

@@ -63,6 +63,13 @@ SIGNATURES = {
     "plx_compute_jacobian_action_cohesive": (_c_int, [_vp, _vp, _vp, _c_int]),
     "plx_update_state_vars": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _vp]),
     "plx_dm_get_auxiliary": (_c_int, [_vp, _c_int, _c_int, _dblp]),
+    "plx_gf_create": (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int]),
+    "plx_gf_destroy": (_c_int, [_vp]),
+    "plx_gf_set_options": (_c_int, [_vp, _c_int, _c_dbl, _c_int, _c_int]),
+    "plx_gf_autotune": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "plx_gf_probe_times": (_c_int, [_vp, _dblp]),
+    "plx_gf_apply": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),
+    "plx_gf_solve": (_c_int, [_vp, _c_int, _vp, _vp, _c_dbl, _c_int, _vp]),
     "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
     "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
     "plx_dm_reset_event_stats": (_c_int, [_vp]),
@@ -76,6 +83,11 @@ class SolveStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GfStats(ctypes.Structure):
+    _fields_ = [("block", _c_int), ("num_blocks", _c_int), ("outer_iterations", _c_int),
+                ("inner_iterations", _c_int), ("converged", _c_int), ("max_relative_residual", _c_dbl)]
 
 
 _lib = None

@@ -24,13 +24,31 @@ def sources():
     return [os.path.join(CSRC, f) for f in names]
 
 
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "integrator_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "integrator_driver")
+
+
 def build(verbose=False):
     cmd = [HIPCC, *COMMON_FLAGS, *DEVICE_FLAGS, "-x", "hip", *sources(), "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose=False):
+    """The C++ IntegratorDomain test driver (host code over the C ABI, linked to the library)."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-L", HERE,
+           "-lpylith_amd", "-Wl,-rpath,$ORIGIN/../../pylith_amd", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -28,7 +28,7 @@ struct CohesiveArgs {
 
 enum { kCohResidual = 0, kCohJacobian = 1, kCohSlipOnly = 2 };
 
-__device__ __forceinline__ void unit_cross(double r[3], const double u[3], const double v[3]) {
+__host__ __device__ inline void unit_cross(double r[3], const double u[3], const double v[3]) {
     r[0] = u[1] * v[2] - u[2] * v[1];
     r[1] = u[2] * v[0] - u[0] * v[2];
     r[2] = u[0] * v[1] - u[1] * v[0];
@@ -40,7 +40,7 @@ __device__ __forceinline__ void unit_cross(double r[3], const double u[3], const
 
 // BoundaryDirections::tangential_directions (BoundaryDirections.hh:50-82), including the sign
 // test as written there (refDir2[1]*n[1] counted twice, refDir2[2]*n[2] never).
-__device__ __forceinline__ void tangents(double t1[3], double t2[3], const double r1[3],
+__host__ __device__ inline void tangents(double t1[3], double t2[3], const double r1[3],
                                          const double r2[3], const double n[3]) {
     double r[3] = {r1[0], r1[1], r1[2]};
     if (fabs(r[0] * n[0] + r[1] * n[1] + r[2] * n[2]) > 0.999) {
@@ -61,7 +61,7 @@ __device__ __forceinline__ void tangents(double t1[3], double t2[3], const doubl
 // Evaluated without FMA contraction, each operation rounded as written: the sign test of the
 // fault frame (tangents) is decided by the exact rounding of the normal, e.g. +0 vs -1e-17 for a
 // fault plane containing the second reference direction.
-__device__ __forceinline__ void face_point(const double X[4][3], int q, double psi[4], double *area,
+__host__ __device__ inline void face_point(const double X[4][3], int q, double psi[4], double *area,
                                            double n[3]) {
 #pragma clang fp contract(off)
     const double g = 0.57735026918962576451;
@@ -295,6 +295,27 @@ __global__ void __launch_bounds__(256) fault_jacobi_kernel(int n, const double *
     z[i] = dinv[i / 3] * r[i];
 }
 
+// jump[i] = R slip[i] for k slip sets (planar fault): slip [k][num_lag][3] -> jump [k][num_lag][3]
+__global__ void __launch_bounds__(256) planar_jump_kernel(int64_t n, const double *__restrict__ slip,
+                                                          double R0, double R1, double R2, double R3,
+                                                          double R4, double R5, double R6, double R7,
+                                                          double R8, double *jump) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s0 = slip[3 * i], s1 = slip[3 * i + 1], s2 = slip[3 * i + 2];
+    jump[3 * i] = R0 * s0 + R1 * s1 + R2 * s2;
+    jump[3 * i + 1] = R3 * s0 + R4 * s1 + R5 * s2;
+    jump[3 * i + 2] = R6 * s0 + R7 * s1 + R8 * s2;
+}
+
+int launch_planar_jump(plx_dm dm, int64_t count, const double *slip, double *jump) {
+    const double *R = dm->fault->R;
+    hipLaunchKernelGGL(planar_jump_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, dm->stream, count,
+                       slip, R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], jump);
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
 int launch_sync_pos(plx_dm dm, const double *jump, double *v) {
     FaultData &F = *dm->fault;
     hipLaunchKernelGGL(sync_pos_kernel, dim3((F.num_lag + 255) / 256), dim3(256), 0, dm->stream, F.num_lag,
@@ -471,6 +492,34 @@ int plx_dm_add_fault(plx_dm dm, int num_cohesive, const int32_t *closure_neg, co
         PLX_HIP(dm, hipMemcpy(F->d_mass_val, mval.data(), mval.size() * sizeof(double), hipMemcpyHostToDevice));
         PLX_HIP(dm, hipMalloc(&F->d_mass_dinv, mdinv.size() * sizeof(double)));
         PLX_HIP(dm, hipMemcpy(F->d_mass_dinv, mdinv.data(), mdinv.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    {
+        // planar fault: one frame R = [n t1 t2] for every point, so M^-1 integral(psi R s) = R s
+        double n0[3] = {0, 0, 0};
+        bool planar = true;
+        for (int c = 0; c < num_cohesive && planar; ++c) {
+            double X[4][3];
+            for (int v = 0; v < 4; ++v)
+                for (int d = 0; d < 3; ++d) X[v][d] = dm->coords[3 * (size_t)cone[4 * c + v] + d];
+            for (int q = 0; q < 4; ++q) {
+                double psi[4], area, n[3];
+                face_point(X, q, psi, &area, n);
+                if (c == 0 && q == 0)
+                    for (int d = 0; d < 3; ++d) n0[d] = n[d];
+                else if (std::fabs(n[0] - n0[0]) + std::fabs(n[1] - n0[1]) + std::fabs(n[2] - n0[2]) > 1e-12)
+                    planar = false;
+            }
+        }
+        F->planar = planar;
+        if (planar) {
+            double t1[3], t2[3];
+            tangents(t1, t2, F->ref_dir_1, F->ref_dir_2, n0);
+            for (int d = 0; d < 3; ++d) {
+                F->R[3 * d + 0] = n0[d];
+                F->R[3 * d + 1] = t1[d];
+                F->R[3 * d + 2] = t2[d];
+            }
+        }
     }
     PLX_HIP(dm, hipMalloc(&F->d_slip, 3 * (size_t)num_lagrange * sizeof(double)));
     PLX_HIP(dm, hipMemset(F->d_slip, 0, 3 * (size_t)num_lagrange * sizeof(double)));

@@ -1,0 +1,793 @@
+// plx_gf.hip -- batched Green's functions: many fault-slip impulses solved together.
+//
+// PyLith's GreensFns problem (problems/GreensFns.cc:236-331) assembles the Jacobian once and
+// solves one linear system per impulse of FaultCohesiveImpulses (FaultCohesiveImpulses.cc:
+// 263-300), each impulse a unit slip on part of the fault.  All impulses share the operator.
+// Here:
+//   * the operator is assembled once into a 3x3-block CSR matrix (BSR) of the merged system:
+//     every positive-side split node is identified with its negative twin, so the matrix is the
+//     stiffness U^T K U of the constraint-eliminated fault problem (the fault disappears from
+//     it), with constrained dofs as identity rows/columns; element matrices are summed in a
+//     fixed order per block (deterministic);
+//   * an impulse's right-hand side is -U^T K u_jump, with u_jump the slip jump R s on the
+//     positive side (matrix-free FP64 apply);
+//   * k impulses form a block: vectors interleave the k columns per dof ([dof][k]), so one
+//     sparse-matrix x block-vector product reads each matrix block once for k columns;
+//   * the block is solved by FP64 iterative refinement around an FP32 block PCG (point-block
+//     Jacobi, per-column scalars on the device), ten iterations per captured hipGraph replay;
+//   * the block width k is chosen by timing probe iterations for k in {8, 16, 32, 64}.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "plx_element.hpp"
+#include "plx_internal.hpp"
+
+namespace plx {
+
+constexpr int kGfRed = 512;  // reduction blocks of the per-column dot products
+enum { G_RZ = 0, G_PQ, G_ALPHA, G_BETA, G_RR, G_COUNT };
+
+// ---------------------------------------------------------------------------------------------
+// Assembly.
+// ---------------------------------------------------------------------------------------------
+// Element stiffness blocks K_ab (3x3, a, b Q1 vertices in PyLith order) of one key cell, the
+// matrix of the operator -J for isotropic C:  K_ab[f][g] = sum_q w |J| (lambda da_f db_g +
+// mu (delta_fg da.db + da_g db_f)).  Thread (cell position p, row vertex a).
+__global__ void __launch_bounds__(256) gf_element_kernel(int ncell, const int32_t *__restrict__ cells,
+                                                         const int32_t *__restrict__ closure,
+                                                         const double *__restrict__ coords,
+                                                         const double *__restrict__ mat, double *Ke) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * ncell) return;
+    const int p = t >> 3, a = t & 7;
+    const int32_t *cl = closure + 8 * (size_t)cells[p];
+    const double lam = mat[p], mu = mat[ncell + p];
+    const double sgn[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                              {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    double X[8][3];
+    for (int v = 0; v < 8; ++v)
+        for (int d = 0; d < 3; ++d) X[v][d] = coords[3 * (size_t)cl[v] + d];
+    double K[8][9];
+    for (int b = 0; b < 8; ++b)
+        for (int i = 0; i < 9; ++i) K[b][i] = 0.0;
+    const double g = 0.57735026918962576451;
+    for (int q = 0; q < 8; ++q) {
+        const double xi[3] = {(q & 1) ? g : -g, (q & 2) ? g : -g, (q & 4) ? g : -g};
+        double dphi[8][3], J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int v = 0; v < 8; ++v) {
+            const double e0 = 1.0 + sgn[v][0] * xi[0], e1 = 1.0 + sgn[v][1] * xi[1], e2 = 1.0 + sgn[v][2] * xi[2];
+            dphi[v][0] = 0.125 * sgn[v][0] * e1 * e2;
+            dphi[v][1] = 0.125 * e0 * sgn[v][1] * e2;
+            dphi[v][2] = 0.125 * e0 * e1 * sgn[v][2];
+            for (int d = 0; d < 3; ++d)
+                for (int r = 0; r < 3; ++r) J[3 * d + r] += X[v][d] * dphi[v][r];
+        }
+        double Ji[9];
+        const double det = inv3(J, Ji);
+        double gr[8][3];
+        for (int v = 0; v < 8; ++v)
+            for (int d = 0; d < 3; ++d)
+                gr[v][d] = dphi[v][0] * Ji[d] + dphi[v][1] * Ji[3 + d] + dphi[v][2] * Ji[6 + d];
+        const double *ga = gr[a];
+        for (int b = 0; b < 8; ++b) {
+            const double *gb = gr[b];
+            const double dd = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+            for (int f = 0; f < 3; ++f)
+                for (int h = 0; h < 3; ++h)
+                    K[b][3 * f + h] += det * (lam * ga[f] * gb[h] + mu * ga[h] * gb[f] + (f == h ? mu * dd : 0.0));
+        }
+    }
+    double *out = Ke + ((size_t)p * 8 + a) * 8 * 9;
+    for (int b = 0; b < 8; ++b)
+        for (int i = 0; i < 9; ++i) out[9 * b + i] = K[b][i];
+}
+
+// One BSR block (row node i, slot s): sum of the element blocks in fixed (cell, vertex) order,
+// then identity rows / columns for constrained dofs and for positive split nodes.
+__global__ void __launch_bounds__(256) gf_assemble_kernel(int nrows, const int32_t *__restrict__ row_ptr,
+                                                          const int32_t *__restrict__ col,
+                                                          const int32_t *__restrict__ inc_ptr,
+                                                          const int32_t *__restrict__ inc,
+                                                          const int32_t *__restrict__ mclosure,
+                                                          const double *__restrict__ Ke,
+                                                          const uint8_t *__restrict__ conmask,
+                                                          const uint8_t *__restrict__ is_pos, double *val) {
+    const int i = blockIdx.x;
+    const int s = threadIdx.x;
+    if (i >= nrows) return;
+    const int e = row_ptr[i] + s;
+    if (e >= row_ptr[i + 1]) return;
+    const int j = col[e];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!is_pos[i]) {
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+            const int p = inc[k] >> 3, a = inc[k] & 7;
+            for (int b = 0; b < 8; ++b)
+                if (mclosure[8 * (size_t)p + b] == j) {
+                    const double *src = Ke + (((size_t)p * 8 + a) * 8 + b) * 9;
+                    for (int m = 0; m < 9; ++m) acc[m] += src[m];
+                }
+        }
+    }
+    for (int f = 0; f < 3; ++f)
+        for (int h = 0; h < 3; ++h) {
+            const bool fixed = is_pos[i] || conmask[3 * (size_t)i + f] || conmask[3 * (size_t)j + h];
+            if (fixed) acc[3 * f + h] = (i == j && f == h) ? 1.0 : 0.0;
+        }
+    for (int m = 0; m < 9; ++m) val[9 * (size_t)e + m] = acc[m];
+}
+
+__global__ void __launch_bounds__(256) gf_diag_kernel(int nrows, const int32_t *__restrict__ diag_pos,
+                                                      const double *__restrict__ val, double *Binv64,
+                                                      float *Binv32, float *val32, int64_t nnzb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrows) {
+        double Ai[9];
+        const double det = inv3(val + 9 * (size_t)diag_pos[i], Ai);
+        for (int m = 0; m < 9; ++m) {
+            const double v = det != 0.0 ? Ai[m] : (m % 4 == 0 ? 1.0 : 0.0);
+            Binv64[9 * (size_t)i + m] = v;
+            Binv32[9 * (size_t)i + m] = (float)v;
+        }
+    }
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 9 * nnzb;
+         t += (int64_t)gridDim.x * blockDim.x)
+        val32[t] = (float)val[t];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block kernels ([dof][k] layout, k divides 256).
+// ---------------------------------------------------------------------------------------------
+// Y = A X: thread (row node i, column r) computes the 3 rows of node i for column r.
+template <typename Real>
+__global__ void __launch_bounds__(256) gf_spmm(int nrows, int k, const int32_t *__restrict__ row_ptr,
+                                               const int32_t *__restrict__ col,
+                                               const Real *__restrict__ val, const Real *__restrict__ X,
+                                               Real *Y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrows * k) return;
+    const int i = (int)(t / k), r = (int)(t - (int64_t)i * k);
+    Real y0 = 0, y1 = 0, y2 = 0;
+    for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+        const size_t j = (size_t)col[e];
+        const Real *A = val + 9 * (size_t)e;
+        const Real x0 = X[(3 * j) * k + r], x1 = X[(3 * j + 1) * k + r], x2 = X[(3 * j + 2) * k + r];
+        y0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+        y1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+        y2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
+    }
+    Y[(3 * (size_t)i) * k + r] = y0;
+    Y[(3 * (size_t)i + 1) * k + r] = y1;
+    Y[(3 * (size_t)i + 2) * k + r] = y2;
+}
+
+// Per-column block reduction (threads with equal t % k share a column; k divides 256).
+__device__ __forceinline__ void col_reduce(double *s, int k) {
+    for (int w = 128; w >= k; w >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    }
+    __syncthreads();
+}
+
+// partials[b][r] = sum over this block's (dof, r) of a*b
+template <typename Real>
+__global__ void __launch_bounds__(256) gf_dot(int64_t ndof, int k, const Real *__restrict__ a,
+                                              const Real *__restrict__ b, double *partials) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        acc += (double)a[t] * (double)b[t];
+    s[threadIdx.x] = acc;
+    col_reduce(s, k);
+    if ((int)threadIdx.x < k) partials[(size_t)blockIdx.x * k + threadIdx.x] = s[threadIdx.x];
+}
+
+// out[r] = sum_b partials[b][r] (one thread per column, fixed order)
+__device__ __forceinline__ double col_sum(const double *partials, int k, int r) {
+    double acc = 0.0;
+    for (int b = 0; b < kGfRed; ++b) acc += partials[(size_t)b * k + r];
+    return acc;
+}
+
+__global__ void gf_finish_norm(int k, const double *__restrict__ partials, double *out) {
+    const int r = threadIdx.x;
+    if (r < k) out[r] = sqrt(col_sum(partials, k, r));
+}
+
+__global__ void gf_finish_alpha(int k, const double *__restrict__ partials, double *scal) {
+    const int r = threadIdx.x;
+    if (r >= k) return;
+    const double pq = col_sum(partials, k, r);
+    scal[G_PQ * 64 + r] = pq;
+    scal[G_ALPHA * 64 + r] = pq != 0.0 ? scal[G_RZ * 64 + r] / pq : 0.0;
+}
+
+__global__ void gf_finish_beta(int k, const double *__restrict__ prz, const double *__restrict__ prr,
+                               double *scal, int start) {
+    const int r = threadIdx.x;
+    if (r >= k) return;
+    const double rz = col_sum(prz, k, r), rr = col_sum(prr, k, r);
+    const double old = scal[G_RZ * 64 + r];
+    scal[G_BETA * 64 + r] = (!start && old != 0.0) ? rz / old : 0.0;
+    scal[G_RZ * 64 + r] = rz;
+    scal[G_RR * 64 + r] = rr;
+}
+
+// z = Binv r; (start: p = z); partials of r.z and r.r.  Thread (node, column).
+__global__ void __launch_bounds__(256) gf_precond(int nrows, int k, const float *__restrict__ Binv,
+                                                  const float *__restrict__ R, float *Z, float *P, int start,
+                                                  double *prz, double *prr) {
+    __shared__ double s1[256], s2[256];
+    double rz = 0.0, rr = 0.0;
+    const int64_t n = (int64_t)nrows * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / k, r = t - i * k;
+        const float *B = Binv + 9 * i;
+        const float r0 = R[(3 * i) * k + r], r1 = R[(3 * i + 1) * k + r], r2 = R[(3 * i + 2) * k + r];
+        const float z0 = B[0] * r0 + B[1] * r1 + B[2] * r2;
+        const float z1 = B[3] * r0 + B[4] * r1 + B[5] * r2;
+        const float z2 = B[6] * r0 + B[7] * r1 + B[8] * r2;
+        Z[(3 * i) * k + r] = z0;
+        Z[(3 * i + 1) * k + r] = z1;
+        Z[(3 * i + 2) * k + r] = z2;
+        if (start) {
+            P[(3 * i) * k + r] = z0;
+            P[(3 * i + 1) * k + r] = z1;
+            P[(3 * i + 2) * k + r] = z2;
+        }
+        rz += (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+        rr += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+    }
+    s1[threadIdx.x] = rz;
+    col_reduce(s1, k);
+    s2[threadIdx.x] = rr;
+    col_reduce(s2, k);
+    if ((int)threadIdx.x < k) {
+        prz[(size_t)blockIdx.x * k + threadIdx.x] = s1[threadIdx.x];
+        prr[(size_t)blockIdx.x * k + threadIdx.x] = s2[threadIdx.x];
+    }
+}
+
+// x += alpha p; r -= alpha q (per column alpha)
+__global__ void __launch_bounds__(256) gf_axpy2(int64_t ndof, int k, const double *__restrict__ scal,
+                                                float *X, float *R, const float *__restrict__ P,
+                                                const float *__restrict__ Q) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const float a = (float)scal[G_ALPHA * 64 + (int)(t % k)];
+        X[t] += a * P[t];
+        R[t] -= a * Q[t];
+    }
+}
+
+// p = z + beta p
+__global__ void __launch_bounds__(256) gf_pupdate(int64_t ndof, int k, const double *__restrict__ scal,
+                                                  const float *__restrict__ Z, float *P) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        P[t] = Z[t] + (float)scal[G_BETA * 64 + (int)(t % k)] * P[t];
+}
+
+// R32 = R64 / nrm[col] ; X32 = 0
+__global__ void __launch_bounds__(256) gf_to32(int64_t ndof, int k, const double *__restrict__ R64,
+                                               const double *__restrict__ nrm, float *R32, float *X32) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const double s = nrm[t % k];
+        R32[t] = s > 0.0 ? (float)(R64[t] / s) : 0.0f;
+        X32[t] = 0.0f;
+    }
+}
+
+// X64 += nrm[col] * X32
+__global__ void __launch_bounds__(256) gf_correct(int64_t ndof, int k, const float *__restrict__ X32,
+                                                  const double *__restrict__ nrm, double *X64) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        X64[t] += nrm[t % k] * (double)X32[t];
+}
+
+// R = B - R (R holds A X)
+__global__ void __launch_bounds__(256) gf_resid(int64_t n, const double *__restrict__ B, double *R) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        R[t] = B[t] - R[t];
+}
+
+// column c of a block <-> a local vector (first ndof entries)
+__global__ void __launch_bounds__(256) gf_pack(int64_t ndof, int k, int c, const double *__restrict__ v,
+                                               double *blk) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ndof; t += (int64_t)gridDim.x * 256)
+        blk[t * k + c] = v[t];
+}
+
+__global__ void __launch_bounds__(256) gf_unpack(int64_t ndof, int k, int c, const double *__restrict__ blk,
+                                                 double *v) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ndof; t += (int64_t)gridDim.x * 256)
+        v[t] = blk[t * k + c];
+}
+
+// v[pos node of i] = jump[i] (v zero elsewhere beforehand)
+__global__ void __launch_bounds__(256) gf_scatter_jump(int n, const int32_t *__restrict__ posn,
+                                                       const double *__restrict__ jump, double *v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 3; ++c) v[3 * (size_t)posn[i] + c] = jump[3 * (size_t)i + c];
+}
+
+__global__ void __launch_bounds__(256) gf_fill(int64_t n, double *a, double v) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) a[t] = v;
+}
+
+__global__ void __launch_bounds__(256) gf_mask(int n, const int32_t *__restrict__ dofs, double *v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[dofs[i]] = 0.0;
+}
+
+}  // namespace plx
+
+using namespace plx;
+
+struct plx_gf_s {
+    plx_dm dm = nullptr;
+    KeyPlan *kp = nullptr;
+    hipStream_t stream = nullptr;
+    int nrows = 0;
+    int64_t ndof = 0, nnzb = 0;
+    int32_t *d_row_ptr = nullptr, *d_col = nullptr;
+    double *d_val64 = nullptr;
+    float *d_val32 = nullptr;
+    double *d_Binv64 = nullptr;
+    float *d_Binv32 = nullptr;
+    // block work vectors, sized for kmax columns
+    int kmax = 0;
+    double *B64 = nullptr, *X64 = nullptr, *R64 = nullptr;
+    float *X32 = nullptr, *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr;
+    double *v1 = nullptr, *v2 = nullptr, *jump = nullptr;  // single local vectors
+    double *scal = nullptr, *prz = nullptr, *prr = nullptr, *nrm = nullptr;
+    // options
+    int block = 0;  // 0: autotune
+    double inner_rtol = 1e-4;
+    int inner_maxit = 4000;
+    bool use_graph = true;
+    // captured 10-iteration inner PCG chunk per block width
+    hipGraphExec_t graph[65] = {};
+    std::vector<double> probe_us;  // per-RHS per-iteration time of each probed width
+};
+
+namespace plx {
+
+static inline dim3 gf_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65535)); }
+
+static int gf_alloc(plx_gf g, int k) {
+    if (k <= g->kmax) return 0;
+    plx_dm dm = g->dm;
+    for (double *p : {g->B64, g->X64, g->R64})
+        if (p) (void)hipFree(p);
+    for (float *p : {g->X32, g->R32, g->P32, g->Q32, g->Z32})
+        if (p) (void)hipFree(p);
+    for (int i = 0; i <= 64; ++i)
+        if (g->graph[i]) {
+            (void)hipGraphExecDestroy(g->graph[i]);
+            g->graph[i] = nullptr;
+        }
+    const size_t n = (size_t)g->ndof * k;
+    for (double **p : {&g->B64, &g->X64, &g->R64}) PLX_HIP(dm, hipMalloc(p, n * sizeof(double)));
+    for (float **p : {&g->X32, &g->R32, &g->P32, &g->Q32, &g->Z32}) PLX_HIP(dm, hipMalloc(p, n * sizeof(float)));
+    g->kmax = k;
+    return 0;
+}
+
+template <typename Real>
+static void gf_spmm_launch(plx_gf g, int k, const Real *val, const Real *X, Real *Y) {
+    const int64_t n = (int64_t)g->nrows * k;
+    hipLaunchKernelGGL(gf_spmm<Real>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g->stream, g->nrows, k,
+                       g->d_row_ptr, g->d_col, val, X, Y);
+}
+
+// Ten iterations of the FP32 block PCG (p, z, scalars carried on the device).
+static void gf_pcg_chunk(plx_gf g, int k) {
+    for (int it = 0; it < 10; ++it) {
+        gf_spmm_launch<float>(g, k, g->d_val32, g->P32, g->Q32);
+        hipLaunchKernelGGL(gf_dot<float>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, g->P32, g->Q32, g->prz);
+        hipLaunchKernelGGL(gf_finish_alpha, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->scal);
+        hipLaunchKernelGGL(gf_axpy2, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, g->scal, g->X32, g->R32,
+                           g->P32, g->Q32);
+        hipLaunchKernelGGL(gf_precond, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, k, g->d_Binv32, g->R32,
+                           g->Z32, g->P32, 0, g->prz, g->prr);
+        hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 0);
+        hipLaunchKernelGGL(gf_pupdate, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, g->scal, g->Z32, g->P32);
+    }
+}
+
+static int gf_run_chunk(plx_gf g, int k) {
+    plx_dm dm = g->dm;
+    if (!g->use_graph) {
+        gf_pcg_chunk(g, k);
+        PLX_HIP(dm, hipGetLastError());
+        return 0;
+    }
+    if (!g->graph[k]) {
+        hipGraph_t graph = nullptr;
+        PLX_HIP(dm, hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        gf_pcg_chunk(g, k);
+        PLX_HIP(dm, hipStreamEndCapture(g->stream, &graph));
+        PLX_HIP(dm, hipGraphInstantiate(&g->graph[k], graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+    }
+    PLX_HIP(dm, hipGraphLaunch(g->graph[k], g->stream));
+    return 0;
+}
+
+// FP32 block PCG on A X32 = R32 from X32 = 0; all columns to inner_rtol (normalized rhs).
+static int gf_inner(plx_gf g, int k, int *its) {
+    plx_dm dm = g->dm;
+    hipLaunchKernelGGL(gf_precond, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, k, g->d_Binv32, g->R32, g->Z32,
+                       g->P32, 1, g->prz, g->prr);
+    hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
+    PLX_HIP(dm, hipGetLastError());
+    std::vector<double> rr(k), rr0(k);
+    PLX_HIP(dm, hipMemcpyAsync(rr0.data(), g->scal + G_RR * 64, k * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    int it = 0;
+    while (it < g->inner_maxit) {
+        bool done = true;
+        for (int r = 0; r < k; ++r)
+            if (rr0[r] > 0.0 && (it == 0 ? rr0[r] : rr[r]) > g->inner_rtol * g->inner_rtol * rr0[r]) done = false;
+        if (done) break;
+        int err = gf_run_chunk(g, k);
+        if (err) return err;
+        it += 10;
+        PLX_HIP(dm, hipMemcpyAsync(rr.data(), g->scal + G_RR * 64, k * sizeof(double), hipMemcpyDeviceToHost,
+                                   g->stream));
+        PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    }
+    *its = it;
+    return 0;
+}
+
+// Column norms of an FP64 block -> g->nrm (device) and host copy.
+static int gf_norms(plx_gf g, int k, const double *V, std::vector<double> &out) {
+    plx_dm dm = g->dm;
+    hipLaunchKernelGGL(gf_dot<double>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, V, V, g->prz);
+    hipLaunchKernelGGL(gf_finish_norm, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->nrm);
+    PLX_HIP(dm, hipGetLastError());
+    out.resize(k);
+    PLX_HIP(dm, hipMemcpyAsync(out.data(), g->nrm, k * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+// Solve A_m X64 = B64 (k columns) by FP64 iterative refinement around the FP32 block PCG.
+static int gf_block_solve(plx_gf g, int k, double rtol, int maxit, plx_gf_stats *st) {
+    plx_dm dm = g->dm;
+    const int64_t n = g->ndof * k;
+    hipLaunchKernelGGL(gf_fill, gf_grid(n), dim3(256), 0, g->stream, n, g->X64, 0.0);
+    std::vector<double> b0, rn;
+    int err = gf_norms(g, k, g->B64, b0);
+    if (err) return err;
+    // R = B (X = 0)
+    PLX_HIP(dm, hipMemcpyAsync(g->R64, g->B64, n * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+    rn = b0;
+    int outer = 0;
+    for (;;) {
+        bool done = true;
+        for (int r = 0; r < k; ++r)
+            if (rn[r] > rtol * b0[r]) done = false;
+        if (done || outer >= maxit) break;
+        if ((err = gf_norms(g, k, g->R64, rn))) return err;
+        hipLaunchKernelGGL(gf_to32, gf_grid(n), dim3(256), 0, g->stream, g->ndof, k, g->R64, g->nrm, g->R32, g->X32);
+        int its = 0;
+        if ((err = gf_inner(g, k, &its))) return err;
+        st->inner_iterations += its;
+        hipLaunchKernelGGL(gf_correct, gf_grid(n), dim3(256), 0, g->stream, g->ndof, k, g->X32, g->nrm, g->X64);
+        gf_spmm_launch<double>(g, k, g->d_val64, g->X64, g->R64);
+        hipLaunchKernelGGL(gf_resid, gf_grid(n), dim3(256), 0, g->stream, n, g->B64, g->R64);
+        PLX_HIP(dm, hipGetLastError());
+        if ((err = gf_norms(g, k, g->R64, rn))) return err;
+        ++outer;
+    }
+    st->outer_iterations += outer;
+    double worst = 0.0;
+    for (int r = 0; r < k; ++r)
+        if (b0[r] > 0.0) worst = std::max(worst, rn[r] / b0[r]);
+    st->max_relative_residual = std::max(st->max_relative_residual, worst);
+    return 0;
+}
+
+int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::vector<int32_t> &col,
+             const std::vector<int32_t> &inc_ptr, const std::vector<int32_t> &inc,
+             const std::vector<int32_t> &mcl, const std::vector<int32_t> &diag_pos,
+             const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos);
+
+}  // namespace plx
+
+extern "C" {
+
+int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
+    if (!out || !dm) return 1;
+    *out = nullptr;
+    if (dm->cell_type != PLX_HEX_Q1) return set_error(dm, 1, "Green's functions are implemented for Q1 hexes");
+    if (!dm->fault) return set_error(dm, 1, "Green's functions need a fault");
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp || !kp->jf3 || !kp->has_aux) return set_error(dm, 1, "the key needs Jacobian kernels and aux");
+    if (kp->cells.size() != (size_t)dm->num_cells) return set_error(dm, 1, "the key must cover every cell");
+    FaultData &F = *dm->fault;
+    const int nn = dm->num_nodes, nc = (int)kp->cells.size();
+    std::vector<int32_t> merged(nn);
+    std::vector<uint8_t> is_pos(nn, 0);
+    for (int i = 0; i < nn; ++i) merged[i] = i;
+    for (int l = 0; l < F.num_lag; ++l) {
+        merged[F.pos_node[l]] = F.neg_node[l];
+        is_pos[F.pos_node[l]] = 1;
+    }
+    std::vector<uint8_t> conmask(3 * (size_t)nn, 0);
+    for (int32_t d : dm->con.dofs) {
+        if (d >= 3 * nn) return set_error(dm, 1, "constrained Lagrange-multiplier dofs are not supported");
+        if (is_pos[d / 3]) return set_error(dm, 1, "constrain the negative-side twin of a split fault node");
+        conmask[d] = 1;
+    }
+    // merged closures of the key cells (key order) and node -> (cell, vertex) incidence
+    std::vector<int32_t> mcl(8 * (size_t)nc);
+    std::vector<int32_t> inc_ptr(nn + 1, 0), inc;
+    for (int p = 0; p < nc; ++p)
+        for (int a = 0; a < 8; ++a) {
+            mcl[8 * (size_t)p + a] = merged[dm->closure[8 * (size_t)kp->cells[p] + a]];
+            inc_ptr[mcl[8 * (size_t)p + a] + 1]++;
+        }
+    for (int i = 0; i < nn; ++i) inc_ptr[i + 1] += inc_ptr[i];
+    inc.resize(inc_ptr[nn]);
+    {
+        std::vector<int32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int p = 0; p < nc; ++p)
+            for (int a = 0; a < 8; ++a) inc[fill[mcl[8 * (size_t)p + a]]++] = 8 * p + a;
+    }
+    // BSR pattern: the merged node graph (positive nodes: diagonal only)
+    std::vector<int32_t> row_ptr(nn + 1, 0), col, diag_pos(nn);
+    std::vector<int32_t> nb;
+    for (int i = 0; i < nn; ++i) {
+        nb.clear();
+        nb.push_back(i);
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+            const int p = inc[k] >> 3;
+            for (int b = 0; b < 8; ++b) nb.push_back(mcl[8 * (size_t)p + b]);
+        }
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        if (nb.size() > 256) return set_error(dm, 1, "node degree above 256");
+        for (int32_t j : nb) {
+            if (j == i) diag_pos[i] = (int32_t)col.size();
+            col.push_back(j);
+        }
+        row_ptr[i + 1] = (int32_t)col.size();
+    }
+    plx_gf g = new plx_gf_s();
+    g->dm = dm;
+    g->kp = kp;
+    g->nrows = nn;
+    g->ndof = 3 * (int64_t)nn;
+    g->nnzb = (int64_t)col.size();
+    const int err = gf_setup(g, nc, row_ptr, col, inc_ptr, inc, mcl, diag_pos, conmask, is_pos);
+    if (err) {
+        plx_gf_destroy(g);
+        return err;
+    }
+    *out = g;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace plx {
+
+int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::vector<int32_t> &col,
+             const std::vector<int32_t> &inc_ptr, const std::vector<int32_t> &inc,
+             const std::vector<int32_t> &mcl, const std::vector<int32_t> &diag_pos,
+             const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos) {
+    plx_dm dm = g->dm;
+    KeyPlan *kp = g->kp;
+    FaultData &F = *dm->fault;
+    const int nn = g->nrows;
+    auto up = [&](auto **d, const auto &h) -> int {
+        using T = typename std::decay<decltype(h)>::type::value_type;
+        PLX_HIP(dm, hipMalloc((void **)d, h.size() * sizeof(T)));
+        PLX_HIP(dm, hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    };
+    int32_t *d_inc_ptr = nullptr, *d_inc = nullptr, *d_mcl = nullptr, *d_diag = nullptr;
+    uint8_t *d_con = nullptr, *d_pos = nullptr;
+    double *d_Ke = nullptr;
+    int err = 0;
+    if ((err = up(&g->d_row_ptr, row_ptr)) || (err = up(&g->d_col, col)) || (err = up(&d_inc_ptr, inc_ptr)) ||
+        (err = up(&d_inc, inc)) || (err = up(&d_mcl, mcl)) || (err = up(&d_diag, diag_pos)) ||
+        (err = up(&d_con, conmask)) || (err = up(&d_pos, is_pos)))
+        return err;
+    if (!dm->d_closure) {
+        PLX_HIP(dm, hipMalloc(&dm->d_closure, dm->closure.size() * sizeof(int32_t)));
+        PLX_HIP(dm, hipMemcpy(dm->d_closure, dm->closure.data(), dm->closure.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice));
+    }
+    PLX_HIP(dm, hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    PLX_HIP(dm, hipMalloc(&d_Ke, (size_t)nc * 64 * 9 * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->d_val64, (size_t)g->nnzb * 9 * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->d_val32, (size_t)g->nnzb * 9 * sizeof(float)));
+    PLX_HIP(dm, hipMalloc(&g->d_Binv64, 9 * (size_t)nn * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->d_Binv32, 9 * (size_t)nn * sizeof(float)));
+    hipLaunchKernelGGL(gf_element_kernel, dim3((8 * nc + 255) / 256), dim3(256), 0, g->stream, nc, kp->d_cells,
+                       dm->d_closure, dm->d_coords, kp->d_mat64, d_Ke);
+    hipLaunchKernelGGL(gf_assemble_kernel, dim3(nn), dim3(256), 0, g->stream, nn, g->d_row_ptr, g->d_col, d_inc_ptr,
+                       d_inc, d_mcl, d_Ke, d_con, d_pos, g->d_val64);
+    hipLaunchKernelGGL(gf_diag_kernel, dim3((nn + 255) / 256), dim3(256), 0, g->stream, nn, d_diag, g->d_val64,
+                       g->d_Binv64, g->d_Binv32, g->d_val32, g->nnzb);
+    PLX_HIP(dm, hipGetLastError());
+    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    for (void *p : {(void *)d_inc_ptr, (void *)d_inc, (void *)d_mcl, (void *)d_diag, (void *)d_con, (void *)d_pos,
+                    (void *)d_Ke})
+        (void)hipFree(p);
+    PLX_HIP(dm, hipMalloc(&g->scal, G_COUNT * 64 * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->prz, (size_t)kGfRed * 64 * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->prr, (size_t)kGfRed * 64 * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->nrm, 64 * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->v1, dm->local_size * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->v2, dm->local_size * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->jump, 3 * (size_t)F.num_lag * sizeof(double)));
+    return 0;
+}
+
+}  // namespace plx
+
+extern "C" {
+
+int plx_gf_destroy(plx_gf g) {
+    if (!g) return 0;
+    for (int i = 0; i <= 64; ++i)
+        if (g->graph[i]) (void)hipGraphExecDestroy(g->graph[i]);
+    void *ptrs[] = {g->d_row_ptr, g->d_col, g->d_val64, g->d_val32, g->d_Binv64, g->d_Binv32, g->B64, g->X64,
+                    g->R64,       g->X32,   g->R32,     g->P32,     g->Q32,      g->Z32,      g->v1,  g->v2,
+                    g->jump,      g->scal,  g->prz,     g->prr,     g->nrm};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+int plx_gf_set_options(plx_gf g, int block, double inner_rtol, int inner_maxit, int use_graph) {
+    if (!g) return 1;
+    if (block != 0 && block != 8 && block != 16 && block != 32 && block != 64)
+        return set_error(g->dm, 1, "block width must be 0 (autotune), 8, 16, 32 or 64");
+    if (!(inner_rtol > 0.0) || inner_maxit <= 0) return set_error(g->dm, 1, "bad inner solver options");
+    g->block = block;
+    g->inner_rtol = inner_rtol;
+    g->inner_maxit = inner_maxit;
+    g->use_graph = use_graph != 0;
+    return 0;
+}
+
+int plx_gf_apply(plx_gf g, int k, int scalar_type, const void *X, void *Y) {
+    if (!g || !X || !Y) return 1;
+    if (k <= 0 || 256 % k != 0 || k > 64) return set_error(g->dm, 1, "k must divide 256 and be <= 64");
+    plx_dm dm = g->dm;
+    if (scalar_type == PLX_FP64)
+        gf_spmm_launch<double>(g, k, g->d_val64, (const double *)X, (double *)Y);
+    else
+        gf_spmm_launch<float>(g, k, g->d_val32, (const float *)X, (float *)Y);
+    PLX_HIP(dm, hipGetLastError());
+    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int plx_gf_autotune(plx_gf g, int *best) {
+    if (!g) return 1;
+    plx_dm dm = g->dm;
+    const int cand[4] = {8, 16, 32, 64};
+    int err = gf_alloc(g, 64);
+    if (err) return err;
+    g->probe_us.assign(4, 0.0);
+    double best_t = 1e300;
+    int best_k = 8;
+    for (int c = 0; c < 4; ++c) {
+        const int k = cand[c];
+        const int64_t n = g->ndof * k;
+        hipLaunchKernelGGL(gf_fill, gf_grid(n), dim3(256), 0, g->stream, n, g->R64, 1.0);
+        hipLaunchKernelGGL(gf_fill, dim3(1), dim3(64), 0, g->stream, 64, g->nrm, std::sqrt((double)g->ndof));
+        hipLaunchKernelGGL(gf_to32, gf_grid(n), dim3(256), 0, g->stream, g->ndof, k, g->R64, g->nrm, g->R32, g->X32);
+        hipLaunchKernelGGL(gf_precond, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, k, g->d_Binv32, g->R32,
+                           g->Z32, g->P32, 1, g->prz, g->prr);
+        hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
+        if ((err = gf_run_chunk(g, k))) return err;  // warm-up (and graph capture)
+        PLX_HIP(dm, hipStreamSynchronize(g->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int rep = 0; rep < 3; ++rep)
+            if ((err = gf_run_chunk(g, k))) return err;
+        PLX_HIP(dm, hipStreamSynchronize(g->stream));
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        const double per = us / (30.0 * k);  // per RHS per iteration
+        g->probe_us[c] = per;
+        if (per < best_t) {
+            best_t = per;
+            best_k = k;
+        }
+    }
+    g->block = best_k;
+    if (best) *best = best_k;
+    return 0;
+}
+
+int plx_gf_probe_times(plx_gf g, double *us_per_rhs_iteration) {
+    if (!g || !us_per_rhs_iteration) return 1;
+    for (size_t i = 0; i < g->probe_us.size(); ++i) us_per_rhs_iteration[i] = g->probe_us[i];
+    return 0;
+}
+
+int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, double rtol, int maxit,
+                 plx_gf_stats *stats) {
+    if (!g || num_impulses <= 0 || !slip || !u || !(rtol > 0.0) || maxit <= 0) return 1;
+    plx_dm dm = g->dm;
+    FaultData &F = *dm->fault;
+    int err = 0;
+    if (g->block == 0 && (err = plx_gf_autotune(g, nullptr))) return err;
+    const int k = g->block;
+    if ((err = gf_alloc(g, k))) return err;
+    plx_gf_stats st{};
+    st.block = k;
+    // run everything (including the matrix-free applies) on the solver's own stream, ordered
+    // after the work already queued on the handle's stream
+    hipEvent_t ev;
+    PLX_HIP(dm, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    PLX_HIP(dm, hipEventRecord(ev, dm->stream));
+    PLX_HIP(dm, hipStreamWaitEvent(g->stream, ev, 0));
+    hipStream_t saved = dm->stream;
+    dm->stream = g->stream;
+    const int64_t nloc = dm->local_size, ndof = g->ndof, nl3 = 3 * (int64_t)F.num_lag;
+    const int nc = (int)dm->con.dofs.size();
+    for (int j0 = 0; j0 < num_impulses && !err; j0 += k) {
+        const int kb = std::min(k, num_impulses - j0);
+        // right-hand sides: B_j = fold(J u_jump_j), masked; columns beyond kb stay zero
+        hipLaunchKernelGGL(gf_fill, gf_grid(ndof * k), dim3(256), 0, g->stream, ndof * k, g->B64, 0.0);
+        for (int c = 0; c < kb && !err; ++c) {
+            const double *s = slip + (size_t)(j0 + c) * nl3;
+            if (F.planar) {
+                err = launch_planar_jump(dm, F.num_lag, s, g->jump);
+            } else {
+                err = set_error(dm, 1, "Green's functions need a planar fault");
+            }
+            if (err) break;
+            hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, g->v1, 0.0);
+            hipLaunchKernelGGL(gf_scatter_jump, dim3((F.num_lag + 255) / 256), dim3(256), 0, g->stream, F.num_lag,
+                               F.d_pos_node, g->jump, g->v1);
+            if ((err = launch_apply(dm, g->kp, PLX_FP64, true, g->v1, g->v2, PLX_INSERT_VALUES, 1.0))) break;
+            if ((err = launch_fold(dm, PLX_FP64, g->v2, kFold))) break;
+            if (nc > 0)
+                hipLaunchKernelGGL(gf_mask, dim3((nc + 255) / 256), dim3(256), 0, g->stream, nc, dm->con.d_dofs, g->v2);
+            hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->v2, g->B64);
+        }
+        if (err) break;
+        PLX_HIP(dm, hipGetLastError());
+        if ((err = gf_block_solve(g, k, rtol, maxit, &st))) break;
+        // solutions: u = U w + u_jump (Lagrange multipliers left at zero)
+        for (int c = 0; c < kb && !err; ++c) {
+            double *uj = u + (size_t)(j0 + c) * nloc;
+            hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, uj, 0.0);
+            hipLaunchKernelGGL(gf_unpack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->X64, uj);
+            if ((err = launch_planar_jump(dm, F.num_lag, slip + (size_t)(j0 + c) * nl3, g->jump))) break;
+            err = launch_sync_pos(dm, g->jump, uj);
+        }
+        st.num_blocks++;
+    }
+    dm->stream = saved;
+    if (!err) {
+        PLX_HIP(dm, hipEventRecord(ev, g->stream));
+        PLX_HIP(dm, hipStreamWaitEvent(dm->stream, ev, 0));
+    }
+    (void)hipEventDestroy(ev);
+    if (err) return err;
+    st.converged = st.max_relative_residual <= rtol ? 1 : 0;
+    if (stats) *stats = st;
+    return 0;
+}
+
+}  // extern "C"

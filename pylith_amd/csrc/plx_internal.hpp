@@ -114,6 +114,8 @@ struct FaultData {
     int32_t *d_color_cells = nullptr;
     int32_t *d_neg_node = nullptr, *d_pos_node = nullptr;
     double *d_slip = nullptr;              // [num_lag][3] in the fault frame (normal, t1, t2)
+    bool planar = false;                   // one fault frame everywhere: jump = R slip exactly
+    double R[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // columns n, t1, t2 (row-major 3x3)
     // scalar fault-face mass matrix M_ab = integral psi_a psi_b over the fault (CSR over Lagrange
     // nodes, the same 2x2 Gauss rule as the cohesive kernels) and its inverse diagonal
     int32_t *d_mass_ptr = nullptr, *d_mass_col = nullptr;
@@ -191,6 +193,7 @@ enum { kFold = 0, kUnfold = 1, kClearPos = 2 };
 int launch_fold(plx_dm dm, int scalar_type, void *v, int op);
 int launch_gather_pos(plx_dm dm, const double *v, double *out);
 int launch_sync_pos(plx_dm dm, const double *jump, double *v);
+int launch_planar_jump(plx_dm dm, int64_t count, const double *slip, double *jump);
 int launch_fault_mass(plx_dm dm, const double *x, double *y);
 int launch_fault_jacobi(plx_dm dm, const double *r, double *z);
 int launch_fault_slip_rhs(plx_dm dm, double *lag_out);

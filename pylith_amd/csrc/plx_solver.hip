@@ -396,6 +396,7 @@ static int fault_mass_solve(plx_solver s, const double *rhs, double *out) {
 static int fault_jump(plx_solver s, const double *b) {
     plx_dm dm = s->dm;
     const int64_t n = 3 * (int64_t)dm->fault->num_lag;
+    if (dm->fault->planar && !b) return launch_planar_jump(dm, dm->fault->num_lag, dm->fault->d_slip, s->jump);
     int err = launch_fault_slip_rhs(dm, s->lx);  // lx = -g
     if (err) return err;
     if (b)

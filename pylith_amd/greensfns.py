@@ -1,0 +1,63 @@
+"""Batched Green's functions over the C ABI: many fault-slip impulses solved with one operator.
+
+Mirrors pylith.problems.GreensFns (libsrc/pylith/problems/GreensFns.cc:236-331): the operator
+is assembled once, then every impulse of the fault's impulse basis (FaultCohesiveImpulses,
+faults/FaultCohesiveImpulses.cc:263-300) is solved.  Impulses are solved in blocks of k columns
+(FP32 block PCG inside FP64 iterative refinement), k autotuned over {8, 16, 32, 64}.
+"""
+import ctypes
+
+import torch
+
+from . import _lib as L
+from .dm import PlxError, _ptr
+
+
+class GreensFunctions:
+    def __init__(self, dm, label_value=0, part=0):
+        self.dm = dm
+        self.lib = dm.lib
+        h = ctypes.c_void_p()
+        err = self.lib.plx_gf_create(ctypes.byref(h), dm.h, label_value, part)
+        if err:
+            raise PlxError(f"plx_gf_create failed ({err}): {self.lib.plx_dm_last_error(dm.h).decode()}")
+        self.h = h
+
+    def _check(self, err):
+        if err:
+            raise PlxError(f"error {err}: {self.lib.plx_dm_last_error(self.dm.h).decode()}")
+
+    def set_options(self, block=0, inner_rtol=1e-4, inner_maxit=4000, use_graph=True):
+        self._check(self.lib.plx_gf_set_options(self.h, block, inner_rtol, inner_maxit, int(use_graph)))
+
+    def autotune(self):
+        best = ctypes.c_int()
+        self._check(self.lib.plx_gf_autotune(self.h, ctypes.byref(best)))
+        times = (ctypes.c_double * 4)()
+        self._check(self.lib.plx_gf_probe_times(self.h, times))
+        return best.value, {k: times[i] for i, k in enumerate((8, 16, 32, 64))}
+
+    def apply(self, X, Y, k):
+        """Y = A X for a [dof, k] block (float64 or float32 CUDA tensors)."""
+        st = L.FP64 if X.dtype == torch.float64 else L.FP32
+        self._check(self.lib.plx_gf_apply(self.h, k, st, _ptr(X), _ptr(Y)))
+
+    def solve(self, slip, u, rtol=1e-10, maxit=20):
+        """slip: [n, num_lag, 3] float64 CUDA tensor; u: [n, local size] float64 CUDA tensor."""
+        if slip.dtype != torch.float64 or u.dtype != torch.float64:
+            raise PlxError("slip and u must be float64")
+        st = L.GfStats()
+        self._check(self.lib.plx_gf_solve(self.h, slip.shape[0], _ptr(slip), _ptr(u), rtol, maxit,
+                                          ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in L.GfStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.plx_gf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -359,3 +359,18 @@ def tapered_patch_slip(fault, center, half_width, slip, component=1, taper=None)
     s = np.zeros((fault.num_lag, 3))
     s[:, component] = slip * w
     return s
+
+
+def fault_vertex_impulses(fault, j_range, k_range, component=1, amplitude=1.0):
+    """Unit-slip impulses of FaultCohesiveImpulses (faults/FaultCohesiveImpulses.cc:271-285): one
+    impulse per fault vertex of the region j in j_range, k in k_range (fault grid positions),
+    `amplitude` on slip component `component` (1 = left-lateral).  Returns (slip
+    [num_impulses, num_lag, 3], the Lagrange-node id of each impulse)."""
+    jk = fault.lag_jk
+    sel = np.nonzero((jk[:, 0] >= j_range[0]) & (jk[:, 0] < j_range[1]) &
+                     (jk[:, 1] >= k_range[0]) & (jk[:, 1] < k_range[1]))[0]
+    order = np.lexsort((jk[sel, 0], jk[sel, 1]))  # strike fastest
+    sel = sel[order]
+    slip = np.zeros((sel.size, fault.num_lag, 3))
+    slip[np.arange(sel.size), sel, component] = amplitude
+    return slip, sel
