@@ -93,6 +93,217 @@ def cpu_baseline(problem, u, colors, budget_s=10.0):
     return out, dt, n, nthreads
 
 
+def gm_bytes_update(nn, nc):
+    """Generalized Maxwell state update per launch: state read + write (24 doubles per point x 8
+    points), next-step state stress (6 x 8 doubles), per-cell material (8 doubles), closure (8
+    int32), u and coordinates at the nodes (FP64)."""
+    return nc * (2 * 24 * 8 * 8 + 6 * 8 * 8 + 8 * 8 + 32) + 48 * nn
+
+
+def bench_c2_solve(torch, mg, L):
+    """C2 numerics: P2 64^3 x 6 with the shear Dirichlet values, solved to 1e-8 relative residual
+    by FGMRES with an FP32 PCG preconditioner (FP64 true residual) and by plain FP64 PCG."""
+    import numpy as np
+    from pylith_amd.dm import Solver
+    mesh = mg.tet_p2_box(64, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(mesh.num_cells, seed=SEED, vs_sigma=0.2, vp_vs=1.732)
+    dm = setup(mesh, aux, sizes)
+    dofs, vals = mg.shear_bcs(mesh.node_coords, 0.0, 6000.0)
+    dm.set_constraints(dofs)
+    solver = Solver(dm)
+    out = {}
+    sols = {}
+    for name, method in (("fp64_cg", L.KSP_CG), ("fp32_fgmres_ir", L.KSP_FGMRES_IR)):
+        x = torch.zeros(3 * mesh.num_nodes, dtype=torch.float64, device="cuda")
+        dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(x, method=method, rtol=1e-8, maxit=20000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sols[name] = x.cpu().numpy()
+        out[name] = {"seconds": dt, "converged": bool(st["converged"]), "outer": st["outer_iterations"],
+                     "inner_fp32": st["inner_iterations"],
+                     "rel_residual": st["final_residual"] / max(st["initial_residual"], 1e-300)}
+    a, b = sols["fp32_fgmres_ir"], sols["fp64_cg"]
+    out["rel_diff_fp32ir_vs_fp64"] = float(np.abs(a - b).max() / np.abs(b).max())
+    out["speedup_fp32ir_vs_fp64"] = out["fp64_cg"]["seconds"] / out["fp32_fgmres_ir"]["seconds"]
+    out["workload"] = "P2 64^3 x 6 tets, -x fixed, +x u_y = 1 m, rtol 1e-8"
+    solver.close()
+    dm.close()
+    return out
+
+
+def bench_c3(torch, mg, L):
+    """C3: generalized Maxwell (3 elements, eta log-uniform [1e17, 1e19] per cell) + vertical
+    fault (2 m left-lateral on a 4 x 4 km patch: 2 x 2 km at full slip tapering to zero at the
+    patch edge), 48^3 Q1 hexes in a 6 km cube, 10 steps of 0.1 yr: per step the state term,
+    an FP64 solve (FGMRES + FP32 PCG), the state update."""
+    import numpy as np
+    from pylith_amd.dm import DM, Solver
+    n = 48
+    m, f = mg.hex_box_with_fault(n, (0.0, 0.0, -6000.0), (6000.0, 6000.0, 0.0), n // 2, brick=8)
+    aux, sizes = mg.genmax_aux(m.num_cells, 8, seed=SEED, log_eta_range=(1e17, 1e19), ratios=(0.3, 0.3, 0.3))
+    dt = 0.1 * mg.YEAR
+    dm = DM(m)
+    dm.add_residual(0, 0, 0, 3)
+    dm.add_jacobian(0, 0, 0, 0, 0, 4)
+    dm.set_auxiliary(0, 0, aux, sizes, per_qp=True)
+    dm.set_constants(0, 0, [dt])
+    dm.add_fault(f)
+    slip = mg.tapered_patch_slip(f, (3000.0, -3000.0), 1000.0, 2.0, component=1, taper=1000.0)
+    dm.set_fault_slip(slip)
+    fixed = np.nonzero(np.isclose(m.node_coords[:, 0], 0.0) | np.isclose(m.node_coords[:, 0], 6000.0))[0]
+    dofs = np.sort((3 * fixed[:, None] + np.arange(3)).ravel()).astype(np.int32)
+    dm.set_constraints(dofs)
+    solver = Solver(dm)
+    nloc = dm.sizes()[0]
+    nu = 3 * m.num_nodes
+    x = torch.zeros(nloc, dtype=torch.float64, device="cuda")
+    zero = torch.zeros_like(x)
+    b = torch.zeros_like(x)
+    res = torch.zeros_like(x)
+    steps = []
+    dm.reset_event_stats()
+    dm.set_event_logging(True)
+    t_all = time.perf_counter()
+    for step in range(10):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dm.compute_residual(0, 0, zero, b, mode=L.INSERT_VALUES)  # state term F(0)
+        st = solver.solve(x, b=b, method=L.KSP_FGMRES_IR, rtol=1e-10, maxit=200)
+        dm.update_state_vars(0, 0, x)
+        torch.cuda.synchronize()
+        steps.append({"seconds": time.perf_counter() - t0, "converged": bool(st["converged"]),
+                      "inner_fp32": st["inner_iterations"]})
+    total = time.perf_counter() - t_all
+    n_upd, ms_upd = dm.event_stats("plx:genmax_update")
+    n_coh, ms_coh = dm.event_stats("plx:cohesive")
+    dm.set_event_logging(False)
+    # the C3 unit: one residual apply (elastic operator + state term, cohesive rows) + one update
+    r_apply = lambda: (dm.compute_residual(0, 0, x, res, mode=L.INSERT_VALUES),  # noqa: E731
+                       dm.compute_residual_cohesive(x, res, mode=L.ADD_VALUES))
+    t_res = time_applies(r_apply, 20, 3, torch)
+    k_res, _ = kernel_avg_ms(dm, r_apply, 20, torch)
+    t_upd = time_applies(lambda: dm.update_state_vars(0, 0, x), 20, 3, torch)
+    k_upd, _ = kernel_avg_ms(dm, lambda: dm.update_state_vars(0, 0, x), 20, torch, "plx:genmax_update")
+    bu = gm_bytes_update(m.num_nodes, m.num_cells)
+    br = q1_bytes(m.num_nodes, m.num_cells, 8) + 6 * 8 * 8 * m.num_cells
+    # full residual at the last solution: F(x) with the fault rows (FP64 check)
+    dm.compute_residual(0, 0, x, res, mode=L.INSERT_VALUES)
+    dm.compute_residual_cohesive(x, res, mode=L.ADD_VALUES)
+    rn = res.cpu().numpy()
+    rn[dofs] = 0.0
+    out = {
+        "workload": "48^3 Q1 GenMaxwell (3 Maxwell elements/point) + vertical fault, 10 x 0.1 yr, FP64",
+        "dofs": nu, "lagrange_dofs": nloc - nu, "cohesive_cells": f.num_cohesive,
+        "steps_seconds_total": total, "step_ms_mean": 1e3 * total / 10,
+        "all_converged": all(s["converged"] for s in steps),
+        "unit_ms_residual_plus_update": 1e3 * (t_res + t_upd),
+        "residual_apply_ms": 1e3 * t_res, "residual_kernel_ms": k_res,
+        "residual_hbm_frac": br / (k_res * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "state_update_ms": 1e3 * t_upd, "state_update_kernel_ms": k_upd,
+        "state_update_algorithmic_bytes": bu,
+        "state_update_hbm_frac": bu / (k_upd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "cohesive_kernel_ms_per_call": ms_coh / max(n_coh, 1),
+        "final_residual_rel": float(np.abs(rn).max() / max(np.abs(b.cpu().numpy()).max(), 1e-300)),
+    }
+    solver.close()
+    dm.close()
+    return out
+
+
+def bench_c4(torch, mg, L, n_impulses=512):
+    """C4: 64^3 Q1 (80 km cube, h = 1.25 km) sheared so the mid-plane fault dips 60 degrees,
+    x faces fixed; one unit left-lateral impulse per fault vertex of a 32 (strike) x 16 (dip)
+    region (the per-vertex impulse basis of FaultCohesiveImpulses): 512 right-hand sides."""
+    import numpy as np
+    from pylith_amd import workloads
+    from pylith_amd.dm import Solver
+    from pylith_amd.greensfns import GreensFunctions
+    prob = workloads.faulted_box(64, length=80000.0, dip_deg=60.0)
+    f = prob.fault
+    slip, verts = mg.fault_vertex_impulses(f, (16, 48), (48, 64), component=1)
+    slip = slip[:n_impulses]
+    nimp = slip.shape[0]
+    nloc = prob.local_size
+    nu = 3 * prob.mesh.num_nodes
+    gf = GreensFunctions(prob.dm)
+    t0 = time.perf_counter()
+    best, probe = gf.autotune()
+    t_tune = time.perf_counter() - t0
+    rtol = 1e-11
+    s_dev = torch.from_numpy(slip).cuda()
+    U = torch.zeros(nimp, nloc, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = gf.solve(s_dev, U, rtol=rtol, maxit=30)
+    torch.cuda.synchronize()
+    t_gf = time.perf_counter() - t0
+    # plain FP64 GPU path (matrix-free PCG, one impulse at a time) on a sample, and the FP64
+    # reference solutions (tighter tolerance) for the error
+    solver = Solver(prob.dm)
+    sample = [0, nimp // 3, (2 * nimp) // 3, nimp - 1]
+    errs, t64 = [], []
+    for j in sample:
+        prob.dm.set_fault_slip(slip[j])
+        x = torch.zeros(nloc, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s1 = solver.solve(x, method=L.KSP_CG, rtol=rtol, maxit=20000)
+        torch.cuda.synchronize()
+        t64.append(time.perf_counter() - t1)
+        xr = torch.zeros_like(x)
+        solver.solve(xr, method=L.KSP_CG, rtol=1e-14, maxit=40000)
+        ref = xr.cpu().numpy()[:nu]
+        errs.append(float(np.abs(U[j].cpu().numpy()[:nu] - ref).max() / np.abs(ref).max()))
+    fp64_rate = 1.0 / float(np.mean(t64))
+    gf_rate = nimp / t_gf
+    out = {
+        "workload": "64^3 Q1, 80 km cube, fault dipping 60 deg, 32 x 16 vertex impulses (512 RHS)",
+        "impulses": nimp, "block": st["block"], "autotune_s": t_tune,
+        "probe_us_per_rhs_iteration": probe,
+        "seconds": t_gf, "solves_per_s": gf_rate,
+        "outer_iterations": st["outer_iterations"], "inner_iterations": st["inner_iterations"],
+        "max_rel_residual": st["max_relative_residual"],
+        "max_rel_error_vs_fp64": max(errs), "error_sample": len(sample),
+        "fp64_single_solves_per_s": fp64_rate,
+        "speedup_vs_fp64": gf_rate / fp64_rate,
+    }
+    solver.close()
+    gf.close()
+    prob.dm.close()
+    return out
+
+
+def bench_c5(torch, n_requests=120):
+    """C5: Poisson stream of fault-slip solves on 16^3/32^3/64^3/96^3 meshes (40/30/20/10%),
+    64 patches with N(0, 0.5 m) slip, lognormal Vs per cell (per mesh size), 200 GB budget;
+    the arrival rate is 70% of the capacity measured with all requests queued at once."""
+    from pylith_amd import serving
+    sizes, probs = (16, 32, 64, 96), (0.4, 0.3, 0.2, 0.1)
+    eng = serving.ServingEngine(sizes, block=16, rtol=1e-11)
+    t0 = time.perf_counter()
+    eng.warm()
+    t_warm = time.perf_counter() - t0
+    cap_reqs = serving.make_requests(n_requests, sizes, probs, rate=None, seed=11)
+    done = eng.run(cap_reqs)
+    cap = serving.latency_stats(done)["solves_per_s"]
+    rate = 0.7 * cap
+    reqs = serving.make_requests(n_requests, sizes, probs, rate=rate, seed=12)
+    done = eng.run(reqs)
+    stats = serving.latency_stats(done)
+    per = {}
+    for sz in sizes:
+        d = [r for r in done if r.size == sz]
+        if d:
+            per[str(sz)] = serving.latency_stats(d)
+    return {"workload": "Poisson stream, meshes 16^3/32^3/64^3/96^3 at 40/30/20/10%, 64 fault patches",
+            "requests": n_requests, "capacity_solves_per_s": cap, "arrival_rate_per_s": rate,
+            "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "solves_per_s": stats["solves_per_s"],
+            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-11, "block": 16}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +312,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--q1-large", type=int, default=192, help="edge of the large Q1 cube")
+    ap.add_argument("--no-systems", action="store_true",
+                    help="skip the solver / viscoelastic / Green's-function / serving measurements")
+    ap.add_argument("--only", choices=["c2solve", "c3", "c4", "c5"], default=None,
+                    help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -115,6 +330,13 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.only:
+        from pylith_amd import _lib as L
+        fn = {"c2solve": lambda: bench_c2_solve(torch, mg, L), "c3": lambda: bench_c3(torch, mg, L),
+              "c4": lambda: bench_c4(torch, mg, L), "c5": lambda: bench_c5(torch)}[args.only]
+        print(f"[bench] {args.only}", file=sys.stderr, flush=True)
+        print(json.dumps({args.only: fn()}))
+        return
     print("[bench] building configs[1] workload", file=sys.stderr, flush=True)
     mesh = mg.tet_p2_box(64, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
     aux, sizes = mg.elastic_aux(mesh.num_cells, seed=SEED, vs_sigma=0.2, vp_vs=1.732)
@@ -243,6 +465,17 @@ def main():
             del hx, hr
             torch.cuda.empty_cache()
         result["extra"] = extra
+    if not args.no_systems:
+        from pylith_amd import _lib as L
+        systems = {}
+        for name, fn in (("c2_solve_fp32_refinement", lambda: bench_c2_solve(torch, mg, L)),
+                         ("c3_genmaxwell_fault", lambda: bench_c3(torch, mg, L)),
+                         ("c4_greens_functions", lambda: bench_c4(torch, mg, L)),
+                         ("c5_serving", lambda: bench_c5(torch))):
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            systems[name] = fn()
+            torch.cuda.empty_cache()
+        result["systems"] = systems
     print(json.dumps(result))
 
 

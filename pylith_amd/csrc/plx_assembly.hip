@@ -550,15 +550,22 @@ __global__ void __launch_bounds__(256) pbdiag_cells(TilePlanDev P, const int32_t
     }
 }
 
-int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks) {
-    int err = ensure_tables(dm);
-    if (err) return err;
+// Device copies of the closure and cone index (set-up kernels only), each uploaded on first use.
+int ensure_mesh_indices(plx_dm dm) {
     if (!dm->d_closure) {
         PLX_HIP(dm, hipMalloc(&dm->d_closure, dm->closure.size() * sizeof(int32_t)));
         PLX_HIP(dm, hipMemcpy(dm->d_closure, dm->closure.data(), dm->closure.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    if (!dm->d_cone) {
         PLX_HIP(dm, hipMalloc(&dm->d_cone, dm->cone.size() * sizeof(int32_t)));
         PLX_HIP(dm, hipMemcpy(dm->d_cone, dm->cone.data(), dm->cone.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
+    return 0;
+}
+
+int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks) {
+    int err = ensure_tables(dm);
+    if (err || (err = ensure_mesh_indices(dm))) return err;
     const int n = (int)kp->cells.size();
     const TilePlanDev P = plan_dev(dm, kp);
     if (dm->cell_type == PLX_HEX_Q1)

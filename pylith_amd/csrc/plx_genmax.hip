@@ -149,15 +149,6 @@ __global__ void __launch_bounds__(256) genmax_update_kernel(GmArgs A, const doub
     }
 }
 
-static int ensure_closure(plx_dm dm) {
-    if (!dm->d_closure) {
-        PLX_HIP(dm, hipMalloc(&dm->d_closure, dm->closure.size() * sizeof(int32_t)));
-        PLX_HIP(dm, hipMemcpy(dm->d_closure, dm->closure.data(), dm->closure.size() * sizeof(int32_t),
-                              hipMemcpyHostToDevice));
-    }
-    return 0;
-}
-
 static GmArgs gm_args(plx_dm dm, KeyPlan *kp) {
     GmArgs A;
     A.n = (int)kp->cells.size();
@@ -181,7 +172,7 @@ int launch_genmax_prestress(plx_dm dm, KeyPlan *kp) {
 }
 
 int launch_genmax_update(plx_dm dm, KeyPlan *kp, const double *u) {
-    int err = ensure_closure(dm);
+    int err = ensure_mesh_indices(dm);
     if (err) return err;
     const GmArgs A = gm_args(dm, kp);
     if (A.n == 0) return 0;

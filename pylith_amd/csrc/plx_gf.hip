@@ -605,11 +605,7 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
         (err = up(&d_inc, inc)) || (err = up(&d_mcl, mcl)) || (err = up(&d_diag, diag_pos)) ||
         (err = up(&d_con, conmask)) || (err = up(&d_pos, is_pos)))
         return err;
-    if (!dm->d_closure) {
-        PLX_HIP(dm, hipMalloc(&dm->d_closure, dm->closure.size() * sizeof(int32_t)));
-        PLX_HIP(dm, hipMemcpy(dm->d_closure, dm->closure.data(), dm->closure.size() * sizeof(int32_t),
-                              hipMemcpyHostToDevice));
-    }
+    if ((err = ensure_mesh_indices(dm))) return err;
     PLX_HIP(dm, hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     PLX_HIP(dm, hipMalloc(&d_Ke, (size_t)nc * 64 * 9 * sizeof(double)));
     PLX_HIP(dm, hipMalloc(&g->d_val64, (size_t)g->nnzb * 9 * sizeof(double)));

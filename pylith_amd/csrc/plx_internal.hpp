@@ -180,6 +180,7 @@ void events_release(plx_dm dm);
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
                  void *r, int mode, double scale = 1.0);
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
+int ensure_mesh_indices(plx_dm dm);  // uploads d_closure and d_cone (each if absent)
 // Generalized Maxwell state (plx_genmax.hip).
 int launch_genmax_prestress(plx_dm dm, KeyPlan *kp);
 int launch_genmax_update(plx_dm dm, KeyPlan *kp, const double *u);
