@@ -172,6 +172,12 @@ def bench_c3(torch, mg, L):
         t0 = time.perf_counter()
         dm.compute_residual(0, 0, zero, b, mode=L.INSERT_VALUES)  # state term F(0)
         st = solver.solve(x, b=b, method=L.KSP_FGMRES_IR, rtol=1e-10, maxit=200)
+        if step == 9:  # full residual (domain + cohesive rows) at this step's solution
+            dm.compute_residual(0, 0, x, res, mode=L.INSERT_VALUES)
+            dm.compute_residual_cohesive(x, res, mode=L.ADD_VALUES)
+            rn = res.cpu().numpy()
+            rn[dofs] = 0.0
+            fres = float(np.abs(rn).max() / max(np.abs(b.cpu().numpy()).max(), 1e-300))
         dm.update_state_vars(0, 0, x)
         torch.cuda.synchronize()
         steps.append({"seconds": time.perf_counter() - t0, "converged": bool(st["converged"]),
@@ -189,11 +195,6 @@ def bench_c3(torch, mg, L):
     k_upd, _ = kernel_avg_ms(dm, lambda: dm.update_state_vars(0, 0, x), 20, torch, "plx:genmax_update")
     bu = gm_bytes_update(m.num_nodes, m.num_cells)
     br = q1_bytes(m.num_nodes, m.num_cells, 8) + 6 * 8 * 8 * m.num_cells
-    # full residual at the last solution: F(x) with the fault rows (FP64 check)
-    dm.compute_residual(0, 0, x, res, mode=L.INSERT_VALUES)
-    dm.compute_residual_cohesive(x, res, mode=L.ADD_VALUES)
-    rn = res.cpu().numpy()
-    rn[dofs] = 0.0
     out = {
         "workload": "48^3 Q1 GenMaxwell (3 Maxwell elements/point) + vertical fault, 10 x 0.1 yr, FP64",
         "dofs": nu, "lagrange_dofs": nloc - nu, "cohesive_cells": f.num_cohesive,
@@ -206,7 +207,7 @@ def bench_c3(torch, mg, L):
         "state_update_algorithmic_bytes": bu,
         "state_update_hbm_frac": bu / (k_upd * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cohesive_kernel_ms_per_call": ms_coh / max(n_coh, 1),
-        "final_residual_rel": float(np.abs(rn).max() / max(np.abs(b.cpu().numpy()).max(), 1e-300)),
+        "last_step_full_residual_rel": fres,
     }
     solver.close()
     dm.close()
@@ -232,7 +233,7 @@ def bench_c4(torch, mg, L, n_impulses=512):
     t0 = time.perf_counter()
     best, probe = gf.autotune()
     t_tune = time.perf_counter() - t0
-    rtol = 1e-11
+    rtol = 1e-8  # both paths to the same relative residual; errors vs a 1e-14 FP64 solve reported
     s_dev = torch.from_numpy(slip).cuda()
     U = torch.zeros(nimp, nloc, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
@@ -244,19 +245,20 @@ def bench_c4(torch, mg, L, n_impulses=512):
     # reference solutions (tighter tolerance) for the error
     solver = Solver(prob.dm)
     sample = [0, nimp // 3, (2 * nimp) // 3, nimp - 1]
-    errs, t64 = [], []
+    errs, errs64, t64 = [], [], []
     for j in sample:
         prob.dm.set_fault_slip(slip[j])
         x = torch.zeros(nloc, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        s1 = solver.solve(x, method=L.KSP_CG, rtol=rtol, maxit=20000)
+        solver.solve(x, method=L.KSP_CG, rtol=rtol, maxit=20000)
         torch.cuda.synchronize()
         t64.append(time.perf_counter() - t1)
         xr = torch.zeros_like(x)
         solver.solve(xr, method=L.KSP_CG, rtol=1e-14, maxit=40000)
         ref = xr.cpu().numpy()[:nu]
         errs.append(float(np.abs(U[j].cpu().numpy()[:nu] - ref).max() / np.abs(ref).max()))
+        errs64.append(float(np.abs(x.cpu().numpy()[:nu] - ref).max() / np.abs(ref).max()))
     fp64_rate = 1.0 / float(np.mean(t64))
     gf_rate = nimp / t_gf
     out = {
@@ -266,7 +268,8 @@ def bench_c4(torch, mg, L, n_impulses=512):
         "seconds": t_gf, "solves_per_s": gf_rate,
         "outer_iterations": st["outer_iterations"], "inner_iterations": st["inner_iterations"],
         "max_rel_residual": st["max_relative_residual"],
-        "max_rel_error_vs_fp64": max(errs), "error_sample": len(sample),
+        "max_rel_error_vs_fp64": max(errs), "error_sample": len(sample), "rtol": rtol,
+        "fp64_path_max_rel_error": max(errs64),
         "fp64_single_solves_per_s": fp64_rate,
         "speedup_vs_fp64": gf_rate / fp64_rate,
     }
@@ -282,7 +285,7 @@ def bench_c5(torch, n_requests=120):
     the arrival rate is 70% of the capacity measured with all requests queued at once."""
     from pylith_amd import serving
     sizes, probs = (16, 32, 64, 96), (0.4, 0.3, 0.2, 0.1)
-    eng = serving.ServingEngine(sizes, block=16, rtol=1e-11)
+    eng = serving.ServingEngine(sizes, block=16, rtol=1e-8)
     t0 = time.perf_counter()
     eng.warm()
     t_warm = time.perf_counter() - t0
@@ -301,7 +304,7 @@ def bench_c5(torch, n_requests=120):
     return {"workload": "Poisson stream, meshes 16^3/32^3/64^3/96^3 at 40/30/20/10%, 64 fault patches",
             "requests": n_requests, "capacity_solves_per_s": cap, "arrival_rate_per_s": rate,
             "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "solves_per_s": stats["solves_per_s"],
-            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-11, "block": 16}
+            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-8, "block": 16}
 
 
 def main():

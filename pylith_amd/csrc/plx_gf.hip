@@ -26,7 +26,7 @@
 
 namespace plx {
 
-constexpr int kGfRed = 512;  // reduction blocks of the per-column dot products
+constexpr int kGfRed = 2048;  // blocks of the grid-stride block kernels (= per-column partials)
 enum { G_RZ = 0, G_PQ, G_ALPHA, G_BETA, G_RR, G_COUNT };
 
 // ---------------------------------------------------------------------------------------------
@@ -217,59 +217,128 @@ __global__ void gf_finish_beta(int k, const double *__restrict__ prz, const doub
     scal[G_RR * 64 + r] = rr;
 }
 
-// z = Binv r; (start: p = z); partials of r.z and r.r.  Thread (node, column).
-__global__ void __launch_bounds__(256) gf_precond(int nrows, int k, const float *__restrict__ Binv,
-                                                  const float *__restrict__ R, float *Z, float *P, int start,
-                                                  double *prz, double *prr) {
-    __shared__ double s1[256], s2[256];
-    double rz = 0.0, rr = 0.0;
-    const int64_t n = (int64_t)nrows * k;
+// ---- FP32 block PCG kernels with a compile-time block width K (thread = (node, column)) ----
+// q = A p and the per-column partial sums of p.q
+template <int K>
+__global__ void __launch_bounds__(256) gfk_spmm_dot(int nrows, const int32_t *__restrict__ row_ptr,
+                                                    const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                    const float *__restrict__ P, float *Q, double *ppq) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    const int64_t n = (int64_t)nrows * K;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-        const int64_t i = t / k, r = t - i * k;
-        const float *B = Binv + 9 * i;
-        const float r0 = R[(3 * i) * k + r], r1 = R[(3 * i + 1) * k + r], r2 = R[(3 * i + 2) * k + r];
-        const float z0 = B[0] * r0 + B[1] * r1 + B[2] * r2;
-        const float z1 = B[3] * r0 + B[4] * r1 + B[5] * r2;
-        const float z2 = B[6] * r0 + B[7] * r1 + B[8] * r2;
-        Z[(3 * i) * k + r] = z0;
-        Z[(3 * i + 1) * k + r] = z1;
-        Z[(3 * i + 2) * k + r] = z2;
-        if (start) {
-            P[(3 * i) * k + r] = z0;
-            P[(3 * i + 1) * k + r] = z1;
-            P[(3 * i + 2) * k + r] = z2;
+        const int64_t i = t / K;
+        const int r = (int)(t % K);
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+        const int e1 = row_ptr[i + 1];
+        for (int e = row_ptr[i]; e < e1; ++e) {
+            const int64_t j = col[e];
+            const float *A = val + 9 * (size_t)e;
+            const float x0 = P[(3 * j) * K + r], x1 = P[(3 * j + 1) * K + r], x2 = P[(3 * j + 2) * K + r];
+            y0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+            y1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+            y2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
         }
-        rz += (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
-        rr += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+        const size_t o = (3 * (size_t)i) * K + r;
+        Q[o] = y0;
+        Q[o + K] = y1;
+        Q[o + 2 * K] = y2;
+        acc += (double)P[o] * y0 + (double)P[o + K] * y1 + (double)P[o + 2 * K] * y2;
     }
-    s1[threadIdx.x] = rz;
-    col_reduce(s1, k);
-    s2[threadIdx.x] = rr;
-    col_reduce(s2, k);
-    if ((int)threadIdx.x < k) {
-        prz[(size_t)blockIdx.x * k + threadIdx.x] = s1[threadIdx.x];
-        prr[(size_t)blockIdx.x * k + threadIdx.x] = s2[threadIdx.x];
-    }
+    s[threadIdx.x] = acc;
+    col_reduce(s, K);
+    if ((int)threadIdx.x < K) ppq[(size_t)blockIdx.x * K + threadIdx.x] = s[threadIdx.x];
 }
 
-// x += alpha p; r -= alpha q (per column alpha)
-__global__ void __launch_bounds__(256) gf_axpy2(int64_t ndof, int k, const double *__restrict__ scal,
-                                                float *X, float *R, const float *__restrict__ P,
-                                                const float *__restrict__ Q) {
-    const int64_t n = ndof * k;
+// K = 64: one wavefront per block row, lane = column.  The row index is wave-uniform, so the
+// row's column indices and 3x3 blocks come through scalar loads and each neighbour's three
+// values are one coalesced 256-byte vector load per component.
+__global__ void __launch_bounds__(256) gfk_spmm_dot64(int nrows, const int32_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col,
+                                                      const float *__restrict__ val, const float *__restrict__ P,
+                                                      float *Q, double *ppq) {
+    __shared__ double s[256];
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int row0 = blockIdx.x * 4 + (int)(threadIdx.x >> 6); row0 < nrows; row0 += gridDim.x * 4) {
+        const int row = __builtin_amdgcn_readfirstlane(row0);
+        const int e0 = row_ptr[row], e1 = row_ptr[row + 1];
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+        for (int e = e0; e < e1; ++e) {
+            const size_t j = (size_t)col[e];
+            const float *A = val + 9 * (size_t)e;
+            const float x0 = P[(3 * j) * 64 + lane], x1 = P[(3 * j + 1) * 64 + lane], x2 = P[(3 * j + 2) * 64 + lane];
+            y0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+            y1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+            y2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
+        }
+        const size_t o = (3 * (size_t)row) * 64 + lane;
+        Q[o] = y0;
+        Q[o + 64] = y1;
+        Q[o + 128] = y2;
+        acc += (double)P[o] * y0 + (double)P[o + 64] * y1 + (double)P[o + 128] * y2;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64)
+        ppq[(size_t)blockIdx.x * 64 + threadIdx.x] = (s[threadIdx.x] + s[threadIdx.x + 64]) +
+                                                     (s[threadIdx.x + 128] + s[threadIdx.x + 192]);
+}
+
+// x += alpha p; r -= alpha q; z = Binv r; partial sums of r.z and r.r (start: z = Binv r, p = z)
+template <int K, bool START>
+__global__ void __launch_bounds__(256) gfk_update(int nrows, const float *__restrict__ Binv,
+                                                  const double *__restrict__ scal, float *X, float *R,
+                                                  float *P, const float *__restrict__ Q, float *Z, double *prz,
+                                                  double *prr) {
+    __shared__ double s1[256], s2[256];
+    double rz = 0.0, rr = 0.0;
+    const int64_t n = (int64_t)nrows * K;
+    const int rc = (int)(threadIdx.x % K);  // this thread's column (fixed: the stride is a multiple of K)
+    const float a = START ? 0.f : (float)scal[G_ALPHA * 64 + rc];
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-        const float a = (float)scal[G_ALPHA * 64 + (int)(t % k)];
-        X[t] += a * P[t];
-        R[t] -= a * Q[t];
+        const int64_t i = t / K;
+        const size_t o = (3 * (size_t)i) * K + rc;
+        float rv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (START) {
+                rv[c] = R[o + c * K];
+            } else {
+                X[o + c * K] += a * P[o + c * K];
+                rv[c] = R[o + c * K] - a * Q[o + c * K];
+                R[o + c * K] = rv[c];
+            }
+        }
+        const float *B = Binv + 9 * i;
+        float zc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zc[c] = B[3 * c] * rv[0] + B[3 * c + 1] * rv[1] + B[3 * c + 2] * rv[2];
+            Z[o + c * K] = zc[c];
+            if (START) P[o + c * K] = zc[c];
+            rz += (double)rv[c] * zc[c];
+            rr += (double)rv[c] * rv[c];
+        }
+    }
+    s1[threadIdx.x] = rz;
+    col_reduce(s1, K);
+    s2[threadIdx.x] = rr;
+    col_reduce(s2, K);
+    if ((int)threadIdx.x < K) {
+        prz[(size_t)blockIdx.x * K + threadIdx.x] = s1[threadIdx.x];
+        prr[(size_t)blockIdx.x * K + threadIdx.x] = s2[threadIdx.x];
     }
 }
 
 // p = z + beta p
-__global__ void __launch_bounds__(256) gf_pupdate(int64_t ndof, int k, const double *__restrict__ scal,
-                                                  const float *__restrict__ Z, float *P) {
-    const int64_t n = ndof * k;
+template <int K>
+__global__ void __launch_bounds__(256) gfk_pupdate(int64_t n, const double *__restrict__ scal,
+                                                   const float *__restrict__ Z, float *P) {
+    const int rc = (int)(threadIdx.x % K);
+    const float b = (float)scal[G_BETA * 64 + rc];
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
-        P[t] = Z[t] + (float)scal[G_BETA * 64 + (int)(t % k)] * P[t];
+        P[t] = Z[t] + b * P[t];
 }
 
 // R32 = R64 / nrm[col] ; X32 = 0
@@ -388,19 +457,48 @@ static void gf_spmm_launch(plx_gf g, int k, const Real *val, const Real *X, Real
                        g->d_row_ptr, g->d_col, val, X, Y);
 }
 
-// Ten iterations of the FP32 block PCG (p, z, scalars carried on the device).
-static void gf_pcg_chunk(plx_gf g, int k) {
+// Ten iterations of the FP32 block PCG (p, z, scalars carried on the device): per iteration
+// q = A p with p.q fused, the x/r/z update with r.z and r.r fused, and p = z + beta p.
+template <int K>
+static void gf_pcg_chunk_k(plx_gf g) {
     for (int it = 0; it < 10; ++it) {
-        gf_spmm_launch<float>(g, k, g->d_val32, g->P32, g->Q32);
-        hipLaunchKernelGGL(gf_dot<float>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, g->P32, g->Q32, g->prz);
-        hipLaunchKernelGGL(gf_finish_alpha, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->scal);
-        hipLaunchKernelGGL(gf_axpy2, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, g->scal, g->X32, g->R32,
-                           g->P32, g->Q32);
-        hipLaunchKernelGGL(gf_precond, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, k, g->d_Binv32, g->R32,
-                           g->Z32, g->P32, 0, g->prz, g->prr);
-        hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 0);
-        hipLaunchKernelGGL(gf_pupdate, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, g->scal, g->Z32, g->P32);
+        if (K == 64)
+            hipLaunchKernelGGL(gfk_spmm_dot64, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
+                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
+        else
+            hipLaunchKernelGGL(gfk_spmm_dot<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
+                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
+        hipLaunchKernelGGL(gf_finish_alpha, dim3(1), dim3(64), 0, g->stream, K, g->prz, g->scal);
+        hipLaunchKernelGGL((gfk_update<K, false>), dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_Binv32,
+                           g->scal, g->X32, g->R32, g->P32, g->Q32, g->Z32, g->prz, g->prr);
+        hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, K, g->prz, g->prr, g->scal, 0);
+        hipLaunchKernelGGL(gfk_pupdate<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof * K, g->scal, g->Z32,
+                           g->P32);
     }
+}
+
+static void gf_pcg_chunk(plx_gf g, int k) {
+    switch (k) {
+    case 8: gf_pcg_chunk_k<8>(g); break;
+    case 16: gf_pcg_chunk_k<16>(g); break;
+    case 32: gf_pcg_chunk_k<32>(g); break;
+    default: gf_pcg_chunk_k<64>(g); break;
+    }
+}
+
+// PCG start: z = Binv r, p = z, r.z and r.r (X32 already zero)
+static void gf_pcg_start(plx_gf g, int k) {
+#define PLX_GF_START(K)                                                                                      \
+    hipLaunchKernelGGL((gfk_update<K, true>), dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_Binv32, \
+                       g->scal, g->X32, g->R32, g->P32, g->Q32, g->Z32, g->prz, g->prr)
+    switch (k) {
+    case 8: PLX_GF_START(8); break;
+    case 16: PLX_GF_START(16); break;
+    case 32: PLX_GF_START(32); break;
+    default: PLX_GF_START(64); break;
+    }
+#undef PLX_GF_START
+    hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
 }
 
 static int gf_run_chunk(plx_gf g, int k) {
@@ -425,9 +523,7 @@ static int gf_run_chunk(plx_gf g, int k) {
 // FP32 block PCG on A X32 = R32 from X32 = 0; all columns to inner_rtol (normalized rhs).
 static int gf_inner(plx_gf g, int k, int *its) {
     plx_dm dm = g->dm;
-    hipLaunchKernelGGL(gf_precond, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, k, g->d_Binv32, g->R32, g->Z32,
-                       g->P32, 1, g->prz, g->prr);
-    hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
+    gf_pcg_start(g, k);
     PLX_HIP(dm, hipGetLastError());
     std::vector<double> rr(k), rr0(k);
     PLX_HIP(dm, hipMemcpyAsync(rr0.data(), g->scal + G_RR * 64, k * sizeof(double), hipMemcpyDeviceToHost, g->stream));
@@ -691,9 +787,7 @@ int plx_gf_autotune(plx_gf g, int *best) {
         hipLaunchKernelGGL(gf_fill, gf_grid(n), dim3(256), 0, g->stream, n, g->R64, 1.0);
         hipLaunchKernelGGL(gf_fill, dim3(1), dim3(64), 0, g->stream, 64, g->nrm, std::sqrt((double)g->ndof));
         hipLaunchKernelGGL(gf_to32, gf_grid(n), dim3(256), 0, g->stream, g->ndof, k, g->R64, g->nrm, g->R32, g->X32);
-        hipLaunchKernelGGL(gf_precond, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, k, g->d_Binv32, g->R32,
-                           g->Z32, g->P32, 1, g->prz, g->prr);
-        hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
+        gf_pcg_start(g, k);
         if ((err = gf_run_chunk(g, k))) return err;  // warm-up (and graph capture)
         PLX_HIP(dm, hipStreamSynchronize(g->stream));
         const auto t0 = std::chrono::steady_clock::now();
