@@ -186,35 +186,61 @@ __global__ void __launch_bounds__(256) gf_dot(int64_t ndof, int k, const Real *_
     if ((int)threadIdx.x < k) partials[(size_t)blockIdx.x * k + threadIdx.x] = s[threadIdx.x];
 }
 
-// out[r] = sum_b partials[b][r] (one thread per column, fixed order)
+// Column r of the per-block partials summed by block r (256 threads, fixed-order tree).
 __device__ __forceinline__ double col_sum(const double *partials, int k, int r) {
+    __shared__ double s[256];
     double acc = 0.0;
-    for (int b = 0; b < kGfRed; ++b) acc += partials[(size_t)b * k + r];
-    return acc;
+    for (int b = threadIdx.x; b < kGfRed; b += 256) acc += partials[(size_t)b * k + r];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    return s[0];
 }
 
-__global__ void gf_finish_norm(int k, const double *__restrict__ partials, double *out) {
-    const int r = threadIdx.x;
-    if (r < k) out[r] = sqrt(col_sum(partials, k, r));
+// grid = k blocks of 256 threads: block r finishes column r
+__global__ void __launch_bounds__(256) gf_finish_norm(int k, const double *__restrict__ partials, double *out) {
+    const int r = blockIdx.x;
+    const double v = col_sum(partials, k, r);
+    if (threadIdx.x == 0) out[r] = sqrt(v);
 }
 
-__global__ void gf_finish_alpha(int k, const double *__restrict__ partials, double *scal) {
-    const int r = threadIdx.x;
-    if (r >= k) return;
+__global__ void __launch_bounds__(256) gf_finish_alpha(int k, const double *__restrict__ partials, double *scal) {
+    const int r = blockIdx.x;
     const double pq = col_sum(partials, k, r);
-    scal[G_PQ * 64 + r] = pq;
-    scal[G_ALPHA * 64 + r] = pq != 0.0 ? scal[G_RZ * 64 + r] / pq : 0.0;
+    if (threadIdx.x == 0) {
+        scal[G_PQ * 64 + r] = pq;
+        scal[G_ALPHA * 64 + r] = pq != 0.0 ? scal[G_RZ * 64 + r] / pq : 0.0;
+    }
 }
 
-__global__ void gf_finish_beta(int k, const double *__restrict__ prz, const double *__restrict__ prr,
-                               double *scal, int start) {
-    const int r = threadIdx.x;
-    if (r >= k) return;
-    const double rz = col_sum(prz, k, r), rr = col_sum(prr, k, r);
-    const double old = scal[G_RZ * 64 + r];
-    scal[G_BETA * 64 + r] = (!start && old != 0.0) ? rz / old : 0.0;
-    scal[G_RZ * 64 + r] = rz;
-    scal[G_RR * 64 + r] = rr;
+__global__ void __launch_bounds__(256) gf_finish_beta(int k, const double *__restrict__ prz,
+                                                      const double *__restrict__ prr, double *scal, int start) {
+    const int r = blockIdx.x;
+    const double rz = col_sum(prz, k, r);
+    __syncthreads();
+    const double rr = col_sum(prr, k, r);
+    if (threadIdx.x == 0) {
+        const double old = scal[G_RZ * 64 + r];
+        scal[G_BETA * 64 + r] = (!start && old != 0.0) ? rz / old : 0.0;
+        scal[G_RZ * 64 + r] = rz;
+        scal[G_RR * 64 + r] = rr;
+    }
+}
+
+// Row schedule of the SpMM: XCD x (workgroups are dispatched round-robin over the 8 XCDs)
+// owns the slab [lo, hi) of rows; its nbx workgroups walk the slab together, workgroup j taking
+// chunk j of every window of nbx chunks.  At any time an XCD's resident workgroups work on one
+// compact window of rows, so the neighbour vectors those rows share stay in that XCD's L2.
+__device__ __forceinline__ void gf_xcd_slab(int n, int &lo, int &hi, int &j, int &nbx) {
+    const int x = blockIdx.x & 7;
+    j = blockIdx.x >> 3;
+    nbx = gridDim.x >> 3;
+    const int per = (n + 7) / 8;
+    lo = min(n, x * per);
+    hi = min(n, lo + per);
 }
 
 // ---- FP32 block PCG kernels with a compile-time block width K (thread = (node, column)) ----
@@ -225,10 +251,13 @@ __global__ void __launch_bounds__(256) gfk_spmm_dot(int nrows, const int32_t *__
                                                     const float *__restrict__ P, float *Q, double *ppq) {
     __shared__ double s[256];
     double acc = 0.0;
-    const int64_t n = (int64_t)nrows * K;
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-        const int64_t i = t / K;
-        const int r = (int)(t % K);
+    int lo, hi, jb, nbx;
+    gf_xcd_slab(nrows, lo, hi, jb, nbx);
+    constexpr int RW = 256 / K;  // rows per workgroup chunk
+    const int r = (int)(threadIdx.x % K);
+    for (int base = lo + jb * RW; base < hi; base += nbx * RW) {
+        const int64_t i = base + (int)(threadIdx.x / K);
+        if (i >= hi) continue;
         float y0 = 0.f, y1 = 0.f, y2 = 0.f;
         const int e1 = row_ptr[i + 1];
         for (int e = row_ptr[i]; e < e1; ++e) {
@@ -260,11 +289,31 @@ __global__ void __launch_bounds__(256) gfk_spmm_dot64(int nrows, const int32_t *
     __shared__ double s[256];
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
-    for (int row0 = blockIdx.x * 4 + (int)(threadIdx.x >> 6); row0 < nrows; row0 += gridDim.x * 4) {
+    int lo, hi, jb, nbx;
+    gf_xcd_slab(nrows, lo, hi, jb, nbx);
+    for (int row0 = lo + jb * 4 + (int)(threadIdx.x >> 6); row0 < hi; row0 += nbx * 4) {
         const int row = __builtin_amdgcn_readfirstlane(row0);
         const int e0 = row_ptr[row], e1 = row_ptr[row + 1];
         float y0 = 0.f, y1 = 0.f, y2 = 0.f;
-        for (int e = e0; e < e1; ++e) {
+        int e = e0;
+        // groups of 9 neighbours: all 27 loads of a group are issued before the arithmetic
+        for (; e + 9 <= e1; e += 9) {
+            float x[9][3];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const size_t j = (size_t)col[e + m];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) x[m][c] = P[(3 * j + c) * 64 + lane];
+            }
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const float *A = val + 9 * (size_t)(e + m);
+                y0 += A[0] * x[m][0] + A[1] * x[m][1] + A[2] * x[m][2];
+                y1 += A[3] * x[m][0] + A[4] * x[m][1] + A[5] * x[m][2];
+                y2 += A[6] * x[m][0] + A[7] * x[m][1] + A[8] * x[m][2];
+            }
+        }
+        for (; e < e1; ++e) {
             const size_t j = (size_t)col[e];
             const float *A = val + 9 * (size_t)e;
             const float x0 = P[(3 * j) * 64 + lane], x1 = P[(3 * j + 1) * 64 + lane], x2 = P[(3 * j + 2) * 64 + lane];
@@ -468,10 +517,10 @@ static void gf_pcg_chunk_k(plx_gf g) {
         else
             hipLaunchKernelGGL(gfk_spmm_dot<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
                                g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
-        hipLaunchKernelGGL(gf_finish_alpha, dim3(1), dim3(64), 0, g->stream, K, g->prz, g->scal);
+        hipLaunchKernelGGL(gf_finish_alpha, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->scal);
         hipLaunchKernelGGL((gfk_update<K, false>), dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_Binv32,
                            g->scal, g->X32, g->R32, g->P32, g->Q32, g->Z32, g->prz, g->prr);
-        hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, K, g->prz, g->prr, g->scal, 0);
+        hipLaunchKernelGGL(gf_finish_beta, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->prr, g->scal, 0);
         hipLaunchKernelGGL(gfk_pupdate<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof * K, g->scal, g->Z32,
                            g->P32);
     }
@@ -498,7 +547,7 @@ static void gf_pcg_start(plx_gf g, int k) {
     default: PLX_GF_START(64); break;
     }
 #undef PLX_GF_START
-    hipLaunchKernelGGL(gf_finish_beta, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
+    hipLaunchKernelGGL(gf_finish_beta, dim3(k), dim3(256), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
 }
 
 static int gf_run_chunk(plx_gf g, int k) {
@@ -549,7 +598,7 @@ static int gf_inner(plx_gf g, int k, int *its) {
 static int gf_norms(plx_gf g, int k, const double *V, std::vector<double> &out) {
     plx_dm dm = g->dm;
     hipLaunchKernelGGL(gf_dot<double>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, k, V, V, g->prz);
-    hipLaunchKernelGGL(gf_finish_norm, dim3(1), dim3(64), 0, g->stream, k, g->prz, g->nrm);
+    hipLaunchKernelGGL(gf_finish_norm, dim3(k), dim3(256), 0, g->stream, k, g->prz, g->nrm);
     PLX_HIP(dm, hipGetLastError());
     out.resize(k);
     PLX_HIP(dm, hipMemcpyAsync(out.data(), g->nrm, k * sizeof(double), hipMemcpyDeviceToHost, g->stream));
