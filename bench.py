@@ -285,7 +285,7 @@ def bench_c5(torch, n_requests=120):
     the arrival rate is 70% of the capacity measured with all requests queued at once."""
     from pylith_amd import serving
     sizes, probs = (16, 32, 64, 96), (0.4, 0.3, 0.2, 0.1)
-    eng = serving.ServingEngine(sizes, block=16, rtol=1e-8)
+    eng = serving.ServingEngine(sizes, block=32, rtol=1e-8)
     t0 = time.perf_counter()
     eng.warm()
     t_warm = time.perf_counter() - t0
@@ -304,7 +304,30 @@ def bench_c5(torch, n_requests=120):
     return {"workload": "Poisson stream, meshes 16^3/32^3/64^3/96^3 at 40/30/20/10%, 64 fault patches",
             "requests": n_requests, "capacity_solves_per_s": cap, "arrival_rate_per_s": rate,
             "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "solves_per_s": stats["solves_per_s"],
-            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-8, "block": 16}
+            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-8, "max_block": 32}
+
+
+def bench_q1(torch, mg, n_edge, steps, warmup):
+    """Q1 hex FP64 residual apply on an n^3 cube (6 km, u ~ U[-1e-3, 1e-3] m with the shear
+    Dirichlet values inserted): wall time per apply, tile-kernel time and HBM fractions."""
+    hm = mg.hex_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    haux, hs = mg.elastic_aux(hm.num_cells)
+    hu = mg.random_field(hm.num_nodes, SEED)
+    dofs_bc, vals = mg.shear_bcs(hm.node_coords, 0.0, 6000.0)
+    hu[dofs_bc] = vals
+    hdm = setup(hm, haux, hs)
+    hx = torch.from_numpy(hu).cuda()
+    hr = torch.zeros_like(hx)
+    f = lambda: hdm.compute_residual(0, 0, hx, hr, mode=1)  # noqa: E731
+    s = time_applies(f, steps, warmup, torch)
+    km, _ = kernel_avg_ms(hdm, f, steps, torch)
+    hm_ms, _ = kernel_avg_ms(hdm, f, steps, torch, "plx:reduce_halo")
+    hb = q1_bytes(hm.num_nodes, hm.num_cells, 8)
+    out = {"gdofs": 3 * hm.num_nodes / s / 1e9, "ms": s * 1e3, "kernel_ms": km, "halo_reduce_ms": hm_ms,
+           "algorithmic_bytes": hb, "hbm_frac_kernel": hb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "hbm_frac_apply": hb / s / 1e9 / HBM_PEAK_GBS}
+    hdm.close()
+    return out
 
 
 def main():
@@ -317,7 +340,7 @@ def main():
     ap.add_argument("--q1-large", type=int, default=192, help="edge of the large Q1 cube")
     ap.add_argument("--no-systems", action="store_true",
                     help="skip the solver / viscoelastic / Green's-function / serving measurements")
-    ap.add_argument("--only", choices=["c2solve", "c3", "c4", "c5"], default=None,
+    ap.add_argument("--only", choices=["c2solve", "c3", "c4", "c5", "q1"], default=None,
                     help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -336,7 +359,8 @@ def main():
     if args.only:
         from pylith_amd import _lib as L
         fn = {"c2solve": lambda: bench_c2_solve(torch, mg, L), "c3": lambda: bench_c3(torch, mg, L),
-              "c4": lambda: bench_c4(torch, mg, L), "c5": lambda: bench_c5(torch)}[args.only]
+              "c4": lambda: bench_c4(torch, mg, L), "c5": lambda: bench_c5(torch),
+              "q1": lambda: bench_q1(torch, mg, args.q1_large, args.steps, args.warmup)}[args.only]
         print(f"[bench] {args.only}", file=sys.stderr, flush=True)
         print(json.dumps({args.only: fn()}))
         return
@@ -445,27 +469,8 @@ def main():
         torch.cuda.empty_cache()
         for n_edge in (32, args.q1_large):
             print(f"[bench] Q1 hex {n_edge}^3", file=sys.stderr, flush=True)
-            hm = mg.hex_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
-            haux, hs = mg.elastic_aux(hm.num_cells)
-            hu = mg.random_field(hm.num_nodes, SEED)
-            dofs_bc, vals = mg.shear_bcs(hm.node_coords, 0.0, 6000.0)
-            hu[dofs_bc] = vals
-            hdm = setup(hm, haux, hs)
-            hx = torch.from_numpy(hu).cuda()
-            hr = torch.zeros_like(hx)
-            f = lambda: hdm.compute_residual(0, 0, hx, hr, mode=1)  # noqa: E731
             steps = args.steps if n_edge <= 64 else max(10, args.steps // 2)
-            s = time_applies(f, steps, args.warmup, torch)
-            km, _ = kernel_avg_ms(hdm, f, steps, torch)
-            hb = q1_bytes(hm.num_nodes, hm.num_cells, 8)
-            extra[f"q1_fp64_residual_{n_edge}cube"] = {
-                "gdofs": 3 * hm.num_nodes / s / 1e9, "ms": s * 1e3, "kernel_ms": km,
-                "algorithmic_bytes": hb,
-                "hbm_frac_kernel": hb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                "hbm_frac_apply": hb / s / 1e9 / HBM_PEAK_GBS,
-            }
-            hdm.close()
-            del hx, hr
+            extra[f"q1_fp64_residual_{n_edge}cube"] = bench_q1(torch, mg, n_edge, steps, args.warmup)
             torch.cuda.empty_cache()
         result["extra"] = extra
     if not args.no_systems:

@@ -41,6 +41,7 @@ class MeshState:
         self.prob = workloads.faulted_box(size, seed=seed, vs_sigma=0.2)
         self.gf = GreensFunctions(self.prob.dm)
         self.gf.set_options(block=block)
+        self.block = block
         self.rtol = rtol
         self.patch = workloads.patch_index(self.prob.fault, *n_patch)
         self.n_patches = n_patch[0] * n_patch[1]
@@ -53,6 +54,9 @@ class MeshState:
         slip = workloads.patch_slip(self.prob.fault, self.patch, np.stack([r.slip_values for r in reqs]))
         s = torch.from_numpy(slip).cuda()
         u = torch.zeros(len(reqs), self.prob.local_size, dtype=torch.float64, device="cuda")
+        # the narrowest block width that holds the batch: no work on empty columns
+        width = next(k for k in (8, 16, 32, 64) if k >= min(len(reqs), self.block))
+        self.gf.set_options(block=width)
         st = self.gf.solve(s, u, rtol=self.rtol, maxit=30)
         return u, st
 
