@@ -167,3 +167,48 @@ def test_two_blocks_rigid_motion(oracle):
     assert np.abs(u - exact).max() <= 1e-9 * 2.0
     # zero traction on the fault: lambda vanishes relative to mu * slip / h
     assert np.abs(got[nu:]).max() <= 1e-9 * 2.25e10 * 2.0 / 1000.0
+
+
+def test_fault_solve_with_lagrange_rhs_matches_dense(oracle):
+    """A right-hand side in the Lagrange rows (b_lambda != 0) takes the fault-mass solve for the
+    jump (not the planar shortcut): compare with the dense saddle-point solve of F(x) = b."""
+    from pylith_amd.dm import Solver
+    m, f, aux, sizes, dm = faulted((4, 3, 3), 2, shear=0.25)
+    nu = 3 * m.num_nodes
+    nl = 3 * f.num_lag
+    x0 = m.node_coords[:, 0] - 0.25 * m.node_coords[:, 2]
+    neg = np.nonzero(np.isclose(x0, -2000.0))[0]
+    pos = np.nonzero(np.isclose(x0, 2000.0))[0]
+    dofs = np.sort(np.concatenate([3 * neg, 3 * neg + 1, 3 * neg + 2, 3 * pos + 1])).astype(np.int32)
+    dm.set_constraints(dofs)
+    slip = mg.tapered_patch_slip(f, (1500.0, 1500.0), 600.0, 1.0, component=2, taper=800.0)
+    dm.set_fault_slip(slip)
+    rng = np.random.default_rng(17)
+    b = np.zeros(nu + nl)
+    b[nu:] = rng.standard_normal(nl) * 1e3   # Lagrange rows: an imposed extra jump (area-weighted)
+    b[:nu] = rng.standard_normal(nu) * 1e6   # body-force-like load on the displacement rows
+    b[dofs] = 0.0
+    # dense: F(x) = -b  <=>  J x = -F(0) - b
+    P = oracle.Problem(m, aux, sizes)
+    C0 = oracle.Cohesive(m, f, np.zeros((f.num_lag, 3)), nu)
+    n = nu + nl
+    J = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        col = C0.jacobian_action(e)
+        col[:nu] += P.residual(e[:nu], F1)
+        J[:, i] = col
+    F0 = oracle.Cohesive(m, f, slip, nu).residual(np.zeros(n))
+    free = np.setdiff1d(np.arange(n), dofs)
+    ref = np.zeros(n)
+    rhs = -F0 - b
+    ref[free] = np.linalg.solve(J[np.ix_(free, free)], rhs[free])
+    print("[fault solve] Lagrange right-hand side", flush=True)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    st = Solver(dm).solve(x, b=dev(b), method=L.KSP_CG, rtol=1e-12, maxit=2000)
+    torch.cuda.synchronize()
+    got = x.cpu().numpy()
+    assert st["converged"], st
+    assert relerr(got[:nu], ref[:nu]) <= 1e-9
+    assert relerr(got[nu:], ref[nu:]) <= 1e-8
