@@ -246,7 +246,7 @@ def bench_c4(torch, mg, L, n_impulses=512):
     # reference solutions (tighter tolerance) for the error
     solver = Solver(prob.dm)
     sample = [0, nimp // 3, (2 * nimp) // 3, nimp - 1]
-    errs, errs64, t64 = [], [], []
+    errs, errs64, t64, refs = [], [], [], []
     for j in sample:
         prob.dm.set_fault_slip(slip[j])
         x = torch.zeros(nloc, dtype=torch.float64, device="cuda")
@@ -258,10 +258,32 @@ def bench_c4(torch, mg, L, n_impulses=512):
         xr = torch.zeros_like(x)
         solver.solve(xr, method=L.KSP_CG, rtol=1e-14, maxit=40000)
         ref = xr.cpu().numpy()[:nu]
+        refs.append(ref)
         errs.append(float(np.abs(U[j].cpu().numpy()[:nu] - ref).max() / np.abs(ref).max()))
         errs64.append(float(np.abs(x.cpu().numpy()[:nu] - ref).max() / np.abs(ref).max()))
     fp64_rate = 1.0 / float(np.mean(t64))
     gf_rate = nimp / t_gf
+    # the same block solves preconditioned by the smoothed-aggregation multigrid V-cycle
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gf.set_preconditioner("amg")
+    t_amg_setup = time.perf_counter() - t0
+    best_a, probe_a = gf.autotune()
+    Ua = torch.zeros_like(U)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sta = gf.solve(s_dev, Ua, rtol=rtol, maxit=30)
+    torch.cuda.synchronize()
+    t_amg = time.perf_counter() - t0
+    errs_a = []
+    for j, refv in zip(sample, refs):
+        errs_a.append(float(np.abs(Ua[j].cpu().numpy()[:nu] - refv).max() / np.abs(refv).max()))
+    amg = {"setup_s": t_amg_setup, "block": sta["block"], "probe_us_per_rhs_iteration": probe_a,
+           "seconds": t_amg, "solves_per_s": nimp / t_amg,
+           "outer_iterations": sta["outer_iterations"], "inner_iterations": sta["inner_iterations"],
+           "max_rel_residual": sta["max_relative_residual"], "max_rel_error_vs_fp64": max(errs_a),
+           "speedup_vs_fp64": (nimp / t_amg) / fp64_rate,
+           "speedup_vs_fp64_incl_setup": (nimp / (t_amg + t_amg_setup)) / fp64_rate}
     out = {
         "workload": "64^3 Q1, 80 km cube, fault dipping 60 deg, 32 x 16 vertex impulses (512 RHS)",
         "impulses": nimp, "block": st["block"], "autotune_s": t_tune,
@@ -273,6 +295,7 @@ def bench_c4(torch, mg, L, n_impulses=512):
         "fp64_path_max_rel_error": max(errs64),
         "fp64_single_solves_per_s": fp64_rate,
         "speedup_vs_fp64": gf_rate / fp64_rate,
+        "multigrid": amg,
     }
     solver.close()
     gf.close()

@@ -250,6 +250,13 @@ typedef struct {
 int plx_gf_create(plx_gf *gf, plx_dm dm, int label_value, int part);
 int plx_gf_destroy(plx_gf gf);
 int plx_gf_set_options(plx_gf gf, int block, double inner_rtol, int inner_maxit, int use_graph);
+/* Preconditioner of the FP32 block PCG: point-block Jacobi (default) or smoothed-aggregation
+ * algebraic multigrid, the role of the GAMG preconditioner PyLith configures for elasticity
+ * (materials/Elasticity.cc:307-318): aggregates of the node graph, rigid-body near-null space,
+ * smoothed prolongator, Galerkin coarse operators, one V-cycle with Chebyshev smoothing per
+ * iteration.  The hierarchy is built on the first switch to AMG (host set-up, FP32 on the device). */
+enum { PLX_GF_PC_JACOBI = 0, PLX_GF_PC_AMG = 1 };
+int plx_gf_set_preconditioner(plx_gf gf, int kind);
 /* Time probe iterations for k in {8, 16, 32, 64}; keeps the fastest per impulse. */
 int plx_gf_autotune(plx_gf gf, int *best_block);
 /* Microseconds per impulse per FP32 iteration measured by the last autotune, for k = 8..64. */

@@ -1,0 +1,436 @@
+// plx_amg.cpp -- host set-up of the smoothed-aggregation hierarchy (see plx_amg.hpp).
+#include "plx_amg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <random>
+
+namespace plx {
+namespace {
+
+// Inverse of a b x b block (b <= 6) by Gauss-Jordan with partial pivoting; a singular block
+// (zero pivot) yields the identity (its rows are decoupled dummy dofs).
+void block_inverse(const double *A, int b, double *Ai) {
+    double M[6][12];
+    double scale = 0.0;
+    for (int i = 0; i < b; ++i)
+        for (int j = 0; j < b; ++j) {
+            M[i][j] = A[i * b + j];
+            M[i][b + j] = i == j ? 1.0 : 0.0;
+            scale = std::max(scale, std::fabs(A[i * b + j]));
+        }
+    for (int c = 0; c < b; ++c) {
+        int p = c;
+        for (int r = c + 1; r < b; ++r)
+            if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+        if (!(std::fabs(M[p][c]) > 1e-14 * scale)) {
+            for (int i = 0; i < b * b; ++i) Ai[i] = (i % (b + 1) == 0) ? 1.0 : 0.0;
+            return;
+        }
+        if (p != c)
+            for (int j = 0; j < 2 * b; ++j) std::swap(M[p][j], M[c][j]);
+        const double inv = 1.0 / M[c][c];
+        for (int j = 0; j < 2 * b; ++j) M[c][j] *= inv;
+        for (int r = 0; r < b; ++r)
+            if (r != c) {
+                const double f = M[r][c];
+                if (f != 0.0)
+                    for (int j = 0; j < 2 * b; ++j) M[r][j] -= f * M[c][j];
+            }
+    }
+    for (int i = 0; i < b; ++i)
+        for (int j = 0; j < b; ++j) Ai[i * b + j] = M[i][b + j];
+}
+
+// y (b x 6) += alpha * A (b x b) * X (b x 6)
+inline void gemm_b6(int b, double alpha, const double *A, const double *X, double *y) {
+    for (int i = 0; i < b; ++i)
+        for (int k = 0; k < b; ++k) {
+            const double a = alpha * A[i * b + k];
+            if (a == 0.0) continue;
+            for (int j = 0; j < 6; ++j) y[i * 6 + j] += a * X[k * 6 + j];
+        }
+}
+
+struct Csr6 {  // block rows of b x 6 blocks, sorted columns
+    std::vector<int32_t> ptr, col;
+    std::vector<double> val;
+};
+
+// Sparse product rows: out_i = sum_j A_ij * M_j for M with b x 6 blocks; `mark`/`list` scratch.
+void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const double *Aval, const Csr6 &M,
+                int ncols, Csr6 *out) {
+    std::vector<int> mark(ncols, -1);
+    std::vector<int> list;
+    std::vector<double> acc;
+    const int bb = b * 6;
+    out->ptr.assign(1, 0);
+    out->col.clear();
+    out->val.clear();
+    for (int i = 0; i < n; ++i) {
+        list.clear();
+        acc.clear();
+        for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
+            const int j = Acol[e];
+            const double *A = Aval + (size_t)e * b * b;
+            for (int f = M.ptr[j]; f < M.ptr[j + 1]; ++f) {
+                const int c = M.col[f];
+                if (mark[c] < 0) {
+                    mark[c] = (int)list.size();
+                    list.push_back(c);
+                    acc.resize(acc.size() + bb, 0.0);
+                }
+                gemm_b6(b, 1.0, A, M.val.data() + (size_t)f * bb, acc.data() + (size_t)mark[c] * bb);
+            }
+        }
+        std::vector<int> order(list.size());
+        for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+        std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
+        for (int t : order) {
+            out->col.push_back(list[t]);
+            out->val.insert(out->val.end(), acc.begin() + (size_t)t * bb, acc.begin() + (size_t)(t + 1) * bb);
+        }
+        for (int c : list) mark[c] = -1;
+        out->ptr.push_back((int32_t)out->col.size());
+    }
+}
+
+}  // namespace
+
+int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *val0, const double *coords,
+              const uint8_t *fixed0, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err) {
+    h->levels.clear();
+    h->coarse_inv.clear();
+    // current level operator
+    int n = n0, b = 3;
+    std::vector<int32_t> Aptr(ptr0, ptr0 + n0 + 1), Acol(col0, col0 + ptr0[n0]);
+    std::vector<double> Aval(val0, val0 + (size_t)ptr0[n0] * 9);
+    std::vector<uint8_t> fixed(fixed0, fixed0 + 3 * (size_t)n0);
+    std::vector<double> B;  // near-null space [n*b][6], built per aggregate on level 0
+    for (int lev = 0;; ++lev) {
+        AmgLevel L;
+        L.n = n;
+        L.b = b;
+        const int bb = b * b;
+        // inverse diagonal blocks
+        L.Dinv.assign((size_t)n * bb, 0.0);
+        for (int i = 0; i < n; ++i) {
+            int d = -1;
+            for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+                if (Acol[e] == i) d = e;
+            if (d < 0) {
+                *err = "amg: missing diagonal block";
+                return 1;
+            }
+            block_inverse(Aval.data() + (size_t)d * bb, b, L.Dinv.data() + (size_t)i * bb);
+        }
+        // aggregation on the node graph; nodes with every dof fixed stay out
+        std::vector<uint8_t> excl(n, 0);
+        for (int i = 0; i < n; ++i) {
+            bool all = true;
+            for (int c = 0; c < b; ++c) all = all && fixed[(size_t)i * b + c];
+            excl[i] = all;
+        }
+        std::vector<int> agg(n, -1);
+        int na = 0;
+        for (int i = 0; i < n; ++i) {
+            if (excl[i] || agg[i] >= 0) continue;
+            bool free_nbhd = true;
+            for (int e = Aptr[i]; e < Aptr[i + 1] && free_nbhd; ++e) {
+                const int j = Acol[e];
+                if (!excl[j] && agg[j] >= 0) free_nbhd = false;
+            }
+            if (!free_nbhd) continue;
+            for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+                if (!excl[Acol[e]]) agg[Acol[e]] = na;
+            agg[i] = na++;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (excl[i] || agg[i] >= 0) continue;
+            for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
+                const int j = Acol[e];
+                if (!excl[j] && agg[j] >= 0) {
+                    agg[i] = agg[j];
+                    break;
+                }
+            }
+            if (agg[i] < 0) agg[i] = na++;
+        }
+        if (na == 0) {
+            *err = "amg: no free nodes";
+            return 1;
+        }
+        std::vector<std::vector<int>> members(na);
+        for (int i = 0; i < n; ++i)
+            if (agg[i] >= 0) members[agg[i]].push_back(i);
+        // near-null space: rigid-body modes about each aggregate's centroid (level 0), the QR
+        // factors of the finer level below
+        if (lev == 0) {
+            B.assign((size_t)n * 3 * 6, 0.0);
+            for (int a = 0; a < na; ++a) {
+                double cen[3] = {0, 0, 0};
+                for (int i : members[a])
+                    for (int d = 0; d < 3; ++d) cen[d] += coords[3 * (size_t)i + d] / members[a].size();
+                for (int i : members[a]) {
+                    const double x = coords[3 * (size_t)i] - cen[0], y = coords[3 * (size_t)i + 1] - cen[1],
+                                 z = coords[3 * (size_t)i + 2] - cen[2];
+                    double *r0 = &B[(3 * (size_t)i) * 6], *r1 = r0 + 6, *r2 = r1 + 6;
+                    r0[0] = 1.0;
+                    r1[1] = 1.0;
+                    r2[2] = 1.0;
+                    r0[3] = -y;  // rotation about z
+                    r1[3] = x;
+                    r1[4] = -z;  // about x
+                    r2[4] = y;
+                    r0[5] = z;  // about y
+                    r2[5] = -x;
+                }
+            }
+            for (size_t d = 0; d < (size_t)n * 3; ++d)
+                if (fixed[d])
+                    for (int k = 0; k < 6; ++k) B[d * 6 + k] = 0.0;
+        }
+        // tentative prolongator: per-aggregate modified Gram-Schmidt (twice) of the null space rows
+        Csr6 P0;
+        P0.ptr.assign(n + 1, 0);
+        std::vector<double> Q((size_t)n * b * 6, 0.0);  // P0 block of each fine row
+        std::vector<double> Bc((size_t)na * 6 * 6, 0.0);
+        for (int a = 0; a < na; ++a) {
+            const auto &mem = members[a];
+            const int rows = (int)mem.size() * b;
+            std::vector<double> M((size_t)rows * 6);
+            for (size_t t = 0; t < mem.size(); ++t)
+                for (int c = 0; c < b; ++c)
+                    for (int k = 0; k < 6; ++k) M[(t * b + c) * 6 + k] = B[((size_t)mem[t] * b + c) * 6 + k];
+            double colmax = 0.0;
+            for (int k = 0; k < 6; ++k) {
+                double s = 0.0;
+                for (int r = 0; r < rows; ++r) s += M[(size_t)r * 6 + k] * M[(size_t)r * 6 + k];
+                colmax = std::max(colmax, std::sqrt(s));
+            }
+            double R[6][6] = {};
+            for (int k = 0; k < 6; ++k) {
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int j = 0; j < k; ++j) {
+                        double s = 0.0;
+                        for (int r = 0; r < rows; ++r) s += M[(size_t)r * 6 + j] * M[(size_t)r * 6 + k];
+                        R[j][k] += s;
+                        for (int r = 0; r < rows; ++r) M[(size_t)r * 6 + k] -= s * M[(size_t)r * 6 + j];
+                    }
+                double nk = 0.0;
+                for (int r = 0; r < rows; ++r) nk += M[(size_t)r * 6 + k] * M[(size_t)r * 6 + k];
+                nk = std::sqrt(nk);
+                if (nk > 1e-10 * colmax) {
+                    R[k][k] = nk;
+                    for (int r = 0; r < rows; ++r) M[(size_t)r * 6 + k] /= nk;
+                } else {  // rank-deficient aggregate: a dummy coarse dof
+                    for (int r = 0; r < rows; ++r) M[(size_t)r * 6 + k] = 0.0;
+                    for (int j = 0; j < 6; ++j) R[k][j] = 0.0;
+                }
+            }
+            for (size_t t = 0; t < mem.size(); ++t)
+                for (int c = 0; c < b; ++c)
+                    for (int k = 0; k < 6; ++k) Q[((size_t)mem[t] * b + c) * 6 + k] = M[(t * b + c) * 6 + k];
+            for (int k = 0; k < 6; ++k)
+                for (int j = 0; j < 6; ++j) Bc[((size_t)a * 6 + k) * 6 + j] = R[k][j];
+        }
+        for (int i = 0; i < n; ++i) P0.ptr[i + 1] = P0.ptr[i] + (agg[i] >= 0 ? 1 : 0);
+        P0.col.resize(P0.ptr[n]);
+        P0.val.resize((size_t)P0.ptr[n] * b * 6);
+        for (int i = 0; i < n; ++i)
+            if (agg[i] >= 0) {
+                P0.col[P0.ptr[i]] = agg[i];
+                std::copy(&Q[(size_t)i * b * 6], &Q[(size_t)(i + 1) * b * 6], &P0.val[(size_t)P0.ptr[i] * b * 6]);
+            }
+        // spectrum bound of D^-1 A (power iteration, 1.1 x the estimate)
+        {
+            std::mt19937_64 rng(7);
+            std::normal_distribution<double> nd;
+            std::vector<double> v((size_t)n * b), w((size_t)n * b), t((size_t)n * b);
+            for (auto &x : v) x = nd(rng);
+            double lam = 0.0;
+            for (int it = 0; it < 20; ++it) {
+                double vn = 0.0;
+                for (double x : v) vn += x * x;
+                vn = std::sqrt(vn);
+                for (auto &x : v) x /= vn;
+                std::fill(t.begin(), t.end(), 0.0);
+                for (int i = 0; i < n; ++i)
+                    for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
+                        const double *A = &Aval[(size_t)e * bb];
+                        const double *x = &v[(size_t)Acol[e] * b];
+                        for (int r = 0; r < b; ++r)
+                            for (int c = 0; c < b; ++c) t[(size_t)i * b + r] += A[r * b + c] * x[c];
+                    }
+                double wn = 0.0;
+                for (int i = 0; i < n; ++i)
+                    for (int r = 0; r < b; ++r) {
+                        double s = 0.0;
+                        for (int c = 0; c < b; ++c) s += L.Dinv[(size_t)i * bb + r * b + c] * t[(size_t)i * b + c];
+                        w[(size_t)i * b + r] = s;
+                        wn += s * s;
+                    }
+                lam = std::sqrt(wn);
+                v.swap(w);
+            }
+            L.lmax = 1.1 * lam;
+        }
+        // smoothed prolongator P = P0 - omega D^-1 A P0
+        Csr6 AP0, P;
+        spgemm_A_M(n, b, Aptr.data(), Acol.data(), Aval.data(), P0, na, &AP0);
+        const double omega = 4.0 / (3.0 * L.lmax);
+        P.ptr.assign(1, 0);
+        for (int i = 0; i < n; ++i) {
+            for (int f = AP0.ptr[i]; f < AP0.ptr[i + 1]; ++f) {
+                const int c = AP0.col[f];
+                std::vector<double> blk(b * 6, 0.0);
+                if (agg[i] == c)
+                    std::copy(&P0.val[(size_t)P0.ptr[i] * b * 6], &P0.val[(size_t)(P0.ptr[i] + 1) * b * 6], blk.begin());
+                gemm_b6(b, -omega, &L.Dinv[(size_t)i * bb], &AP0.val[(size_t)f * b * 6], blk.data());
+                P.col.push_back(c);
+                P.val.insert(P.val.end(), blk.begin(), blk.end());
+            }
+            // P0's own aggregate is always among AP0's columns when row i is free (A_ii P0_i); a
+            // fixed row keeps P = 0 there (its A row is the identity and P0 row is zero)
+            P.ptr.push_back((int32_t)P.col.size());
+        }
+        // Galerkin coarse operator Ac = P^T (A P)
+        Csr6 AP;
+        spgemm_A_M(n, b, Aptr.data(), Acol.data(), Aval.data(), P, na, &AP);
+        // transpose of P: for coarse row a, the fine rows i with their b x 6 blocks
+        std::vector<int32_t> Rptr(na + 1, 0), Rrow(P.col.size()), Rsrc(P.col.size());
+        for (int32_t c : P.col) Rptr[c + 1]++;
+        for (int a = 0; a < na; ++a) Rptr[a + 1] += Rptr[a];
+        {
+            std::vector<int32_t> fill(Rptr.begin(), Rptr.end() - 1);
+            for (int i = 0; i < n; ++i)
+                for (int f = P.ptr[i]; f < P.ptr[i + 1]; ++f) {
+                    Rrow[fill[P.col[f]]] = i;
+                    Rsrc[fill[P.col[f]]++] = f;
+                }
+        }
+        std::vector<int32_t> Cptr(1, 0), Ccol;
+        std::vector<double> Cval;
+        {
+            std::vector<int> mark(na, -1), list;
+            std::vector<double> acc;
+            for (int a = 0; a < na; ++a) {
+                list.clear();
+                acc.clear();
+                for (int t = Rptr[a]; t < Rptr[a + 1]; ++t) {
+                    const int i = Rrow[t];
+                    const double *Pia = &P.val[(size_t)Rsrc[t] * b * 6];  // b x 6
+                    for (int f = AP.ptr[i]; f < AP.ptr[i + 1]; ++f) {
+                        const int c = AP.col[f];
+                        if (mark[c] < 0) {
+                            mark[c] = (int)list.size();
+                            list.push_back(c);
+                            acc.resize(acc.size() + 36, 0.0);
+                        }
+                        const double *APic = &AP.val[(size_t)f * b * 6];  // b x 6
+                        double *o = &acc[(size_t)mark[c] * 36];
+                        for (int r = 0; r < 6; ++r)
+                            for (int k = 0; k < b; ++k) {
+                                const double p = Pia[k * 6 + r];
+                                if (p == 0.0) continue;
+                                for (int s = 0; s < 6; ++s) o[r * 6 + s] += p * APic[k * 6 + s];
+                            }
+                    }
+                }
+                std::vector<int> order(list.size());
+                for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+                std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
+                bool has_diag = false;
+                for (int t : order) {
+                    double *o = &acc[(size_t)t * 36];
+                    if (list[t] == a) {
+                        has_diag = true;
+                        for (int r = 0; r < 6; ++r)  // dummy (rank-deficient) coarse dofs: identity
+                            if (o[r * 6 + r] == 0.0) o[r * 6 + r] = 1.0;
+                    }
+                    Ccol.push_back(list[t]);
+                    Cval.insert(Cval.end(), o, o + 36);
+                }
+                if (!has_diag) {  // an aggregate disconnected from everything: identity block
+                    Ccol.push_back(a);
+                    for (int r = 0; r < 36; ++r) Cval.push_back(r % 7 == 0 ? 1.0 : 0.0);
+                }
+                for (int c : list) mark[c] = -1;
+                Cptr.push_back((int32_t)Ccol.size());
+            }
+        }
+        // store this level's transfer operators
+        L.nc = na;
+        L.Pptr = P.ptr;
+        L.Pcol = P.col;
+        L.Pval = P.val;
+        L.Rptr = Rptr;
+        L.Rcol = Rrow;
+        L.Rval.resize((size_t)P.col.size() * 6 * b);
+        for (int t = 0; t < (int)Rrow.size(); ++t) {
+            const double *Pia = &P.val[(size_t)Rsrc[t] * b * 6];
+            for (int r = 0; r < 6; ++r)
+                for (int k = 0; k < b; ++k) L.Rval[(size_t)t * 6 * b + r * b + k] = Pia[k * 6 + r];
+        }
+        if (lev > 0) {
+            L.Aptr = Aptr;
+            L.Acol = Acol;
+            L.Aval = Aval;
+        }
+        h->levels.push_back(std::move(L));
+        // next level
+        n = na;
+        b = 6;
+        Aptr.swap(Cptr);
+        Acol.swap(Ccol);
+        Aval.swap(Cval);
+        B.swap(Bc);
+        fixed.assign((size_t)n * 6, 0);
+        if (6 * n <= coarse_max || (int)h->levels.size() >= max_levels) break;
+    }
+    // coarsest: dense inverse (Gauss-Jordan with partial pivoting)
+    const int m = 6 * n;
+    if (m > 4096) {
+        *err = "amg: coarsest level too large";
+        return 1;
+    }
+    std::vector<double> D((size_t)m * m, 0.0), I((size_t)m * m, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) D[(size_t)(6 * i + r) * m + 6 * Acol[e] + c] = Aval[(size_t)e * 36 + r * 6 + c];
+    for (int i = 0; i < m; ++i) I[(size_t)i * m + i] = 1.0;
+    for (int c = 0; c < m; ++c) {
+        int p = c;
+        for (int r = c + 1; r < m; ++r)
+            if (std::fabs(D[(size_t)r * m + c]) > std::fabs(D[(size_t)p * m + c])) p = r;
+        if (D[(size_t)p * m + c] == 0.0) {
+            *err = "amg: singular coarsest operator";
+            return 1;
+        }
+        if (p != c)
+            for (int j = 0; j < m; ++j) {
+                std::swap(D[(size_t)p * m + j], D[(size_t)c * m + j]);
+                std::swap(I[(size_t)p * m + j], I[(size_t)c * m + j]);
+            }
+        const double inv = 1.0 / D[(size_t)c * m + c];
+        for (int j = 0; j < m; ++j) {
+            D[(size_t)c * m + j] *= inv;
+            I[(size_t)c * m + j] *= inv;
+        }
+        for (int r = 0; r < m; ++r)
+            if (r != c) {
+                const double f = D[(size_t)r * m + c];
+                if (f == 0.0) continue;
+                for (int j = 0; j < m; ++j) {
+                    D[(size_t)r * m + j] -= f * D[(size_t)c * m + j];
+                    I[(size_t)r * m + j] -= f * I[(size_t)c * m + j];
+                }
+            }
+    }
+    h->coarse_n = m;
+    h->coarse_inv.swap(I);
+    return 0;
+}
+
+}  // namespace plx

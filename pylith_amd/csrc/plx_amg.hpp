@@ -1,0 +1,43 @@
+// plx_amg.hpp -- smoothed-aggregation algebraic multigrid for the assembled elasticity operator
+// (host set-up in plx_amg.cpp, device V-cycle in plx_gf.hip).
+//
+// The role of PETSc's GAMG preconditioner that PyLith selects for elasticity
+// (share/settings/solver_elasticity.cfg, materials/Elasticity.cc:307-318): aggregates of
+// strongly connected nodes, the six rigid-body modes as near-null space (translations and
+// rotations from the node coordinates), a per-aggregate QR for the tentative prolongator, one
+// damped-Jacobi smoothing step of it, Galerkin coarse operators, Chebyshev smoothing on every level
+// and a dense inverse on the coarsest.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace plx {
+
+struct AmgLevel {
+    int n = 0;       // block rows
+    int b = 0;       // block size (3 on the finest level, 6 below)
+    int nc = 0;      // block rows of the next level
+    // operator of this level (block CSR, b x b blocks row-major); level 0 uses the caller's copy
+    std::vector<int32_t> Aptr, Acol;
+    std::vector<double> Aval;
+    // prolongator (n rows, b x 6 blocks) and its transpose (nc rows, 6 x b blocks)
+    std::vector<int32_t> Pptr, Pcol, Rptr, Rcol;
+    std::vector<double> Pval, Rval;
+    std::vector<double> Dinv;  // [n][b][b] inverse diagonal blocks
+    double lmax = 0.0;         // upper bound of the spectrum of D^-1 A
+};
+
+struct AmgHierarchy {
+    std::vector<AmgLevel> levels;  // levels[l] maps to levels[l+1] (or to the coarsest)
+    int coarse_n = 0;              // dofs of the coarsest operator
+    std::vector<double> coarse_inv;  // dense inverse of the coarsest operator [coarse_n][coarse_n]
+};
+
+// Build the hierarchy of the 3x3-block operator (ptr, col, val) on n nodes.  fixed[3 n] marks
+// identity rows/columns (constrained dofs, positive split nodes); coords [n][3].
+int amg_setup(int n, const int32_t *ptr, const int32_t *col, const double *val, const double *coords,
+              const uint8_t *fixed, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err);
+
+}  // namespace plx

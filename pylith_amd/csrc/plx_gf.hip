@@ -21,6 +21,7 @@
 #include <cmath>
 #include <vector>
 
+#include "plx_amg.hpp"
 #include "plx_element.hpp"
 #include "plx_internal.hpp"
 
@@ -390,6 +391,207 @@ __global__ void __launch_bounds__(256) gfk_pupdate(int64_t n, const double *__re
         P[t] = Z[t] + b * P[t];
 }
 
+// ---- multigrid preconditioner kernels ([dof][K] layout, thread = (block row, column)) ----
+// Y (+)= A X for BR x BC blocks
+template <int BR, int BC, int K>
+__global__ void __launch_bounds__(256) amg_spmm(int nrows, const int32_t *__restrict__ ptr,
+                                                const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                const float *__restrict__ X, float *Y, int accumulate) {
+    const int64_t n = (int64_t)nrows * K;
+    const int r = (int)(threadIdx.x % K);
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / K;
+        float y[BR];
+#pragma unroll
+        for (int a = 0; a < BR; ++a) y[a] = 0.f;
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+            const int64_t j = col[e];
+            const float *A = val + (size_t)BR * BC * e;
+            float x[BC];
+#pragma unroll
+            for (int c = 0; c < BC; ++c) x[c] = X[(BC * j + c) * K + r];
+#pragma unroll
+            for (int a = 0; a < BR; ++a)
+#pragma unroll
+                for (int c = 0; c < BC; ++c) y[a] += A[a * BC + c] * x[c];
+        }
+#pragma unroll
+        for (int a = 0; a < BR; ++a) {
+            const size_t o = (BR * (size_t)i + a) * K + r;
+            Y[o] = accumulate ? Y[o] + y[a] : y[a];
+        }
+    }
+}
+
+// K = 64, 3 x 3 blocks: the wave-per-row product of the fine operator (no dot product)
+__global__ void __launch_bounds__(256) amg_spmm33_64(int nrows, const int32_t *__restrict__ row_ptr,
+                                                     const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                     const float *__restrict__ X, float *Y) {
+    const int lane = threadIdx.x & 63;
+    int lo, hi, jb, nbx;
+    gf_xcd_slab(nrows, lo, hi, jb, nbx);
+    for (int row0 = lo + jb * 4 + (int)(threadIdx.x >> 6); row0 < hi; row0 += nbx * 4) {
+        const int row = __builtin_amdgcn_readfirstlane(row0);
+        const int e0 = row_ptr[row], e1 = row_ptr[row + 1];
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+        int e = e0;
+        for (; e + 9 <= e1; e += 9) {
+            float x[9][3];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const size_t j = (size_t)col[e + m];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) x[m][c] = X[(3 * j + c) * 64 + lane];
+            }
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const float *A = val + 9 * (size_t)(e + m);
+                y0 += A[0] * x[m][0] + A[1] * x[m][1] + A[2] * x[m][2];
+                y1 += A[3] * x[m][0] + A[4] * x[m][1] + A[5] * x[m][2];
+                y2 += A[6] * x[m][0] + A[7] * x[m][1] + A[8] * x[m][2];
+            }
+        }
+        for (; e < e1; ++e) {
+            const size_t j = (size_t)col[e];
+            const float *A = val + 9 * (size_t)e;
+            const float x0 = X[(3 * j) * 64 + lane], x1 = X[(3 * j + 1) * 64 + lane], x2 = X[(3 * j + 2) * 64 + lane];
+            y0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+            y1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+            y2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
+        }
+        const size_t o = (3 * (size_t)row) * 64 + lane;
+        Y[o] = y0;
+        Y[o + 64] = y1;
+        Y[o + 128] = y2;
+    }
+}
+
+// Chebyshev start: d = Dinv r / theta (and x = 0 when asked)
+template <int B, int K>
+__global__ void __launch_bounds__(256) amg_cheb_init(int n, const float *__restrict__ Dinv,
+                                                     const float *__restrict__ R, float *D, float *X, float inv_theta,
+                                                     int zero_x) {
+    const int64_t nt = (int64_t)n * K;
+    const int r = (int)(threadIdx.x % K);
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nt; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / K;
+        const float *Di = Dinv + (size_t)B * B * i;
+        float rv[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) rv[c] = R[(B * i + c) * K + r];
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+            float z = 0.f;
+#pragma unroll
+            for (int c = 0; c < B; ++c) z += Di[a * B + c] * rv[c];
+            const size_t o = (B * i + a) * K + r;
+            D[o] = inv_theta * z;
+            if (zero_x) X[o] = 0.f;
+        }
+    }
+}
+
+// Chebyshev step: x += d; r -= t (t = A d); d = c1 d + c2 Dinv r
+template <int B, int K>
+__global__ void __launch_bounds__(256) amg_cheb_step(int n, const float *__restrict__ Dinv,
+                                                     const float *__restrict__ T, float *R, float *D, float *X,
+                                                     float c1, float c2) {
+    const int64_t nt = (int64_t)n * K;
+    const int r = (int)(threadIdx.x % K);
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nt; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / K;
+        const float *Di = Dinv + (size_t)B * B * i;
+        float rv[B], dv[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const size_t o = (B * i + c) * K + r;
+            dv[c] = D[o];
+            X[o] += dv[c];
+            rv[c] = R[o] - T[o];
+            R[o] = rv[c];
+        }
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+            float z = 0.f;
+#pragma unroll
+            for (int c = 0; c < B; ++c) z += Di[a * B + c] * rv[c];
+            D[(B * i + a) * K + r] = c1 * dv[a] + c2 * z;
+        }
+    }
+}
+
+// r = b - t
+__global__ void __launch_bounds__(256) amg_resid(int64_t n, const float *__restrict__ Bv, const float *__restrict__ T,
+                                                 float *R) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) R[t] = Bv[t] - T[t];
+}
+
+__global__ void __launch_bounds__(256) amg_copy(int64_t n, const float *__restrict__ A, float *B) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) B[t] = A[t];
+}
+
+// coarsest level: x = Ainv b (dense, m x m)
+template <int K>
+__global__ void __launch_bounds__(256) amg_dense(int m, const float *__restrict__ Ainv, const float *__restrict__ Bv,
+                                                 float *X) {
+    const int64_t nt = (int64_t)m * K;
+    const int r = (int)(threadIdx.x % K);
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nt; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / K;
+        const float *a = Ainv + (size_t)m * i;
+        float acc = 0.f;
+        for (int j = 0; j < m; ++j) acc += a[j] * Bv[(size_t)j * K + r];
+        X[i * K + r] = acc;
+    }
+}
+
+// x += alpha p; r -= alpha q; partial sums of r.r (preconditioner applied separately)
+template <int K>
+__global__ void __launch_bounds__(256) gfk_update_xr(int64_t ndof, const double *__restrict__ scal, float *X,
+                                                     float *R, const float *__restrict__ P,
+                                                     const float *__restrict__ Q, double *prr) {
+    __shared__ double s[256];
+    double rr = 0.0;
+    const int64_t n = ndof * K;
+    const int rc = (int)(threadIdx.x % K);
+    const float a = (float)scal[G_ALPHA * 64 + rc];
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        X[t] += a * P[t];
+        const float rv = R[t] - a * Q[t];
+        R[t] = rv;
+        rr += (double)rv * rv;
+    }
+    s[threadIdx.x] = rr;
+    col_reduce(s, K);
+    if ((int)threadIdx.x < K) prr[(size_t)blockIdx.x * K + threadIdx.x] = s[threadIdx.x];
+}
+
+// per-column partials of a.b (and, when bb is set, of b.b) -- fixed column per thread
+template <int K>
+__global__ void __launch_bounds__(256) gfk_dot2(int64_t ndof, const float *__restrict__ a, const float *__restrict__ b,
+                                                double *pab, double *pbb) {
+    __shared__ double s1[256], s2[256];
+    double ab = 0.0, bb = 0.0;
+    const int64_t n = ndof * K;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        ab += (double)a[t] * b[t];
+        bb += (double)b[t] * b[t];
+    }
+    s1[threadIdx.x] = ab;
+    col_reduce(s1, K);
+    s2[threadIdx.x] = bb;
+    col_reduce(s2, K);
+    if ((int)threadIdx.x < K) {
+        pab[(size_t)blockIdx.x * K + threadIdx.x] = s1[threadIdx.x];
+        if (pbb) pbb[(size_t)blockIdx.x * K + threadIdx.x] = s2[threadIdx.x];
+    }
+}
+
+// p = z (PCG start)
+__global__ void __launch_bounds__(256) gf_copyf(int64_t n, const float *__restrict__ A, float *B) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) B[t] = A[t];
+}
+
 // R32 = R64 / nrm[col] ; X32 = 0
 __global__ void __launch_bounds__(256) gf_to32(int64_t ndof, int k, const double *__restrict__ R64,
                                                const double *__restrict__ nrm, float *R32, float *X32) {
@@ -474,6 +676,19 @@ struct plx_gf_s {
     // captured 10-iteration inner PCG chunk per block width
     hipGraphExec_t graph[65] = {};
     std::vector<double> probe_us;  // per-RHS per-iteration time of each probed width
+    // smoothed-aggregation multigrid preconditioner (PLX_GF_PC_AMG)
+    int pc = 0;
+    struct Level {
+        int n = 0, b = 0, nc = 0;
+        int32_t *Aptr = nullptr, *Acol = nullptr, *Pptr = nullptr, *Pcol = nullptr, *Rptr = nullptr, *Rcol = nullptr;
+        float *Aval = nullptr, *Pval = nullptr, *Rval = nullptr, *Dinv = nullptr;
+        float lmax = 0.f;
+        float *r = nullptr, *d = nullptr, *t = nullptr, *bv = nullptr, *xv = nullptr;  // workspace, kmax columns
+    };
+    std::vector<Level> lv;
+    int coarse_m = 0;
+    float *coarse_inv = nullptr, *cb = nullptr, *cx = nullptr;
+    int amg_kmax = 0;
 };
 
 namespace plx {
@@ -526,7 +741,21 @@ static void gf_pcg_chunk_k(plx_gf g) {
     }
 }
 
+template <int K>
+static void gf_pcg_chunk_amg(plx_gf g);
+template <int K>
+static void gf_pcg_start_amg(plx_gf g);
+
 static void gf_pcg_chunk(plx_gf g, int k) {
+    if (g->pc == PLX_GF_PC_AMG) {
+        switch (k) {
+        case 8: gf_pcg_chunk_amg<8>(g); break;
+        case 16: gf_pcg_chunk_amg<16>(g); break;
+        case 32: gf_pcg_chunk_amg<32>(g); break;
+        default: gf_pcg_chunk_amg<64>(g); break;
+        }
+        return;
+    }
     switch (k) {
     case 8: gf_pcg_chunk_k<8>(g); break;
     case 16: gf_pcg_chunk_k<16>(g); break;
@@ -537,6 +766,15 @@ static void gf_pcg_chunk(plx_gf g, int k) {
 
 // PCG start: z = Binv r, p = z, r.z and r.r (X32 already zero)
 static void gf_pcg_start(plx_gf g, int k) {
+    if (g->pc == PLX_GF_PC_AMG) {
+        switch (k) {
+        case 8: gf_pcg_start_amg<8>(g); break;
+        case 16: gf_pcg_start_amg<16>(g); break;
+        case 32: gf_pcg_start_amg<32>(g); break;
+        default: gf_pcg_start_amg<64>(g); break;
+        }
+        return;
+    }
 #define PLX_GF_START(K)                                                                                      \
     hipLaunchKernelGGL((gfk_update<K, true>), dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_Binv32, \
                        g->scal, g->X32, g->R32, g->P32, g->Q32, g->Z32, g->prz, g->prr)
@@ -548,6 +786,124 @@ static void gf_pcg_start(plx_gf g, int k) {
     }
 #undef PLX_GF_START
     hipLaunchKernelGGL(gf_finish_beta, dim3(k), dim3(256), 0, g->stream, k, g->prz, g->prr, g->scal, 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Multigrid V-cycle: z = M r on the k-column block (all launches on the handle's stream, no host
+// synchronisation: capturable).  Chebyshev degree 2 on [lmax/30, lmax] of D^-1 A pre and post.
+// ---------------------------------------------------------------------------------------------
+constexpr int kChebDeg = 2;
+
+static void cheb_coeffs(float lmax, float *inv_theta, float c1[kChebDeg], float c2[kChebDeg]) {
+    const double a = lmax / 30.0, b = lmax;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    *inv_theta = (float)(1.0 / theta);
+    for (int s = 0; s < kChebDeg; ++s) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c1[s] = (float)(rn * rho);
+        c2[s] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+}
+
+template <int K>
+static void amg_level_spmm(plx_gf g, int l, const float *X, float *Y) {
+    const plx_gf_s::Level &L = g->lv[l];
+    if (l == 0) {
+        if (K == 64)
+            hipLaunchKernelGGL(amg_spmm33_64, dim3(kGfRed), dim3(256), 0, g->stream, L.n, g->d_row_ptr, g->d_col,
+                               g->d_val32, X, Y);
+        else
+            hipLaunchKernelGGL((amg_spmm<3, 3, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, g->d_row_ptr,
+                               g->d_col, g->d_val32, X, Y, 0);
+    } else {
+        hipLaunchKernelGGL((amg_spmm<6, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Aptr, L.Acol, L.Aval,
+                           X, Y, 0);
+    }
+}
+
+template <int B, int K>
+static void amg_smooth(plx_gf g, int l, const float *bv, float *xv, bool zero_x) {
+    const plx_gf_s::Level &L = g->lv[l];
+    const int64_t nd = (int64_t)L.n * B;
+    float inv_theta, c1[kChebDeg], c2[kChebDeg];
+    cheb_coeffs(L.lmax, &inv_theta, c1, c2);
+    if (zero_x) {
+        hipLaunchKernelGGL(amg_copy, gf_grid(nd * K), dim3(256), 0, g->stream, nd * K, bv, L.r);
+    } else {
+        amg_level_spmm<K>(g, l, xv, L.t);
+        hipLaunchKernelGGL(amg_resid, gf_grid(nd * K), dim3(256), 0, g->stream, nd * K, bv, L.t, L.r);
+    }
+    hipLaunchKernelGGL((amg_cheb_init<B, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Dinv, L.r, L.d, xv,
+                       inv_theta, zero_x ? 1 : 0);
+    for (int s2 = 0; s2 < kChebDeg; ++s2) {
+        amg_level_spmm<K>(g, l, L.d, L.t);
+        hipLaunchKernelGGL((amg_cheb_step<B, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Dinv, L.t, L.r, L.d, xv,
+                           c1[s2], c2[s2]);
+    }
+}
+
+template <int K>
+static void amg_vcycle(plx_gf g, int l, const float *bv, float *xv) {
+    const int nl = (int)g->lv.size();
+    if (l == nl) {
+        hipLaunchKernelGGL(amg_dense<K>, dim3(std::max(1, std::min(kGfRed, (g->coarse_m * K + 255) / 256))), dim3(256),
+                           0, g->stream, g->coarse_m, g->coarse_inv, bv, xv);
+        return;
+    }
+    const plx_gf_s::Level &L = g->lv[l];
+    float *nb = l + 1 < nl ? g->lv[l + 1].bv : g->cb;
+    float *nx = l + 1 < nl ? g->lv[l + 1].xv : g->cx;
+    if (L.b == 3) amg_smooth<3, K>(g, l, bv, xv, true);
+    else amg_smooth<6, K>(g, l, bv, xv, true);
+    // restrict the residual (L.r = b - A x after the smoother)
+    if (L.b == 3)
+        hipLaunchKernelGGL((amg_spmm<6, 3, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.nc, L.Rptr, L.Rcol, L.Rval, L.r,
+                           nb, 0);
+    else
+        hipLaunchKernelGGL((amg_spmm<6, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.nc, L.Rptr, L.Rcol, L.Rval, L.r,
+                           nb, 0);
+    amg_vcycle<K>(g, l + 1, nb, nx);
+    if (L.b == 3)
+        hipLaunchKernelGGL((amg_spmm<3, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Pptr, L.Pcol, L.Pval, nx,
+                           xv, 1);
+    else
+        hipLaunchKernelGGL((amg_spmm<6, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Pptr, L.Pcol, L.Pval, nx,
+                           xv, 1);
+    if (L.b == 3) amg_smooth<3, K>(g, l, bv, xv, false);
+    else amg_smooth<6, K>(g, l, bv, xv, false);
+}
+
+// Ten iterations of the multigrid-preconditioned FP32 block PCG.
+template <int K>
+static void gf_pcg_chunk_amg(plx_gf g) {
+    for (int it = 0; it < 10; ++it) {
+        if (K == 64)
+            hipLaunchKernelGGL(gfk_spmm_dot64, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
+                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
+        else
+            hipLaunchKernelGGL(gfk_spmm_dot<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
+                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
+        hipLaunchKernelGGL(gf_finish_alpha, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->scal);
+        hipLaunchKernelGGL(gfk_update_xr<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, g->scal, g->X32, g->R32,
+                           g->P32, g->Q32, g->prr);
+        amg_vcycle<K>(g, 0, g->R32, g->Z32);
+        hipLaunchKernelGGL(gfk_dot2<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, g->Z32, g->R32, g->prz,
+                           (double *)nullptr);
+        hipLaunchKernelGGL(gf_finish_beta, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->prr, g->scal, 0);
+        hipLaunchKernelGGL(gfk_pupdate<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof * K, g->scal, g->Z32,
+                           g->P32);
+    }
+}
+
+// start: z = M r, p = z, r.z and r.r
+template <int K>
+static void gf_pcg_start_amg(plx_gf g) {
+    amg_vcycle<K>(g, 0, g->R32, g->Z32);
+    hipLaunchKernelGGL(gf_copyf, gf_grid(g->ndof * K), dim3(256), 0, g->stream, g->ndof * K, g->Z32, g->P32);
+    hipLaunchKernelGGL(gfk_dot2<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, g->Z32, g->R32, g->prz, g->prr);
+    hipLaunchKernelGGL(gf_finish_beta, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->prr, g->scal, 1);
 }
 
 static int gf_run_chunk(plx_gf g, int k) {
@@ -640,6 +996,89 @@ static int gf_block_solve(plx_gf g, int k, double rtol, int maxit, plx_gf_stats 
     for (int r = 0; r < k; ++r)
         if (b0[r] > 0.0) worst = std::max(worst, rn[r] / b0[r]);
     st->max_relative_residual = std::max(st->max_relative_residual, worst);
+    return 0;
+}
+
+static void gf_amg_free(plx_gf g) {
+    for (auto &L : g->lv)
+        for (void *p : {(void *)L.Aptr, (void *)L.Acol, (void *)L.Aval, (void *)L.Pptr, (void *)L.Pcol, (void *)L.Pval,
+                        (void *)L.Rptr, (void *)L.Rcol, (void *)L.Rval, (void *)L.Dinv, (void *)L.r, (void *)L.d,
+                        (void *)L.t, (void *)L.bv, (void *)L.xv})
+            if (p) (void)hipFree(p);
+    g->lv.clear();
+    for (void *p : {(void *)g->coarse_inv, (void *)g->cb, (void *)g->cx})
+        if (p) (void)hipFree(p);
+    g->coarse_inv = g->cb = g->cx = nullptr;
+    g->coarse_m = 0;
+}
+
+// Build the multigrid hierarchy of the merged FP64 operator on the host and upload it in FP32
+// with workspace for 64 columns.
+static int gf_amg_setup(plx_gf g) {
+    plx_dm dm = g->dm;
+    const int nn = g->nrows;
+    std::vector<int32_t> ptr(nn + 1), col(g->nnzb);
+    std::vector<double> val((size_t)g->nnzb * 9);
+    PLX_HIP(dm, hipMemcpy(ptr.data(), g->d_row_ptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PLX_HIP(dm, hipMemcpy(col.data(), g->d_col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PLX_HIP(dm, hipMemcpy(val.data(), g->d_val64, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // fixed dofs: identity rows of the merged operator (constrained dofs, positive split nodes)
+    std::vector<uint8_t> fixed(3 * (size_t)nn, 0);
+    for (int i = 0; i < nn; ++i)
+        for (int c = 0; c < 3; ++c) {
+            bool ident = true;
+            for (int e = ptr[i]; e < ptr[i + 1] && ident; ++e)
+                for (int d = 0; d < 3; ++d) {
+                    const double v = val[(size_t)e * 9 + c * 3 + d];
+                    if (v != ((col[e] == i && d == c) ? 1.0 : 0.0)) {
+                        ident = false;
+                        break;
+                    }
+                }
+            fixed[3 * (size_t)i + c] = ident;
+        }
+    AmgHierarchy h;
+    std::string msg;
+    if (amg_setup(nn, ptr.data(), col.data(), val.data(), dm->coords.data(), fixed.data(), 768, 8, &h, &msg))
+        return set_error(dm, 1, msg.c_str());
+    gf_amg_free(g);
+    auto upf = [&](float **d, const std::vector<double> &v) -> int {
+        std::vector<float> f(v.begin(), v.end());
+        PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, f.size()) * sizeof(float)));
+        PLX_HIP(dm, hipMemcpy(*d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        return 0;
+    };
+    auto upi = [&](int32_t **d, const std::vector<int32_t> &v) -> int {
+        PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, v.size()) * sizeof(int32_t)));
+        PLX_HIP(dm, hipMemcpy(*d, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        return 0;
+    };
+    auto ws = [&](float **d, size_t n) -> int {
+        PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, n) * 64 * sizeof(float)));
+        return 0;
+    };
+    g->lv.resize(h.levels.size());
+    int err = 0;
+    for (size_t l = 0; l < h.levels.size(); ++l) {
+        const AmgLevel &H = h.levels[l];
+        plx_gf_s::Level &L = g->lv[l];
+        L.n = H.n;
+        L.b = H.b;
+        L.nc = H.nc;
+        L.lmax = (float)H.lmax;
+        if (l > 0 && ((err = upi(&L.Aptr, H.Aptr)) || (err = upi(&L.Acol, H.Acol)) || (err = upf(&L.Aval, H.Aval))))
+            return err;
+        if ((err = upi(&L.Pptr, H.Pptr)) || (err = upi(&L.Pcol, H.Pcol)) || (err = upf(&L.Pval, H.Pval)) ||
+            (err = upi(&L.Rptr, H.Rptr)) || (err = upi(&L.Rcol, H.Rcol)) || (err = upf(&L.Rval, H.Rval)) ||
+            (err = upf(&L.Dinv, H.Dinv)))
+            return err;
+        const size_t nd = (size_t)H.n * H.b;
+        if ((err = ws(&L.r, nd)) || (err = ws(&L.d, nd)) || (err = ws(&L.t, nd))) return err;
+        if (l > 0 && ((err = ws(&L.bv, nd)) || (err = ws(&L.xv, nd)))) return err;
+    }
+    g->coarse_m = h.coarse_n;
+    if ((err = upf(&g->coarse_inv, h.coarse_inv)) || (err = ws(&g->cb, h.coarse_n)) || (err = ws(&g->cx, h.coarse_n)))
+        return err;
     return 0;
 }
 
@@ -791,6 +1230,7 @@ int plx_gf_destroy(plx_gf g) {
                     g->jump,      g->scal,  g->prz,     g->prr,     g->nrm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    gf_amg_free(g);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return 0;
@@ -805,6 +1245,28 @@ int plx_gf_set_options(plx_gf g, int block, double inner_rtol, int inner_maxit, 
     g->inner_rtol = inner_rtol;
     g->inner_maxit = inner_maxit;
     g->use_graph = use_graph != 0;
+    return 0;
+}
+
+int plx_gf_set_preconditioner(plx_gf g, int kind) {
+    if (!g) return 1;
+    if (kind != PLX_GF_PC_JACOBI && kind != PLX_GF_PC_AMG)
+        return set_error(g->dm, 1, "preconditioner must be PLX_GF_PC_JACOBI or PLX_GF_PC_AMG");
+    if (kind == g->pc) return 0;
+    if (kind == PLX_GF_PC_AMG && g->lv.empty()) {
+        const int err = gf_amg_setup(g);
+        if (err) {
+            gf_amg_free(g);
+            return err;
+        }
+    }
+    for (int i = 0; i <= 64; ++i)  // captured chunks belong to the previous preconditioner
+        if (g->graph[i]) {
+            (void)hipGraphExecDestroy(g->graph[i]);
+            g->graph[i] = nullptr;
+        }
+    g->pc = kind;
+    g->probe_us.assign(4, 0.0);
     return 0;
 }
 

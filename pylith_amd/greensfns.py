@@ -3,7 +3,8 @@
 Mirrors pylith.problems.GreensFns (libsrc/pylith/problems/GreensFns.cc:236-331): the operator
 is assembled once, then every impulse of the fault's impulse basis (FaultCohesiveImpulses,
 faults/FaultCohesiveImpulses.cc:263-300) is solved.  Impulses are solved in blocks of k columns
-(FP32 block PCG inside FP64 iterative refinement), k autotuned over {8, 16, 32, 64}.
+(FP32 block PCG inside FP64 iterative refinement), k autotuned over {8, 16, 32, 64}; the PCG is
+preconditioned by point-block Jacobi or by a smoothed-aggregation multigrid V-cycle.
 """
 import ctypes
 
@@ -29,6 +30,12 @@ class GreensFunctions:
 
     def set_options(self, block=0, inner_rtol=1e-4, inner_maxit=4000, use_graph=True):
         self._check(self.lib.plx_gf_set_options(self.h, block, inner_rtol, inner_maxit, int(use_graph)))
+
+    def set_preconditioner(self, kind):
+        """'jacobi' (point-block Jacobi, the default) or 'amg' (smoothed-aggregation multigrid,
+        the GAMG role of materials/Elasticity.cc:307-318); builds the hierarchy on first use."""
+        code = {"jacobi": L.GF_PC_JACOBI, "amg": L.GF_PC_AMG}.get(kind, kind)
+        self._check(self.lib.plx_gf_set_preconditioner(self.h, int(code)))
 
     def autotune(self):
         best = ctypes.c_int()
