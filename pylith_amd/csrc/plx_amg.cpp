@@ -4,14 +4,35 @@
 #include <algorithm>
 #include <cmath>
 #include <random>
+#include <thread>
 
 namespace plx {
 namespace {
 
+int num_threads() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hc));
+}
+
+// f(lo, hi, tid) over T contiguous chunks of [0, n) on T threads
+template <typename F>
+void parallel_chunks(int n, int T, F f) {
+    if (T <= 1 || n < 1024) {
+        f(0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) {
+        const int lo = (int)((int64_t)n * t / T), hi = (int)((int64_t)n * (t + 1) / T);
+        th.emplace_back([=, &f] { f(lo, hi, t); });
+    }
+    for (auto &x : th) x.join();
+}
+
 // Inverse of a b x b block (b <= 6) by Gauss-Jordan with partial pivoting; a singular block
 // (zero pivot) yields the identity (its rows are decoupled dummy dofs).
 void block_inverse(const double *A, int b, double *Ai) {
-    double M[6][12];
+    alignas(64) double M[6][12];
     double scale = 0.0;
     for (int i = 0; i < b; ++i)
         for (int j = 0; j < b; ++j) {
@@ -57,17 +78,47 @@ struct Csr6 {  // block rows of b x 6 blocks, sorted columns
     std::vector<double> val;
 };
 
-// Sparse product rows: out_i = sum_j A_ij * M_j for M with b x 6 blocks; `mark`/`list` scratch.
-void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const double *Aval, const Csr6 &M,
-                int ncols, Csr6 *out) {
-    std::vector<int> mark(ncols, -1);
-    std::vector<int> list;
-    std::vector<double> acc;
-    const int bb = b * 6;
+// Row-parallel construction of a block-CSR matrix: row(i, col, val, tid) appends row i's
+// blocks (bs values each); chunks are built per thread and concatenated in row order.
+template <typename RowFn>
+void build_rows(int n, int bs, Csr6 *out, RowFn row) {
+    const int T = num_threads();
+    std::vector<Csr6> part(T);
+    parallel_chunks(n, T, [&](int lo, int hi, int t) {
+        Csr6 &P = part[t];
+        P.ptr.assign(1, 0);
+        for (int i = lo; i < hi; ++i) {
+            row(i, P.col, P.val, t);
+            P.ptr.push_back((int32_t)P.col.size());
+        }
+    });
     out->ptr.assign(1, 0);
     out->col.clear();
     out->val.clear();
-    for (int i = 0; i < n; ++i) {
+    size_t nnz = 0;
+    for (auto &P : part) nnz += P.col.size();
+    out->ptr.reserve(n + 1);
+    out->col.reserve(nnz);
+    out->val.reserve(nnz * bs);
+    for (auto &P : part) {
+        if (P.ptr.empty()) continue;
+        const int32_t base = (int32_t)out->col.size();
+        for (size_t r = 1; r < P.ptr.size(); ++r) out->ptr.push_back(base + P.ptr[r]);
+        out->col.insert(out->col.end(), P.col.begin(), P.col.end());
+        out->val.insert(out->val.end(), P.val.begin(), P.val.end());
+    }
+}
+
+// Sparse product rows: out_i = sum_j A_ij * M_j for M with b x 6 blocks.
+void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const double *Aval, const Csr6 &M,
+                int ncols, Csr6 *out) {
+    const int T = num_threads();
+    std::vector<std::vector<int>> marks(T, std::vector<int>(ncols, -1)), lists(T);
+    std::vector<std::vector<double>> accs(T);
+    const int bb = b * 6;
+    build_rows(n, bb, out, [&](int i, std::vector<int32_t> &ocol, std::vector<double> &oval, int t) {
+        std::vector<int> &mark = marks[t], &list = lists[t];
+        std::vector<double> &acc = accs[t];
         list.clear();
         acc.clear();
         for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
@@ -84,15 +135,14 @@ void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const do
             }
         }
         std::vector<int> order(list.size());
-        for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+        for (size_t u = 0; u < order.size(); ++u) order[u] = (int)u;
         std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
-        for (int t : order) {
-            out->col.push_back(list[t]);
-            out->val.insert(out->val.end(), acc.begin() + (size_t)t * bb, acc.begin() + (size_t)(t + 1) * bb);
+        for (int u : order) {
+            ocol.push_back(list[u]);
+            oval.insert(oval.end(), acc.begin() + (size_t)u * bb, acc.begin() + (size_t)(u + 1) * bb);
         }
         for (int c : list) mark[c] = -1;
-        out->ptr.push_back((int32_t)out->col.size());
-    }
+    });
 }
 
 }  // namespace
@@ -107,6 +157,7 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
     std::vector<double> Aval(val0, val0 + (size_t)ptr0[n0] * 9);
     std::vector<uint8_t> fixed(fixed0, fixed0 + 3 * (size_t)n0);
     std::vector<double> B;  // near-null space [n*b][6], built per aggregate on level 0
+    const int T = num_threads();
     for (int lev = 0;; ++lev) {
         AmgLevel L;
         L.n = n;
@@ -114,15 +165,22 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         const int bb = b * b;
         // inverse diagonal blocks
         L.Dinv.assign((size_t)n * bb, 0.0);
-        for (int i = 0; i < n; ++i) {
-            int d = -1;
-            for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
-                if (Acol[e] == i) d = e;
-            if (d < 0) {
-                *err = "amg: missing diagonal block";
-                return 1;
+        std::vector<uint8_t> no_diag(T, 0);
+        parallel_chunks(n, T, [&](int lo, int hi, int t) {
+            for (int i = lo; i < hi; ++i) {
+                int d = -1;
+                for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+                    if (Acol[e] == i) d = e;
+                if (d < 0) {
+                    no_diag[t] = 1;
+                    return;
+                }
+                block_inverse(Aval.data() + (size_t)d * bb, b, L.Dinv.data() + (size_t)i * bb);
             }
-            block_inverse(Aval.data() + (size_t)d * bb, b, L.Dinv.data() + (size_t)i * bb);
+        });
+        if (std::find(no_diag.begin(), no_diag.end(), 1) != no_diag.end()) {
+            *err = "amg: missing diagonal block";
+            return 1;
         }
         // aggregation on the node graph; nodes with every dof fixed stay out
         std::vector<uint8_t> excl(n, 0);
@@ -195,7 +253,8 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         P0.ptr.assign(n + 1, 0);
         std::vector<double> Q((size_t)n * b * 6, 0.0);  // P0 block of each fine row
         std::vector<double> Bc((size_t)na * 6 * 6, 0.0);
-        for (int a = 0; a < na; ++a) {
+        parallel_chunks(na, T, [&](int alo, int ahi, int) {
+          for (int a = alo; a < ahi; ++a) {
             const auto &mem = members[a];
             const int rows = (int)mem.size() * b;
             std::vector<double> M((size_t)rows * 6);
@@ -233,7 +292,8 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                     for (int k = 0; k < 6; ++k) Q[((size_t)mem[t] * b + c) * 6 + k] = M[(t * b + c) * 6 + k];
             for (int k = 0; k < 6; ++k)
                 for (int j = 0; j < 6; ++j) Bc[((size_t)a * 6 + k) * 6 + j] = R[k][j];
-        }
+          }
+        });
         for (int i = 0; i < n; ++i) P0.ptr[i + 1] = P0.ptr[i] + (agg[i] >= 0 ? 1 : 0);
         P0.col.resize(P0.ptr[n]);
         P0.val.resize((size_t)P0.ptr[n] * b * 6);
@@ -246,32 +306,45 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         {
             std::mt19937_64 rng(7);
             std::normal_distribution<double> nd;
-            std::vector<double> v((size_t)n * b), w((size_t)n * b), t((size_t)n * b);
+            const size_t nd_ = (size_t)n * b;
+            std::vector<double> v(nd_), w(nd_);
             for (auto &x : v) x = nd(rng);
             double lam = 0.0;
-            for (int it = 0; it < 20; ++it) {
-                double vn = 0.0;
-                for (double x : v) vn += x * x;
-                vn = std::sqrt(vn);
-                for (auto &x : v) x /= vn;
-                std::fill(t.begin(), t.end(), 0.0);
-                for (int i = 0; i < n; ++i)
-                    for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
-                        const double *A = &Aval[(size_t)e * bb];
-                        const double *x = &v[(size_t)Acol[e] * b];
-                        for (int r = 0; r < b; ++r)
-                            for (int c = 0; c < b; ++c) t[(size_t)i * b + r] += A[r * b + c] * x[c];
+            std::vector<double> part(T);
+            double vn = 0.0;
+            for (double x : v) vn += x * x;
+            vn = std::sqrt(vn);
+            for (int it = 0; it < 12; ++it) {
+                std::fill(part.begin(), part.end(), 0.0);
+                parallel_chunks(n, T, [&](int lo, int hi, int t) {
+                    double acc = 0.0;
+                    for (int i = lo; i < hi; ++i) {
+                        double y[6] = {0, 0, 0, 0, 0, 0};
+                        for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
+                            const double *A = &Aval[(size_t)e * bb];
+                            const double *x = &v[(size_t)Acol[e] * b];
+                            for (int r = 0; r < b; ++r) {
+                                double sacc = 0.0;
+                                for (int c = 0; c < b; ++c) sacc += A[r * b + c] * x[c];
+                                y[r] += sacc;
+                            }
+                        }
+                        const double *Di = &L.Dinv[(size_t)i * bb];
+                        for (int r = 0; r < b; ++r) {
+                            double sacc = 0.0;
+                            for (int c = 0; c < b; ++c) sacc += Di[r * b + c] * y[c];
+                            w[(size_t)i * b + r] = sacc / vn;
+                            acc += sacc * sacc;
+                        }
                     }
+                    part[t] = acc;
+                });
                 double wn = 0.0;
-                for (int i = 0; i < n; ++i)
-                    for (int r = 0; r < b; ++r) {
-                        double s = 0.0;
-                        for (int c = 0; c < b; ++c) s += L.Dinv[(size_t)i * bb + r * b + c] * t[(size_t)i * b + c];
-                        w[(size_t)i * b + r] = s;
-                        wn += s * s;
-                    }
-                lam = std::sqrt(wn);
+                for (double x : part) wn += x;
+                wn = std::sqrt(wn) / vn;  // || D^-1 A v || / || v || with v scaled to unit norm
+                lam = wn;
                 v.swap(w);
+                vn = wn;
             }
             L.lmax = 1.1 * lam;
         }
@@ -279,21 +352,18 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         Csr6 AP0, P;
         spgemm_A_M(n, b, Aptr.data(), Acol.data(), Aval.data(), P0, na, &AP0);
         const double omega = 4.0 / (3.0 * L.lmax);
-        P.ptr.assign(1, 0);
-        for (int i = 0; i < n; ++i) {
+        // P0's own aggregate is always among AP0's columns when row i is free (A_ii P0_i); a
+        // fixed row keeps P = 0 there (its A row is the identity and P0 row is zero)
+        build_rows(n, b * 6, &P, [&](int i, std::vector<int32_t> &ocol, std::vector<double> &oval, int) {
             for (int f = AP0.ptr[i]; f < AP0.ptr[i + 1]; ++f) {
                 const int c = AP0.col[f];
-                std::vector<double> blk(b * 6, 0.0);
-                if (agg[i] == c)
-                    std::copy(&P0.val[(size_t)P0.ptr[i] * b * 6], &P0.val[(size_t)(P0.ptr[i] + 1) * b * 6], blk.begin());
-                gemm_b6(b, -omega, &L.Dinv[(size_t)i * bb], &AP0.val[(size_t)f * b * 6], blk.data());
-                P.col.push_back(c);
-                P.val.insert(P.val.end(), blk.begin(), blk.end());
+                double blk[36] = {};
+                if (agg[i] == c) std::copy(&P0.val[(size_t)P0.ptr[i] * b * 6], &P0.val[(size_t)(P0.ptr[i] + 1) * b * 6], blk);
+                gemm_b6(b, -omega, &L.Dinv[(size_t)i * bb], &AP0.val[(size_t)f * b * 6], blk);
+                ocol.push_back(c);
+                oval.insert(oval.end(), blk, blk + b * 6);
             }
-            // P0's own aggregate is always among AP0's columns when row i is free (A_ii P0_i); a
-            // fixed row keeps P = 0 there (its A row is the identity and P0 row is zero)
-            P.ptr.push_back((int32_t)P.col.size());
-        }
+        });
         // Galerkin coarse operator Ac = P^T (A P)
         Csr6 AP;
         spgemm_A_M(n, b, Aptr.data(), Acol.data(), Aval.data(), P, na, &AP);
@@ -309,17 +379,18 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                     Rsrc[fill[P.col[f]]++] = f;
                 }
         }
-        std::vector<int32_t> Cptr(1, 0), Ccol;
-        std::vector<double> Cval;
+        Csr6 C;
         {
-            std::vector<int> mark(na, -1), list;
-            std::vector<double> acc;
-            for (int a = 0; a < na; ++a) {
+            std::vector<std::vector<int>> marks(T, std::vector<int>(na, -1)), lists(T);
+            std::vector<std::vector<double>> accs(T);
+            build_rows(na, 36, &C, [&](int a, std::vector<int32_t> &ocol, std::vector<double> &oval, int t) {
+                std::vector<int> &mark = marks[t], &list = lists[t];
+                std::vector<double> &acc = accs[t];
                 list.clear();
                 acc.clear();
-                for (int t = Rptr[a]; t < Rptr[a + 1]; ++t) {
-                    const int i = Rrow[t];
-                    const double *Pia = &P.val[(size_t)Rsrc[t] * b * 6];  // b x 6
+                for (int u = Rptr[a]; u < Rptr[a + 1]; ++u) {
+                    const int i = Rrow[u];
+                    const double *Pia = &P.val[(size_t)Rsrc[u] * b * 6];  // b x 6
                     for (int f = AP.ptr[i]; f < AP.ptr[i + 1]; ++f) {
                         const int c = AP.col[f];
                         if (mark[c] < 0) {
@@ -333,49 +404,57 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                             for (int k = 0; k < b; ++k) {
                                 const double p = Pia[k * 6 + r];
                                 if (p == 0.0) continue;
-                                for (int s = 0; s < 6; ++s) o[r * 6 + s] += p * APic[k * 6 + s];
+                                for (int q = 0; q < 6; ++q) o[r * 6 + q] += p * APic[k * 6 + q];
                             }
                     }
                 }
                 std::vector<int> order(list.size());
-                for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+                for (size_t u = 0; u < order.size(); ++u) order[u] = (int)u;
                 std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
                 bool has_diag = false;
-                for (int t : order) {
-                    double *o = &acc[(size_t)t * 36];
-                    if (list[t] == a) {
+                for (int u : order) {
+                    double *o = &acc[(size_t)u * 36];
+                    if (list[u] == a) {
                         has_diag = true;
                         for (int r = 0; r < 6; ++r)  // dummy (rank-deficient) coarse dofs: identity
                             if (o[r * 6 + r] == 0.0) o[r * 6 + r] = 1.0;
                     }
-                    Ccol.push_back(list[t]);
-                    Cval.insert(Cval.end(), o, o + 36);
+                    ocol.push_back(list[u]);
+                    oval.insert(oval.end(), o, o + 36);
                 }
                 if (!has_diag) {  // an aggregate disconnected from everything: identity block
-                    Ccol.push_back(a);
-                    for (int r = 0; r < 36; ++r) Cval.push_back(r % 7 == 0 ? 1.0 : 0.0);
+                    // keep the columns sorted: insert the diagonal at its place
+                    size_t pos = ocol.size() - order.size();
+                    while (pos < ocol.size() && ocol[pos] < a) ++pos;
+                    ocol.insert(ocol.begin() + pos, a);
+                    double I6[36];
+                    for (int r = 0; r < 36; ++r) I6[r] = r % 7 == 0 ? 1.0 : 0.0;
+                    oval.insert(oval.begin() + pos * 36, I6, I6 + 36);
                 }
                 for (int c : list) mark[c] = -1;
-                Cptr.push_back((int32_t)Ccol.size());
-            }
+            });
         }
+        std::vector<int32_t> &Cptr = C.ptr, &Ccol = C.col;
+        std::vector<double> &Cval = C.val;
         // store this level's transfer operators
         L.nc = na;
-        L.Pptr = P.ptr;
-        L.Pcol = P.col;
-        L.Pval = P.val;
-        L.Rptr = Rptr;
-        L.Rcol = Rrow;
         L.Rval.resize((size_t)P.col.size() * 6 * b);
-        for (int t = 0; t < (int)Rrow.size(); ++t) {
-            const double *Pia = &P.val[(size_t)Rsrc[t] * b * 6];
-            for (int r = 0; r < 6; ++r)
-                for (int k = 0; k < b; ++k) L.Rval[(size_t)t * 6 * b + r * b + k] = Pia[k * 6 + r];
-        }
-        if (lev > 0) {
-            L.Aptr = Aptr;
-            L.Acol = Acol;
-            L.Aval = Aval;
+        parallel_chunks((int)Rrow.size(), T, [&](int lo, int hi, int) {
+            for (int t = lo; t < hi; ++t) {
+                const double *Pia = &P.val[(size_t)Rsrc[t] * b * 6];
+                for (int r = 0; r < 6; ++r)
+                    for (int k = 0; k < b; ++k) L.Rval[(size_t)t * 6 * b + r * b + k] = Pia[k * 6 + r];
+            }
+        });
+        L.Pptr = std::move(P.ptr);
+        L.Pcol = std::move(P.col);
+        L.Pval = std::move(P.val);
+        L.Rptr = std::move(Rptr);
+        L.Rcol = std::move(Rrow);
+        if (lev > 0) {  // the next level's operator replaces these below
+            L.Aptr = std::move(Aptr);
+            L.Acol = std::move(Acol);
+            L.Aval = std::move(Aval);
         }
         h->levels.push_back(std::move(L));
         // next level

@@ -391,143 +391,105 @@ __global__ void __launch_bounds__(256) gfk_pupdate(int64_t n, const double *__re
         P[t] = Z[t] + b * P[t];
 }
 
-// ---- multigrid preconditioner kernels ([dof][K] layout, thread = (block row, column)) ----
-// Y (+)= A X for BR x BC blocks
-template <int BR, int BC, int K>
-__global__ void __launch_bounds__(256) amg_spmm(int nrows, const int32_t *__restrict__ ptr,
-                                                const int32_t *__restrict__ col, const float *__restrict__ val,
-                                                const float *__restrict__ X, float *Y, int accumulate) {
-    const int64_t n = (int64_t)nrows * K;
-    const int r = (int)(threadIdx.x % K);
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-        const int64_t i = t / K;
+// ---- multigrid preconditioner kernels ([dof][K] layout) ----
+// Block-sparse products of the V-cycle, one wave per (64 / K) block rows, lane = (row, column);
+// with K = 64 the row is wave-uniform and the matrix words are scalar loads.  Rows follow the
+// per-XCD slab schedule of the fine-level product.
+//   kBsrPlain: Y = T X (or Y += T X with accumulate)          transfers (P, R = P^T)
+//   kBsrResid: Y = B - A X                                    pre-smoother residual
+//   kBsrPost:  Y = X + s Dinv (B - A X)                       post-smoother (damped Jacobi)
+enum { kBsrPlain = 0, kBsrResid = 1, kBsrPost = 2 };
+template <int BR, int BC, int K, int MODE>
+__global__ void __launch_bounds__(256) amg_bsr(int nrows, const int32_t *__restrict__ ptr,
+                                               const int32_t *__restrict__ col, const float *__restrict__ val,
+                                               const float *__restrict__ X, const float *__restrict__ Bv,
+                                               const float *__restrict__ Dinv, float *Y, float s, int accumulate) {
+    constexpr int RPW = 64 / K;             // block rows per wave
+    constexpr int U = BC == 3 ? 9 : 4;      // entries whose operands are loaded together
+    const int lane = threadIdx.x & 63;
+    const int c = lane % K;
+    int lo, hi, jb, nbx;
+    gf_xcd_slab(nrows, lo, hi, jb, nbx);
+    for (int row0 = lo + (jb * 4 + (int)(threadIdx.x >> 6)) * RPW; row0 < hi; row0 += nbx * 4 * RPW) {
+        int row = row0 + lane / K;
+        if (K == 64) row = __builtin_amdgcn_readfirstlane(row);
+        if (row >= hi) continue;
+        const int e0 = ptr[row], e1 = ptr[row + 1];
         float y[BR];
 #pragma unroll
         for (int a = 0; a < BR; ++a) y[a] = 0.f;
-        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
-            const int64_t j = col[e];
-            const float *A = val + (size_t)BR * BC * e;
-            float x[BC];
-#pragma unroll
-            for (int c = 0; c < BC; ++c) x[c] = X[(BC * j + c) * K + r];
-#pragma unroll
-            for (int a = 0; a < BR; ++a)
-#pragma unroll
-                for (int c = 0; c < BC; ++c) y[a] += A[a * BC + c] * x[c];
-        }
-#pragma unroll
-        for (int a = 0; a < BR; ++a) {
-            const size_t o = (BR * (size_t)i + a) * K + r;
-            Y[o] = accumulate ? Y[o] + y[a] : y[a];
-        }
-    }
-}
-
-// K = 64, 3 x 3 blocks: the wave-per-row product of the fine operator (no dot product)
-__global__ void __launch_bounds__(256) amg_spmm33_64(int nrows, const int32_t *__restrict__ row_ptr,
-                                                     const int32_t *__restrict__ col, const float *__restrict__ val,
-                                                     const float *__restrict__ X, float *Y) {
-    const int lane = threadIdx.x & 63;
-    int lo, hi, jb, nbx;
-    gf_xcd_slab(nrows, lo, hi, jb, nbx);
-    for (int row0 = lo + jb * 4 + (int)(threadIdx.x >> 6); row0 < hi; row0 += nbx * 4) {
-        const int row = __builtin_amdgcn_readfirstlane(row0);
-        const int e0 = row_ptr[row], e1 = row_ptr[row + 1];
-        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
         int e = e0;
-        for (; e + 9 <= e1; e += 9) {
-            float x[9][3];
+        for (; e + U <= e1; e += U) {
+            float x[U][BC];
 #pragma unroll
-            for (int m = 0; m < 9; ++m) {
+            for (int m = 0; m < U; ++m) {
                 const size_t j = (size_t)col[e + m];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) x[m][c] = X[(3 * j + c) * 64 + lane];
+                for (int q = 0; q < BC; ++q) x[m][q] = X[(BC * j + q) * K + c];
             }
 #pragma unroll
-            for (int m = 0; m < 9; ++m) {
-                const float *A = val + 9 * (size_t)(e + m);
-                y0 += A[0] * x[m][0] + A[1] * x[m][1] + A[2] * x[m][2];
-                y1 += A[3] * x[m][0] + A[4] * x[m][1] + A[5] * x[m][2];
-                y2 += A[6] * x[m][0] + A[7] * x[m][1] + A[8] * x[m][2];
+            for (int m = 0; m < U; ++m) {
+                const float *A = val + (size_t)BR * BC * (e + m);
+#pragma unroll
+                for (int a = 0; a < BR; ++a)
+#pragma unroll
+                    for (int q = 0; q < BC; ++q) y[a] += A[a * BC + q] * x[m][q];
             }
         }
         for (; e < e1; ++e) {
             const size_t j = (size_t)col[e];
-            const float *A = val + 9 * (size_t)e;
-            const float x0 = X[(3 * j) * 64 + lane], x1 = X[(3 * j + 1) * 64 + lane], x2 = X[(3 * j + 2) * 64 + lane];
-            y0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
-            y1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
-            y2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
+            const float *A = val + (size_t)BR * BC * e;
+            float x[BC];
+#pragma unroll
+            for (int q = 0; q < BC; ++q) x[q] = X[(BC * j + q) * K + c];
+#pragma unroll
+            for (int a = 0; a < BR; ++a)
+#pragma unroll
+                for (int q = 0; q < BC; ++q) y[a] += A[a * BC + q] * x[q];
         }
-        const size_t o = (3 * (size_t)row) * 64 + lane;
-        Y[o] = y0;
-        Y[o + 64] = y1;
-        Y[o + 128] = y2;
+        const size_t o = (size_t)BR * row * K + c;
+        if (MODE == kBsrPlain) {
+#pragma unroll
+            for (int a = 0; a < BR; ++a) Y[o + a * K] = accumulate ? Y[o + a * K] + y[a] : y[a];
+        } else if (MODE == kBsrResid) {
+#pragma unroll
+            for (int a = 0; a < BR; ++a) Y[o + a * K] = Bv[o + a * K] - y[a];
+        } else {
+            float rv[BR];
+#pragma unroll
+            for (int a = 0; a < BR; ++a) rv[a] = Bv[o + a * K] - y[a];
+            const float *Di = Dinv + (size_t)BR * BR * row;
+#pragma unroll
+            for (int a = 0; a < BR; ++a) {
+                float z = 0.f;
+#pragma unroll
+                for (int q = 0; q < BR; ++q) z += Di[a * BR + q] * rv[q];
+                Y[o + a * K] = X[o + a * K] + s * z;
+            }
+        }
     }
 }
 
-// Chebyshev start: d = Dinv r / theta (and x = 0 when asked)
+// pre-smoother from x = 0: x = s Dinv b (thread = (block row, column))
 template <int B, int K>
-__global__ void __launch_bounds__(256) amg_cheb_init(int n, const float *__restrict__ Dinv,
-                                                     const float *__restrict__ R, float *D, float *X, float inv_theta,
-                                                     int zero_x) {
+__global__ void __launch_bounds__(256) amg_jacobi0(int n, const float *__restrict__ Dinv, const float *__restrict__ Bv,
+                                                   float *X, float s) {
     const int64_t nt = (int64_t)n * K;
     const int r = (int)(threadIdx.x % K);
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nt; t += (int64_t)gridDim.x * 256) {
         const int64_t i = t / K;
         const float *Di = Dinv + (size_t)B * B * i;
-        float rv[B];
+        float bv[B];
 #pragma unroll
-        for (int c = 0; c < B; ++c) rv[c] = R[(B * i + c) * K + r];
-#pragma unroll
-        for (int a = 0; a < B; ++a) {
-            float z = 0.f;
-#pragma unroll
-            for (int c = 0; c < B; ++c) z += Di[a * B + c] * rv[c];
-            const size_t o = (B * i + a) * K + r;
-            D[o] = inv_theta * z;
-            if (zero_x) X[o] = 0.f;
-        }
-    }
-}
-
-// Chebyshev step: x += d; r -= t (t = A d); d = c1 d + c2 Dinv r
-template <int B, int K>
-__global__ void __launch_bounds__(256) amg_cheb_step(int n, const float *__restrict__ Dinv,
-                                                     const float *__restrict__ T, float *R, float *D, float *X,
-                                                     float c1, float c2) {
-    const int64_t nt = (int64_t)n * K;
-    const int r = (int)(threadIdx.x % K);
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nt; t += (int64_t)gridDim.x * 256) {
-        const int64_t i = t / K;
-        const float *Di = Dinv + (size_t)B * B * i;
-        float rv[B], dv[B];
-#pragma unroll
-        for (int c = 0; c < B; ++c) {
-            const size_t o = (B * i + c) * K + r;
-            dv[c] = D[o];
-            X[o] += dv[c];
-            rv[c] = R[o] - T[o];
-            R[o] = rv[c];
-        }
+        for (int q = 0; q < B; ++q) bv[q] = Bv[(B * i + q) * K + r];
 #pragma unroll
         for (int a = 0; a < B; ++a) {
             float z = 0.f;
 #pragma unroll
-            for (int c = 0; c < B; ++c) z += Di[a * B + c] * rv[c];
-            D[(B * i + a) * K + r] = c1 * dv[a] + c2 * z;
+            for (int q = 0; q < B; ++q) z += Di[a * B + q] * bv[q];
+            X[(B * i + a) * K + r] = s * z;
         }
     }
-}
-
-// r = b - t
-__global__ void __launch_bounds__(256) amg_resid(int64_t n, const float *__restrict__ Bv, const float *__restrict__ T,
-                                                 float *R) {
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) R[t] = Bv[t] - T[t];
-}
-
-__global__ void __launch_bounds__(256) amg_copy(int64_t n, const float *__restrict__ A, float *B) {
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) B[t] = A[t];
 }
 
 // coarsest level: x = Ainv b (dense, m x m)
@@ -683,7 +645,7 @@ struct plx_gf_s {
         int32_t *Aptr = nullptr, *Acol = nullptr, *Pptr = nullptr, *Pcol = nullptr, *Rptr = nullptr, *Rcol = nullptr;
         float *Aval = nullptr, *Pval = nullptr, *Rval = nullptr, *Dinv = nullptr;
         float lmax = 0.f;
-        float *r = nullptr, *d = nullptr, *t = nullptr, *bv = nullptr, *xv = nullptr;  // workspace, kmax columns
+        float *r = nullptr, *d = nullptr, *bv = nullptr, *xv = nullptr;  // workspace, 64 columns
     };
     std::vector<Level> lv;
     int coarse_m = 0;
@@ -790,95 +752,64 @@ static void gf_pcg_start(plx_gf g, int k) {
 
 // ---------------------------------------------------------------------------------------------
 // Multigrid V-cycle: z = M r on the k-column block (all launches on the handle's stream, no host
-// synchronisation: capturable).  Chebyshev degree 2 on [lmax/30, lmax] of D^-1 A pre and post.
+// synchronisation: capturable).  Smoother: Chebyshev of degree 1 on [lmax/30, lmax] of D^-1 A,
+// i.e. block-Jacobi damped by s = 2 / (lmax + lmax/30), once before and once after the coarse
+// correction (a symmetric V-cycle, as PCG needs):
+//   x = s D^-1 b;  r = b - A x;  x += P V(R r);  z = x + s D^-1 (b - A x)
 // ---------------------------------------------------------------------------------------------
-constexpr int kChebDeg = 2;
-
-static void cheb_coeffs(float lmax, float *inv_theta, float c1[kChebDeg], float c2[kChebDeg]) {
-    const double a = lmax / 30.0, b = lmax;
-    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
-    double rho = 1.0 / sigma;
-    *inv_theta = (float)(1.0 / theta);
-    for (int s = 0; s < kChebDeg; ++s) {
-        const double rn = 1.0 / (2.0 * sigma - rho);
-        c1[s] = (float)(rn * rho);
-        c2[s] = (float)(2.0 * rn / delta);
-        rho = rn;
-    }
+static dim3 amg_grid(int nrows, int rows_per_wave) {
+    const int64_t waves = ((int64_t)nrows + rows_per_wave - 1) / rows_per_wave;
+    int64_t blocks = (waves + 3) / 4;
+    blocks = std::min<int64_t>(kGfRed, std::max<int64_t>(8, (blocks + 7) / 8 * 8));
+    return dim3((unsigned)blocks);
 }
 
-template <int K>
-static void amg_level_spmm(plx_gf g, int l, const float *X, float *Y) {
-    const plx_gf_s::Level &L = g->lv[l];
-    if (l == 0) {
-        if (K == 64)
-            hipLaunchKernelGGL(amg_spmm33_64, dim3(kGfRed), dim3(256), 0, g->stream, L.n, g->d_row_ptr, g->d_col,
-                               g->d_val32, X, Y);
-        else
-            hipLaunchKernelGGL((amg_spmm<3, 3, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, g->d_row_ptr,
-                               g->d_col, g->d_val32, X, Y, 0);
-    } else {
-        hipLaunchKernelGGL((amg_spmm<6, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Aptr, L.Acol, L.Aval,
-                           X, Y, 0);
-    }
+template <int BR, int BC, int K, int MODE>
+static void amg_bsr_launch(plx_gf g, int nrows, const int32_t *ptr, const int32_t *col, const float *val,
+                           const float *X, const float *Bv, const float *Dinv, float *Y, float s, int accumulate) {
+    hipLaunchKernelGGL((amg_bsr<BR, BC, K, MODE>), amg_grid(nrows, 64 / K), dim3(256), 0, g->stream, nrows, ptr, col,
+                       val, X, Bv, Dinv, Y, s, accumulate);
 }
 
 template <int B, int K>
-static void amg_smooth(plx_gf g, int l, const float *bv, float *xv, bool zero_x) {
-    const plx_gf_s::Level &L = g->lv[l];
-    const int64_t nd = (int64_t)L.n * B;
-    float inv_theta, c1[kChebDeg], c2[kChebDeg];
-    cheb_coeffs(L.lmax, &inv_theta, c1, c2);
-    if (zero_x) {
-        hipLaunchKernelGGL(amg_copy, gf_grid(nd * K), dim3(256), 0, g->stream, nd * K, bv, L.r);
-    } else {
-        amg_level_spmm<K>(g, l, xv, L.t);
-        hipLaunchKernelGGL(amg_resid, gf_grid(nd * K), dim3(256), 0, g->stream, nd * K, bv, L.t, L.r);
-    }
-    hipLaunchKernelGGL((amg_cheb_init<B, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Dinv, L.r, L.d, xv,
-                       inv_theta, zero_x ? 1 : 0);
-    for (int s2 = 0; s2 < kChebDeg; ++s2) {
-        amg_level_spmm<K>(g, l, L.d, L.t);
-        hipLaunchKernelGGL((amg_cheb_step<B, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Dinv, L.t, L.r, L.d, xv,
-                           c1[s2], c2[s2]);
-    }
-}
+static void amg_level(plx_gf g, int l, const float *bv, float *xv);
 
 template <int K>
 static void amg_vcycle(plx_gf g, int l, const float *bv, float *xv) {
-    const int nl = (int)g->lv.size();
-    if (l == nl) {
+    if (l == (int)g->lv.size()) {
         hipLaunchKernelGGL(amg_dense<K>, dim3(std::max(1, std::min(kGfRed, (g->coarse_m * K + 255) / 256))), dim3(256),
                            0, g->stream, g->coarse_m, g->coarse_inv, bv, xv);
         return;
     }
-    const plx_gf_s::Level &L = g->lv[l];
-    float *nb = l + 1 < nl ? g->lv[l + 1].bv : g->cb;
-    float *nx = l + 1 < nl ? g->lv[l + 1].xv : g->cx;
-    if (L.b == 3) amg_smooth<3, K>(g, l, bv, xv, true);
-    else amg_smooth<6, K>(g, l, bv, xv, true);
-    // restrict the residual (L.r = b - A x after the smoother)
-    if (L.b == 3)
-        hipLaunchKernelGGL((amg_spmm<6, 3, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.nc, L.Rptr, L.Rcol, L.Rval, L.r,
-                           nb, 0);
-    else
-        hipLaunchKernelGGL((amg_spmm<6, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.nc, L.Rptr, L.Rcol, L.Rval, L.r,
-                           nb, 0);
-    amg_vcycle<K>(g, l + 1, nb, nx);
-    if (L.b == 3)
-        hipLaunchKernelGGL((amg_spmm<3, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Pptr, L.Pcol, L.Pval, nx,
-                           xv, 1);
-    else
-        hipLaunchKernelGGL((amg_spmm<6, 6, K>), dim3(kGfRed), dim3(256), 0, g->stream, L.n, L.Pptr, L.Pcol, L.Pval, nx,
-                           xv, 1);
-    if (L.b == 3) amg_smooth<3, K>(g, l, bv, xv, false);
-    else amg_smooth<6, K>(g, l, bv, xv, false);
+    if (g->lv[l].b == 3) amg_level<3, K>(g, l, bv, xv);
+    else amg_level<6, K>(g, l, bv, xv);
 }
 
-// Ten iterations of the multigrid-preconditioned FP32 block PCG.
+template <int B, int K>
+static void amg_level(plx_gf g, int l, const float *bv, float *xv) {
+    const plx_gf_s::Level &L = g->lv[l];
+    const int nl = (int)g->lv.size();
+    const int32_t *Ap = l == 0 ? g->d_row_ptr : L.Aptr;
+    const int32_t *Ac = l == 0 ? g->d_col : L.Acol;
+    const float *Av = l == 0 ? g->d_val32 : L.Aval;
+    const float s = (float)(2.0 / (L.lmax + L.lmax / 30.0));
+    float *x = L.d, *r = L.r;
+    float *nb = l + 1 < nl ? g->lv[l + 1].bv : g->cb;
+    float *nx = l + 1 < nl ? g->lv[l + 1].xv : g->cx;
+    hipLaunchKernelGGL((amg_jacobi0<B, K>), gf_grid((int64_t)L.n * K), dim3(256), 0, g->stream, L.n, L.Dinv, bv, x, s);
+    amg_bsr_launch<B, B, K, kBsrResid>(g, L.n, Ap, Ac, Av, x, bv, nullptr, r, 0.f, 0);
+    amg_bsr_launch<6, B, K, kBsrPlain>(g, L.nc, L.Rptr, L.Rcol, L.Rval, r, nullptr, nullptr, nb, 0.f, 0);
+    amg_vcycle<K>(g, l + 1, nb, nx);
+    amg_bsr_launch<B, 6, K, kBsrPlain>(g, L.n, L.Pptr, L.Pcol, L.Pval, nx, nullptr, nullptr, x, 0.f, 1);
+    amg_bsr_launch<B, B, K, kBsrPost>(g, L.n, Ap, Ac, Av, x, bv, L.Dinv, xv, s, 0);
+}
+
+// kAmgChunk iterations of the multigrid-preconditioned FP32 block PCG (few: the V-cycle
+// converges in a handful of iterations, and a convergence check costs one small copy).
+constexpr int kAmgChunk = 2;
 template <int K>
 static void gf_pcg_chunk_amg(plx_gf g) {
-    for (int it = 0; it < 10; ++it) {
+    for (int it = 0; it < kAmgChunk; ++it) {
         if (K == 64)
             hipLaunchKernelGGL(gfk_spmm_dot64, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
                                g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
@@ -941,7 +872,7 @@ static int gf_inner(plx_gf g, int k, int *its) {
         if (done) break;
         int err = gf_run_chunk(g, k);
         if (err) return err;
-        it += 10;
+        it += g->pc == PLX_GF_PC_AMG ? kAmgChunk : 10;
         PLX_HIP(dm, hipMemcpyAsync(rr.data(), g->scal + G_RR * 64, k * sizeof(double), hipMemcpyDeviceToHost,
                                    g->stream));
         PLX_HIP(dm, hipStreamSynchronize(g->stream));
@@ -1003,7 +934,7 @@ static void gf_amg_free(plx_gf g) {
     for (auto &L : g->lv)
         for (void *p : {(void *)L.Aptr, (void *)L.Acol, (void *)L.Aval, (void *)L.Pptr, (void *)L.Pcol, (void *)L.Pval,
                         (void *)L.Rptr, (void *)L.Rcol, (void *)L.Rval, (void *)L.Dinv, (void *)L.r, (void *)L.d,
-                        (void *)L.t, (void *)L.bv, (void *)L.xv})
+                        (void *)L.bv, (void *)L.xv})
             if (p) (void)hipFree(p);
     g->lv.clear();
     for (void *p : {(void *)g->coarse_inv, (void *)g->cb, (void *)g->cx})
@@ -1073,7 +1004,7 @@ static int gf_amg_setup(plx_gf g) {
             (err = upf(&L.Dinv, H.Dinv)))
             return err;
         const size_t nd = (size_t)H.n * H.b;
-        if ((err = ws(&L.r, nd)) || (err = ws(&L.d, nd)) || (err = ws(&L.t, nd))) return err;
+        if ((err = ws(&L.r, nd)) || (err = ws(&L.d, nd))) return err;
         if (l > 0 && ((err = ws(&L.bv, nd)) || (err = ws(&L.xv, nd)))) return err;
     }
     g->coarse_m = h.coarse_n;
@@ -1306,7 +1237,7 @@ int plx_gf_autotune(plx_gf g, int *best) {
             if ((err = gf_run_chunk(g, k))) return err;
         PLX_HIP(dm, hipStreamSynchronize(g->stream));
         const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
-        const double per = us / (30.0 * k);  // per RHS per iteration
+        const double per = us / (3.0 * (g->pc == PLX_GF_PC_AMG ? kAmgChunk : 10) * k);  // per RHS per iteration
         g->probe_us[c] = per;
         if (per < best_t) {
             best_t = per;

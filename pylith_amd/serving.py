@@ -35,12 +35,13 @@ class Request:
 class MeshState:
     """Resident device state of one mesh size: DM, fault, operator, block vectors."""
 
-    def __init__(self, size, seed, block, rtol, n_patch=(8, 8)):
+    def __init__(self, size, seed, block, rtol, n_patch=(8, 8), precond="amg"):
         from .greensfns import GreensFunctions
         self.size = size
         self.prob = workloads.faulted_box(size, seed=seed, vs_sigma=0.2)
         self.gf = GreensFunctions(self.prob.dm)
         self.gf.set_options(block=block)
+        self.gf.set_preconditioner(precond)
         self.block = block
         self.rtol = rtol
         self.patch = workloads.patch_index(self.prob.fault, *n_patch)
@@ -49,6 +50,8 @@ class MeshState:
         nnzb = 27 * n
         # operator (FP64 + FP32), block vectors (3 FP64 + 5 FP32 per column), DM and plan
         self.bytes = nnzb * 9 * 12 + 3 * n * block * (3 * 8 + 5 * 4) + 3 * n * 8 * 8 + self.prob.mesh.num_cells * 64
+        if precond == "amg":  # fine-level V-cycle workspace (2 x 64 FP32 columns) + transfer operators
+            self.bytes += 2 * 3 * n * 64 * 4 + 2 * n * 5 * 18 * 4
 
     def solve(self, reqs):
         slip = workloads.patch_slip(self.prob.fault, self.patch, np.stack([r.slip_values for r in reqs]))
@@ -62,8 +65,10 @@ class MeshState:
 
 
 class ServingEngine:
-    def __init__(self, sizes, budget_bytes=200 * GB, block=16, rtol=1e-10, seed=20261202, keep_results=False):
+    def __init__(self, sizes, budget_bytes=200 * GB, block=16, rtol=1e-10, seed=20261202, keep_results=False,
+                 precond="amg"):
         self.sizes = list(sizes)
+        self.precond = precond
         self.budget = budget_bytes
         self.block = block
         self.rtol = rtol
@@ -82,7 +87,7 @@ class ServingEngine:
     # --- resident state, admitted by HBM budget ------------------------------------------------
     def state(self, size):
         if size not in self.states:
-            st = MeshState(size, self.seed + size, self.block, self.rtol)
+            st = MeshState(size, self.seed + size, self.block, self.rtol, precond=self.precond)
             with self.cv:
                 if self.used + st.bytes > self.budget:
                     raise MemoryError(f"resident state of mesh size {size} exceeds the HBM budget")
