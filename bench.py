@@ -135,7 +135,48 @@ def bench_c2_solve(torch, mg, L):
     return out
 
 
-def bench_c3(torch, mg, L):
+def bench_c1_solve(torch, mg, L):
+    """C1 full solve: Q1 32^3 (6 km), -x fixed, +x u_y = 1 m, uniform moduli (rho 2500, Vs 3000,
+    Vp 5292), FP64 CG to 1e-8 relative residual with point-block Jacobi and with the multigrid
+    V-cycle (set-up timed separately), and FGMRES with the FP32 multigrid PCG inside."""
+    import numpy as np
+    from pylith_amd.dm import Solver
+    m = mg.hex_box(32, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(m.num_cells)
+    dm = setup(m, aux, sizes)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    dm.set_constraints(dofs)
+    solver = Solver(dm)
+    out, sols = {}, {}
+    for name, pc, method in (("cg_pbjacobi", "pbjacobi", L.KSP_CG), ("cg_amg", "amg", L.KSP_CG),
+                             ("fgmres_fp32_amg", "amg", L.KSP_FGMRES_IR)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.set_preconditioner(pc)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        x = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+        dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(x, method=method, rtol=1e-8, maxit=20000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sols[name] = x.cpu().numpy()
+        out[name] = {"seconds": dt, "preconditioner_setup_s": t_setup, "converged": bool(st["converged"]),
+                     "iterations": st["outer_iterations"], "inner_fp32": st["inner_iterations"],
+                     "rel_residual": st["final_residual"] / max(st["initial_residual"], 1e-300)}
+    ref = sols["cg_pbjacobi"]
+    for name in ("cg_amg", "fgmres_fp32_amg"):
+        out[name]["rel_diff_vs_pbjacobi"] = float(np.abs(sols[name] - ref).max() / np.abs(ref).max())
+        out[name]["speedup_vs_pbjacobi"] = out["cg_pbjacobi"]["seconds"] / out[name]["seconds"]
+    out["workload"] = "Q1 32^3 (107,811 dofs), -x fixed, +x u_y = 1 m, rtol 1e-8"
+    solver.close()
+    dm.close()
+    return out
+
+
+def bench_c3(torch, mg, L, precond="pbjacobi"):
     """C3: generalized Maxwell (3 elements, eta log-uniform [1e17, 1e19] per cell) + vertical
     fault (2 m left-lateral on a 4 x 4 km patch: 2 x 2 km at full slip tapering to zero at the
     patch edge), 48^3 Q1 hexes in a 6 km cube, 10 steps of 0.1 yr: per step the state term,
@@ -158,6 +199,9 @@ def bench_c3(torch, mg, L):
     dofs = np.sort((3 * fixed[:, None] + np.arange(3)).ravel()).astype(np.int32)
     dm.set_constraints(dofs)
     solver = Solver(dm)
+    t0 = time.perf_counter()
+    solver.set_preconditioner(precond)
+    t_pc = time.perf_counter() - t0
     nloc = dm.sizes()[0]
     nu = 3 * m.num_nodes
     x = torch.zeros(nloc, dtype=torch.float64, device="cuda")
@@ -198,6 +242,8 @@ def bench_c3(torch, mg, L):
     br = q1_bytes(m.num_nodes, m.num_cells, 8) + 6 * 8 * 8 * m.num_cells
     out = {
         "workload": "48^3 Q1 GenMaxwell (3 Maxwell elements/point) + vertical fault, 10 x 0.1 yr, FP64",
+        "preconditioner": precond, "preconditioner_setup_s": t_pc,
+        "inner_fp32_per_step": [s["inner_fp32"] for s in steps],
         "dofs": nu, "lagrange_dofs": nloc - nu, "cohesive_cells": f.num_cohesive,
         "steps_seconds_total": total, "step_ms_mean": 1e3 * total / 10,
         "all_converged": all(s["converged"] for s in steps),
@@ -386,7 +432,7 @@ def main():
     ap.add_argument("--q1-large", type=int, default=192, help="edge of the large Q1 cube")
     ap.add_argument("--no-systems", action="store_true",
                     help="skip the solver / viscoelastic / Green's-function / serving measurements")
-    ap.add_argument("--only", choices=["c2solve", "c3", "c4", "c5", "q1"], default=None,
+    ap.add_argument("--only", choices=["c1solve", "c2solve", "c3", "c3amg", "c4", "c5", "q1"], default=None,
                     help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -404,13 +450,15 @@ def main():
         sys.exit(1)
     if args.only:
         from pylith_amd import _lib as L
-        fn = {"c2solve": lambda: bench_c2_solve(torch, mg, L), "c3": lambda: bench_c3(torch, mg, L),
+        fn = {"c1solve": lambda: bench_c1_solve(torch, mg, L),
+              "c2solve": lambda: bench_c2_solve(torch, mg, L), "c3": lambda: bench_c3(torch, mg, L),
+              "c3amg": lambda: bench_c3(torch, mg, L, "amg"),
               "c4": lambda: bench_c4(torch, mg, L), "c5": lambda: bench_c5(torch),
               "q1": lambda: bench_q1(torch, mg, args.q1_large, args.steps, args.warmup)}[args.only]
         print(f"[bench] {args.only}", file=sys.stderr, flush=True)
         print(json.dumps({args.only: fn()}))
         return
-    print("[bench] building configs[1] workload", file=sys.stderr, flush=True)
+    print("[bench] building the P2 64^3 x 6 workload", file=sys.stderr, flush=True)
     mesh = mg.tet_p2_box(64, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
     aux, sizes = mg.elastic_aux(mesh.num_cells, seed=SEED, vs_sigma=0.2, vp_vs=1.732)
     u = mg.sinusoid_field(mesh.node_coords, SEED)
@@ -522,8 +570,10 @@ def main():
     if not args.no_systems:
         from pylith_amd import _lib as L
         systems = {}
-        for name, fn in (("c2_solve_fp32_refinement", lambda: bench_c2_solve(torch, mg, L)),
+        for name, fn in (("c1_solve", lambda: bench_c1_solve(torch, mg, L)),
+                         ("c2_solve_fp32_refinement", lambda: bench_c2_solve(torch, mg, L)),
                          ("c3_genmaxwell_fault", lambda: bench_c3(torch, mg, L)),
+                         ("c3_genmaxwell_fault_multigrid", lambda: bench_c3(torch, mg, L, "amg")),
                          ("c4_greens_functions", lambda: bench_c4(torch, mg, L)),
                          ("c5_serving", lambda: bench_c5(torch))):
             print(f"[bench] {name}", file=sys.stderr, flush=True)

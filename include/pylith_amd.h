@@ -226,6 +226,13 @@ int plx_solver_destroy(plx_solver solver);
 int plx_solver_setup(plx_solver solver);
 /* Inner FP32 PCG relative tolerance / iteration cap and the FGMRES restart length. */
 int plx_solver_set_inner(plx_solver solver, double rtol, int maxit, int restart);
+/* Preconditioner of every PCG the solver runs (the FP64 CG and the FP32 inner PCG): the inverse
+ * 3x3 point blocks (default; PyLith's vpbjacobi fine smoother) or one FP32 V-cycle of
+ * smoothed-aggregation multigrid on the assembled operator (the GAMG role of
+ * materials/Elasticity.cc:307-318; Q1 keys covering every cell).  plx_solver_setup rebuilds
+ * the hierarchy from the current operator. */
+enum { PLX_PC_PBJACOBI = 0, PLX_PC_AMG = 1 };
+int plx_solver_set_preconditioner(plx_solver solver, int kind);
 int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, double rtol,
                      int maxit, plx_solve_stats *stats);
 

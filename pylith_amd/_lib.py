@@ -10,6 +10,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpylith_a
 HEX_Q1, TET_P2 = 1, 2
 FP64, FP32 = 0, 1
 GF_PC_JACOBI, GF_PC_AMG = 0, 1
+PC_PBJACOBI, PC_AMG = 0, 1
 ADD_VALUES, INSERT_VALUES = 0, 1
 LHS, RHS, LHS_LUMPED_INV, LHS_WEIGHTED = 0, 1, 2, 3
 FN_NONE = 0
@@ -68,6 +69,7 @@ SIGNATURES = {
     "plx_gf_destroy": (_c_int, [_vp]),
     "plx_gf_set_options": (_c_int, [_vp, _c_int, _c_dbl, _c_int, _c_int]),
     "plx_gf_set_preconditioner": (_c_int, [_vp, _c_int]),
+    "plx_solver_set_preconditioner": (_c_int, [_vp, _c_int]),
     "plx_gf_autotune": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
     "plx_gf_probe_times": (_c_int, [_vp, _dblp]),
     "plx_gf_apply": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),

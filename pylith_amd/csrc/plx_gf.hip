@@ -470,6 +470,80 @@ __global__ void __launch_bounds__(256) amg_bsr(int nrows, const int32_t *__restr
     }
 }
 
+// Single-column (K = 1) form of amg_bsr: a group of G lanes per block row walks the row's
+// blocks (consecutive lanes read consecutive blocks), then the group's partial sums are combined
+// with cross-lane shuffles and the group's first lane writes the row.
+template <int BR, int BC, int G, int MODE>
+__global__ void __launch_bounds__(256) amg_bsr1(int nrows, const int32_t *__restrict__ ptr,
+                                                const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                const float *__restrict__ X, const float *__restrict__ Bv,
+                                                const float *__restrict__ Dinv, float *Y, float s, int accumulate) {
+    constexpr int RPW = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int j = lane % G;
+    int lo, hi, jb, nbx;
+    gf_xcd_slab(nrows, lo, hi, jb, nbx);
+    for (int row0 = lo + (jb * 4 + (int)(threadIdx.x >> 6)) * RPW; row0 < hi; row0 += nbx * 4 * RPW) {
+        const int row = row0 + lane / G;
+        float y[BR];
+#pragma unroll
+        for (int a = 0; a < BR; ++a) y[a] = 0.f;
+        if (row < hi) {
+            for (int e = ptr[row] + j; e < ptr[row + 1]; e += G) {
+                const size_t jc = (size_t)col[e];
+                const float *A = val + (size_t)BR * BC * e;
+                float x[BC];
+#pragma unroll
+                for (int q = 0; q < BC; ++q) x[q] = X[BC * jc + q];
+#pragma unroll
+                for (int a = 0; a < BR; ++a)
+#pragma unroll
+                    for (int q = 0; q < BC; ++q) y[a] += A[a * BC + q] * x[q];
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int a = 0; a < BR; ++a) y[a] += __shfl_xor(y[a], off, 64);
+        if (row < hi && j == 0) {
+            const size_t o = (size_t)BR * row;
+            if (MODE == kBsrPlain) {
+#pragma unroll
+                for (int a = 0; a < BR; ++a) Y[o + a] = accumulate ? Y[o + a] + y[a] : y[a];
+            } else if (MODE == kBsrResid) {
+#pragma unroll
+                for (int a = 0; a < BR; ++a) Y[o + a] = Bv[o + a] - y[a];
+            } else {
+                float rv[BR];
+#pragma unroll
+                for (int a = 0; a < BR; ++a) rv[a] = Bv[o + a] - y[a];
+                const float *Di = Dinv + (size_t)BR * BR * row;
+#pragma unroll
+                for (int a = 0; a < BR; ++a) {
+                    float z = 0.f;
+#pragma unroll
+                    for (int q = 0; q < BR; ++q) z += Di[a * BR + q] * rv[q];
+                    Y[o + a] = X[o + a] + s * z;
+                }
+            }
+        }
+    }
+}
+
+// coarsest level, one column: a wave per row of the dense inverse
+__global__ void __launch_bounds__(256) amg_dense1(int m, const float *__restrict__ Ainv, const float *__restrict__ Bv,
+                                                  float *X) {
+    const int lane = threadIdx.x & 63;
+    for (int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < m; i += gridDim.x * 4) {
+        const float *a = Ainv + (size_t)m * i;
+        float acc = 0.f;
+        for (int jj = lane; jj < m; jj += 64) acc += a[jj] * Bv[jj];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) X[i] = acc;
+    }
+}
+
 // pre-smoother from x = 0: x = s Dinv b (thread = (block row, column))
 template <int B, int K>
 __global__ void __launch_bounds__(256) amg_jacobi0(int n, const float *__restrict__ Dinv, const float *__restrict__ Bv,
@@ -764,11 +838,17 @@ static dim3 amg_grid(int nrows, int rows_per_wave) {
     return dim3((unsigned)blocks);
 }
 
-template <int BR, int BC, int K, int MODE>
+// G: lanes per block row of the single-column kernel (operator rows ~25-50 blocks, restriction
+// rows up to a few hundred, prolongation rows ~4-8)
+template <int BR, int BC, int K, int MODE, int G = 16>
 static void amg_bsr_launch(plx_gf g, int nrows, const int32_t *ptr, const int32_t *col, const float *val,
                            const float *X, const float *Bv, const float *Dinv, float *Y, float s, int accumulate) {
-    hipLaunchKernelGGL((amg_bsr<BR, BC, K, MODE>), amg_grid(nrows, 64 / K), dim3(256), 0, g->stream, nrows, ptr, col,
-                       val, X, Bv, Dinv, Y, s, accumulate);
+    if (K == 1)
+        hipLaunchKernelGGL((amg_bsr1<BR, BC, G, MODE>), amg_grid(nrows, 64 / G), dim3(256), 0, g->stream, nrows, ptr,
+                           col, val, X, Bv, Dinv, Y, s, accumulate);
+    else
+        hipLaunchKernelGGL((amg_bsr<BR, BC, K, MODE>), amg_grid(nrows, 64 / K), dim3(256), 0, g->stream, nrows, ptr,
+                           col, val, X, Bv, Dinv, Y, s, accumulate);
 }
 
 template <int B, int K>
@@ -777,8 +857,12 @@ static void amg_level(plx_gf g, int l, const float *bv, float *xv);
 template <int K>
 static void amg_vcycle(plx_gf g, int l, const float *bv, float *xv) {
     if (l == (int)g->lv.size()) {
-        hipLaunchKernelGGL(amg_dense<K>, dim3(std::max(1, std::min(kGfRed, (g->coarse_m * K + 255) / 256))), dim3(256),
-                           0, g->stream, g->coarse_m, g->coarse_inv, bv, xv);
+        if (K == 1)
+            hipLaunchKernelGGL(amg_dense1, dim3((g->coarse_m + 3) / 4), dim3(256), 0, g->stream, g->coarse_m,
+                               g->coarse_inv, bv, xv);
+        else
+            hipLaunchKernelGGL(amg_dense<K>, dim3(std::max(1, std::min(kGfRed, (g->coarse_m * K + 255) / 256))),
+                               dim3(256), 0, g->stream, g->coarse_m, g->coarse_inv, bv, xv);
         return;
     }
     if (g->lv[l].b == 3) amg_level<3, K>(g, l, bv, xv);
@@ -798,9 +882,9 @@ static void amg_level(plx_gf g, int l, const float *bv, float *xv) {
     float *nx = l + 1 < nl ? g->lv[l + 1].xv : g->cx;
     hipLaunchKernelGGL((amg_jacobi0<B, K>), gf_grid((int64_t)L.n * K), dim3(256), 0, g->stream, L.n, L.Dinv, bv, x, s);
     amg_bsr_launch<B, B, K, kBsrResid>(g, L.n, Ap, Ac, Av, x, bv, nullptr, r, 0.f, 0);
-    amg_bsr_launch<6, B, K, kBsrPlain>(g, L.nc, L.Rptr, L.Rcol, L.Rval, r, nullptr, nullptr, nb, 0.f, 0);
+    amg_bsr_launch<6, B, K, kBsrPlain, 64>(g, L.nc, L.Rptr, L.Rcol, L.Rval, r, nullptr, nullptr, nb, 0.f, 0);
     amg_vcycle<K>(g, l + 1, nb, nx);
-    amg_bsr_launch<B, 6, K, kBsrPlain>(g, L.n, L.Pptr, L.Pcol, L.Pval, nx, nullptr, nullptr, x, 0.f, 1);
+    amg_bsr_launch<B, 6, K, kBsrPlain, 8>(g, L.n, L.Pptr, L.Pcol, L.Pval, nx, nullptr, nullptr, x, 0.f, 1);
     amg_bsr_launch<B, B, K, kBsrPost>(g, L.n, Ap, Ac, Av, x, bv, L.Dinv, xv, s, 0);
 }
 
@@ -1018,6 +1102,18 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
              const std::vector<int32_t> &mcl, const std::vector<int32_t> &diag_pos,
              const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos);
 
+// One V-cycle on a single column, z = M r (3 x num_nodes FP32 vectors, merged layout), queued on
+// `stream`; used by the matrix-free solver as its multigrid preconditioner.
+int gf_vcycle1(plx_gf g, hipStream_t stream, const float *r, float *z) {
+    if (g->lv.empty()) return set_error(g->dm, 1, "multigrid hierarchy not built");
+    hipStream_t saved = g->stream;
+    g->stream = stream;
+    amg_vcycle<1>(g, 0, r, z);
+    g->stream = saved;
+    PLX_HIP(g->dm, hipGetLastError());
+    return 0;
+}
+
 }  // namespace plx
 
 extern "C" {
@@ -1026,18 +1122,19 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     if (!out || !dm) return 1;
     *out = nullptr;
     if (dm->cell_type != PLX_HEX_Q1) return set_error(dm, 1, "Green's functions are implemented for Q1 hexes");
-    if (!dm->fault) return set_error(dm, 1, "Green's functions need a fault");
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp || !kp->jf3 || !kp->has_aux) return set_error(dm, 1, "the key needs Jacobian kernels and aux");
     if (kp->cells.size() != (size_t)dm->num_cells) return set_error(dm, 1, "the key must cover every cell");
-    FaultData &F = *dm->fault;
     const int nn = dm->num_nodes, nc = (int)kp->cells.size();
     std::vector<int32_t> merged(nn);
     std::vector<uint8_t> is_pos(nn, 0);
     for (int i = 0; i < nn; ++i) merged[i] = i;
-    for (int l = 0; l < F.num_lag; ++l) {
-        merged[F.pos_node[l]] = F.neg_node[l];
-        is_pos[F.pos_node[l]] = 1;
+    if (dm->fault) {  // without a fault the merged operator is the masked stiffness itself
+        const FaultData &F = *dm->fault;
+        for (int l = 0; l < F.num_lag; ++l) {
+            merged[F.pos_node[l]] = F.neg_node[l];
+            is_pos[F.pos_node[l]] = 1;
+        }
     }
     std::vector<uint8_t> conmask(3 * (size_t)nn, 0);
     for (int32_t d : dm->con.dofs) {
@@ -1104,7 +1201,6 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
              const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos) {
     plx_dm dm = g->dm;
     KeyPlan *kp = g->kp;
-    FaultData &F = *dm->fault;
     const int nn = g->nrows;
     auto up = [&](auto **d, const auto &h) -> int {
         using T = typename std::decay<decltype(h)>::type::value_type;
@@ -1144,7 +1240,7 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
     PLX_HIP(dm, hipMalloc(&g->nrm, 64 * sizeof(double)));
     PLX_HIP(dm, hipMalloc(&g->v1, dm->local_size * sizeof(double)));
     PLX_HIP(dm, hipMalloc(&g->v2, dm->local_size * sizeof(double)));
-    PLX_HIP(dm, hipMalloc(&g->jump, 3 * (size_t)F.num_lag * sizeof(double)));
+    if (dm->fault) PLX_HIP(dm, hipMalloc(&g->jump, 3 * (size_t)dm->fault->num_lag * sizeof(double)));
     return 0;
 }
 
@@ -1259,6 +1355,7 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
                  plx_gf_stats *stats) {
     if (!g || num_impulses <= 0 || !slip || !u || !(rtol > 0.0) || maxit <= 0) return 1;
     plx_dm dm = g->dm;
+    if (!dm->fault) return set_error(dm, 1, "Green's-function impulses need a fault");
     FaultData &F = *dm->fault;
     int err = 0;
     if (g->block == 0 && (err = plx_gf_autotune(g, nullptr))) return err;

@@ -200,6 +200,9 @@ int launch_fault_jacobi(plx_dm dm, const double *r, double *z);
 int launch_fault_slip_rhs(plx_dm dm, double *lag_out);
 void free_fault(FaultData *f);
 
+// Multigrid V-cycle of a Green's-function handle on one column (plx_gf.hip).
+int gf_vcycle1(plx_gf g, hipStream_t stream, const float *r, float *z);
+
 // Vector kernels (plx_vector.hip).
 int launch_l2g(plx_dm dm, int scalar_type, const void *loc, void *glob, int mode);
 int launch_g2l(plx_dm dm, int scalar_type, const void *glob, void *loc);

@@ -134,6 +134,46 @@ __global__ void __launch_bounds__(256) p_update(int64_t n, const Real *__restric
         p[i] = z[i] + beta * p[i];
 }
 
+// Multigrid-preconditioned PCG pieces (the preconditioner is applied between kernels):
+// x += alpha p; r -= alpha q; partials of r.r
+template <typename Real>
+__global__ void __launch_bounds__(256) cg_update_xr(int64_t n, Real *x, Real *r, const Real *__restrict__ p,
+                                                    const Real *__restrict__ q, const double *__restrict__ scal,
+                                                    double *part_rr) {
+    __shared__ double s[256];
+    const Real alpha = (Real)scal[S_ALPHA];
+    double rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        x[i] += alpha * p[i];
+        const Real rv = r[i] - alpha * q[i];
+        r[i] = rv;
+        rr += (double)rv * (double)rv;
+    }
+    const double b = block_sum(rr, s);
+    if (threadIdx.x == 0) part_rr[blockIdx.x] = b;
+}
+
+// partials of r.z and (when part_rr is set) r.r; p = z when p is set
+template <typename Real>
+__global__ void __launch_bounds__(256) rz_partials(int64_t n, const Real *__restrict__ r, const Real *__restrict__ z,
+                                                   Real *p, double *part_rz, double *part_rr) {
+    __shared__ double s[256];
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double rv = (double)r[i], zv = (double)z[i];
+        rz += rv * zv;
+        rr += rv * rv;
+        if (p) p[i] = z[i];
+    }
+    const double a = block_sum(rz, s);
+    __syncthreads();
+    const double b = block_sum(rr, s);
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = a;
+        if (part_rr) part_rr[blockIdx.x] = b;
+    }
+}
+
 // z = Binv r, p = z, partials of r.z and r.r (start of PCG)
 template <typename Real>
 __global__ void __launch_bounds__(256) pc_start(int nnodes, const Real *__restrict__ Binv,
@@ -254,6 +294,8 @@ struct plx_solver_s {
     double inner_rtol = 1e-4;
     int inner_maxit = 2000;
     plx_solve_stats last{};
+    // multigrid preconditioner: the assembled merged operator and its hierarchy (Q1 keys)
+    plx_gf amg = nullptr;
     // fault (null-space method): jump across split nodes and Lagrange-space work vectors
     bool fault = false;
     double *jump = nullptr, *lr = nullptr, *lp = nullptr, *lq = nullptr, *lz = nullptr, *lx = nullptr;
@@ -305,11 +347,65 @@ static int dot64(plx_solver s, const double *a, const double *b, double *out) {
     return read_scalar(s, S_RZ, out);
 }
 
+// z = M r with the multigrid V-cycle (FP32; an FP64 pair goes through the FP32 work vectors,
+// which no FP64 iteration uses).
+template <typename Real>
+static int amg_precondition(plx_solver s, const Real *r, Real *z) {
+    plx_dm dm = s->dm;
+    if (std::is_same<Real, float>::value) return gf_vcycle1(s->amg, dm->stream, (const float *)r, (float *)z);
+    hipLaunchKernelGGL((convert_kernel<double, float>), red_grid(), dim3(256), 0, dm->stream, s->n,
+                       (const double *)r, s->r32, 1.0);
+    int err = gf_vcycle1(s->amg, dm->stream, s->r32, s->z32);
+    if (err) return err;
+    hipLaunchKernelGGL((convert_kernel<float, double>), red_grid(), dim3(256), 0, dm->stream, s->n, s->z32,
+                       (double *)z, 1.0);
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
+// PCG with the multigrid preconditioner (same contract as pcg below).
+template <typename Real>
+static int pcg_amg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, double rtol, int maxit, int *its,
+                   double *rnorm_out) {
+    plx_dm dm = s->dm;
+    int err = amg_precondition<Real>(s, r, z);
+    if (err) return err;
+    hipLaunchKernelGGL(rz_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, r, z, p, s->part_a, s->part_b);
+    hipLaunchKernelGGL(finish_start, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+    PLX_HIP(dm, hipGetLastError());
+    double rr0 = 0.0;
+    if ((err = read_scalar(s, S_RR, &rr0))) return err;
+    const double target = rtol * rtol * rr0;
+    int it = 0;
+    double rr = rr0;
+    const int check = 2;  // a V-cycle iteration is expensive next to a convergence check
+    while (it < maxit && rr > target && rr0 > 0.0) {
+        for (int k = 0; k < check && it < maxit; ++k, ++it) {
+            if ((err = masked_apply<Real>(s, p, q))) return err;
+            hipLaunchKernelGGL(dot_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, p, q, s->part_a);
+            hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
+            hipLaunchKernelGGL(cg_update_xr<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, x, r, p, q, s->scal,
+                               s->part_b);
+            if ((err = amg_precondition<Real>(s, r, z))) return err;
+            hipLaunchKernelGGL(rz_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, r, z, (Real *)nullptr,
+                               s->part_a, (double *)nullptr);
+            hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+            hipLaunchKernelGGL(p_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, z, p, s->scal);
+        }
+        PLX_HIP(dm, hipGetLastError());
+        if ((err = read_scalar(s, S_RR, &rr))) return err;
+    }
+    *its = it;
+    *rnorm_out = rr0 > 0.0 ? std::sqrt(rr / rr0) : 0.0;
+    return 0;
+}
+
 // PCG on A x = b (x, b, work vectors of type Real; x must hold the initial guess, masked).
 // Stops when ||r|| <= rtol ||b|| or maxit; checks convergence every `check` iterations.
 template <typename Real>
 static int pcg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, const Real *Binv,
                double rtol, int maxit, int *its, double *rnorm_out) {
+    if (s->amg) return pcg_amg<Real>(s, x, r, p, q, z, rtol, maxit, its, rnorm_out);
     plx_dm dm = s->dm;
     // r = b - A x  (caller stored b in r; x == 0 guess skips the apply)
     hipLaunchKernelGGL(pc_start<Real>, red_grid(), dim3(256), 0, dm->stream, s->nnodes, Binv, r, z, p,
@@ -515,6 +611,7 @@ int plx_solver_destroy(plx_solver s) {
         if (p) (void)hipFree(p);
     for (double *p : s->V) (void)hipFree(p);
     for (double *p : s->Z) (void)hipFree(p);
+    if (s->amg) plx_gf_destroy(s->amg);
     delete s;
     return 0;
 }
@@ -548,6 +645,34 @@ int plx_solver_setup(plx_solver s) {
     PLX_HIP(dm, hipStreamSynchronize(dm->stream));
     (void)hipFree(blocks);
     (void)hipFree(mask);
+    if (s->amg) {  // rebuild the hierarchy from the current operator
+        plx_gf_destroy(s->amg);
+        s->amg = nullptr;
+        return plx_solver_set_preconditioner(s, PLX_PC_AMG);
+    }
+    return 0;
+}
+
+int plx_solver_set_preconditioner(plx_solver s, int kind) {
+    if (!s) return 1;
+    plx_dm dm = s->dm;
+    if (kind == PLX_PC_PBJACOBI) {
+        if (s->amg) plx_gf_destroy(s->amg);
+        s->amg = nullptr;
+        return 0;
+    }
+    if (kind != PLX_PC_AMG) return set_error(dm, 1, "preconditioner must be PLX_PC_PBJACOBI or PLX_PC_AMG");
+    if (s->amg) return 0;
+    plx_gf g = nullptr;
+    int err = plx_gf_create(&g, dm, s->kp->label_value, s->kp->part);
+    if (!err) err = plx_gf_set_preconditioner(g, PLX_GF_PC_AMG);
+    if (err) {
+        if (g) plx_gf_destroy(g);
+        return err;
+    }
+    // the hierarchy is read on this handle's stream from now on
+    PLX_HIP(dm, hipDeviceSynchronize());
+    s->amg = g;
     return 0;
 }
 

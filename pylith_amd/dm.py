@@ -221,6 +221,12 @@ class Solver:
     def set_inner(self, rtol=1e-4, maxit=2000, restart=30):
         self.dm._check(self.lib.plx_solver_set_inner(self.h, rtol, maxit, restart))
 
+    def set_preconditioner(self, kind):
+        """'pbjacobi' (inverse 3x3 point blocks, the default) or 'amg' (smoothed-aggregation
+        multigrid V-cycle on the assembled operator; Q1 keys covering every cell)."""
+        code = {"pbjacobi": L.PC_PBJACOBI, "amg": L.PC_AMG}.get(kind, kind)
+        self.dm._check(self.lib.plx_solver_set_preconditioner(self.h, int(code)))
+
     def setup(self):
         self.dm._check(self.lib.plx_solver_setup(self.h))
 

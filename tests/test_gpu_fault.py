@@ -101,8 +101,9 @@ def dense_saddle_solution(oracle, m, f, aux, sizes, slip, dofs, vals):
     return x
 
 
+@pytest.mark.parametrize("pc", ["pbjacobi", "amg"])
 @pytest.mark.parametrize("method", [L.KSP_CG, L.KSP_FGMRES_IR, L.KSP_IR])
-def test_fault_solve_matches_dense_saddle_point(oracle, method):
+def test_fault_solve_matches_dense_saddle_point(oracle, method, pc):
     from pylith_amd.dm import Solver
     m, f, aux, sizes, dm = faulted((4, 3, 3), 2, shear=0.25)
     nu = 3 * m.num_nodes
@@ -117,10 +118,12 @@ def test_fault_solve_matches_dense_saddle_point(oracle, method):
     slip[:, 2] = 0.5 * slip[:, 1]
     dm.set_fault_slip(slip)
     ref = dense_saddle_solution(oracle, m, f, aux, sizes, slip, dofs, vals)
-    print(f"[fault solve] method={method} n={ref.size}", flush=True)
+    print(f"[fault solve] method={method} pc={pc} n={ref.size}", flush=True)
     x = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
     dm.insert_boundary_values(dev(vals), x)
-    st = Solver(dm).solve(x, method=method, rtol=1e-12, maxit=2000)
+    solver = Solver(dm)
+    solver.set_preconditioner(pc)
+    st = solver.solve(x, method=method, rtol=1e-12, maxit=2000)
     torch.cuda.synchronize()
     got = x.cpu().numpy()
     assert st["converged"], st

@@ -59,6 +59,50 @@ def test_solve_matches_dense_direct(oracle, kind, method):
     assert err <= 1e-9, (err, st)
 
 
+@pytest.mark.parametrize("method", [L.KSP_CG, L.KSP_FGMRES_IR, L.KSP_IR])
+def test_multigrid_solve_matches_dense_direct(oracle, method):
+    m = mg.hex_box(6, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=4, vs_sigma=0.2, vp_vs=1.732)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    ref = dense_solution(oracle, m, aux, sizes, dofs, vals)
+    dm, solver = build(m, aux, sizes, dofs)
+    solver.set_preconditioner("amg")
+    print(f"[solver-amg] method={method} dofs={3 * m.num_nodes}", flush=True)
+    x = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+    dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
+    st = solver.solve(x, method=method, rtol=1e-12, maxit=5000)
+    torch.cuda.synchronize()
+    err = np.abs(x.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert st["converged"], st
+    assert err <= 1e-9, (err, st)
+
+
+def test_multigrid_cuts_iterations_on_config1():
+    """32^3 Q1 (the config-1 mesh, lognormal moduli): FP64 CG with the V-cycle reaches the
+    Jacobi-preconditioned solution in a fraction of the iterations; set-up rebuilds it."""
+    m = mg.hex_box(32, (0, 0, 0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=5, vs_sigma=0.2, vp_vs=1.732)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    dm, solver = build(m, aux, sizes, dofs)
+    sols, its = {}, {}
+    for pc in ("pbjacobi", "amg", "amg-rebuilt"):
+        if pc == "amg":
+            solver.set_preconditioner("amg")
+        if pc == "amg-rebuilt":
+            solver.setup()
+        x = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+        dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
+        st = solver.solve(x, method=L.KSP_CG, rtol=1e-10, maxit=20000)
+        print(f"[solver-amg] config1 pc={pc} {st}", flush=True)
+        assert st["converged"], st
+        sols[pc], its[pc] = x.cpu().numpy(), st["outer_iterations"]
+    assert its["amg"] * 8 <= its["pbjacobi"], its
+    assert its["amg-rebuilt"] == its["amg"]
+    ref = sols["pbjacobi"]
+    for pc in ("amg", "amg-rebuilt"):
+        assert np.abs(sols[pc] - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
 def test_config1_fp32_refinement_matches_fp64(oracle):
     m = mg.hex_box(32, (0, 0, 0), (6000.0, 6000.0, 6000.0))
     aux, sizes = mg.elastic_aux(m.num_cells)
