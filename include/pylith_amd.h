@@ -86,8 +86,9 @@ int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size);
  *   PLX_ALGO_AUTO (default): FACTORED where it applies.
  *   PLX_ACCUMULATE_COLORED: element vectors enter the tile's LDS node buffer colour by colour, a
  *     fixed order: results are bitwise reproducible;
- *   PLX_ACCUMULATE_LDS_ATOMIC: LDS atomic adds, no colour phases; the last bits may vary run to run;
- *   PLX_ACCUMULATE_AUTO (default): LDS atomics for FP64 calls, colour phases for FP32 calls.
+ *   PLX_ACCUMULATE_LDS_ATOMIC: LDS atomic adds into an FP64 accumulator (FP32 calls round once
+ *     when the tile writes out), no colour phases; the last bits may vary run to run;
+ *   PLX_ACCUMULATE_AUTO (default): LDS atomics.
  * min_waves_per_simd (0 = default 2): occupancy the cell kernels' register allocation must
  * allow (2: <= 256 VGPRs, 3: <= 168, 4: <= 128); the rest spills. */
 enum plx_algorithm { PLX_ALGO_AUTO = 0, PLX_ALGO_QUADRATURE_POINTS = 1, PLX_ALGO_FACTORED = 2 };

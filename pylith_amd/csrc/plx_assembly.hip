@@ -192,12 +192,15 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
     Real *r = static_cast<Real *>(A.r);
     Real *halo = static_cast<Real *>(A.halo);
     const bool insert = A.mode == PLX_INSERT_VALUES;
+    // LDS atomics accumulate in FP64 whatever the vector type (native ds_add_f64; FP32 calls
+    // round once at the write-out)
+    using Acc = typename std::conditional<ACC == kAccAtomic, double, Real>::type;
     while (true) {
-        // LDS layout of this tile: coordinates, then nodal values, then the accumulator
+        // LDS layout of this tile: coordinates, the accumulator, then nodal values
         const int nxv = CT == PLX_HEX_Q1 ? cur.nn : cur.nvt;
         double *sX = reinterpret_cast<double *>(smem);
-        Real *sU = reinterpret_cast<Real *>(sX + 3 * nxv);
-        Real *sR = sU + 3 * cur.nn;
+        Acc *sR = reinterpret_cast<Acc *>(sX + 3 * nxv);
+        Real *sU = reinterpret_cast<Real *>(sR + 3 * cur.nn);
 #pragma unroll
         for (int j = 0; j < S::PFN; ++j) {
             const int k = tid + j * NT;
@@ -205,7 +208,7 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     sU[3 * k + c] = cur.pu[j][c];
-                    sR[3 * k + c] = Real(0);
+                    sR[3 * k + c] = Acc(0);
                 }
                 if constexpr (CT == PLX_HEX_Q1) {
 #pragma unroll
@@ -293,7 +296,7 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
 #pragma unroll
                     for (int b = 0; b < NB; ++b)
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) atomicAdd(&sR[3 * lc[b] + c], re[c][b]);
+                        for (int c = 0; c < 3; ++c) atomicAdd(&sR[3 * lc[b] + c], (Acc)re[c][b]);
                 }
             } else {
                 // cells of one colour share no node: fixed summation order (cells i = 0 of the
@@ -316,7 +319,7 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
             const size_t g = (size_t)A.p.tile_nodes[cur.n0 + k];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const Real v = sR[3 * k + c];
+                const Real v = (Real)sR[3 * k + c];
                 r[3 * g + c] = insert ? v : r[3 * g + c] + v;
             }
         }
@@ -324,7 +327,7 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
         for (int k = cur.nown + tid; k < cur.nn; k += NT) {
             const size_t s = hoff + (size_t)(k - cur.nown);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) halo[3 * s + c] = sR[3 * k + c];
+            for (int c = 0; c < 3; ++c) halo[3 * s + c] = (Real)sR[3 * k + c];
         }
         if (!has_next) break;
         __syncthreads();  // every read of this tile's LDS precedes the next tile's staging
@@ -392,12 +395,12 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     A.halo = kp->d_halo;
     A.mode = mode;
     A.records = kp->d_records;
-    const size_t lds = (size_t)kp->max_verts * 3 * sizeof(double) + (size_t)kp->max_nodes * 6 * sizeof(Real);
     // a per-point prestress needs the quadrature-point path
     const bool linear = dm->algorithm != PLX_ALGO_QUADRATURE_POINTS && !prestress;
-    // AUTO: LDS atomics for FP64 (fast native ds_add_f64), colour phases for FP32
-    const bool atomic = dm->accumulate == PLX_ACCUMULATE_LDS_ATOMIC ||
-                        (dm->accumulate == PLX_ACCUMULATE_AUTO && std::is_same<Real, double>::value);
+    // AUTO: LDS atomics (FP64 accumulator, native ds_add_f64) for both vector types
+    const bool atomic = dm->accumulate != PLX_ACCUMULATE_COLORED;
+    const size_t lds = (size_t)kp->max_verts * 3 * sizeof(double) +
+                       (size_t)kp->max_nodes * 3 * (sizeof(Real) + (atomic ? sizeof(double) : sizeof(Real)));
     // persistent grid: WG_PER_CU workgroups per CU (a multiple of 8 keeps each workgroup on
     // the XCD whose tile range it walks), never more than there are tiles
     int ncu = 0;
