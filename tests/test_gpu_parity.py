@@ -95,6 +95,31 @@ def test_q1_reference_fixture_mesh_and_label(oracle):
     check_residual(oracle, m, u, label=24, aux=(aux, sizes))
 
 
+@pytest.mark.parametrize("name", ["mms_hex.msh", "mms_tet.msh"])
+def test_reference_mms_meshes(oracle, name):
+    """The reference's MMS meshes (Gmsh 4.1 binary; tet.msh raised to P2): device residual of the
+    UniformStrain3D displacement against the oracle, interior rows zero (UniformStrain3D.cc)."""
+    from pylith_amd import meshio_gmsh
+    from test_meshio_gmsh import mms_aux, uniform_strain_disp
+    m, groups = meshio_gmsh.read(os.path.join(GOLDEN, name))
+    print(f"[parity] {name} cells={m.num_cells} nodes={m.num_nodes}", flush=True)
+    aux = mms_aux(m)
+    u = uniform_strain_disp(m.node_coords)
+    check_residual(oracle, m, u, label=24, aux=aux)
+    check_residual(oracle, m, mg.random_field(m.num_nodes, 21), label=24, aux=aux)
+    from pylith_amd.dm import DM
+    dm = DM(m)
+    dm.add_residual(24, 0, 0, F1)
+    dm.set_auxiliary(24, 0, *aux)
+    x = dev(u)
+    r = torch.zeros_like(x)
+    dm.compute_residual(24, 0, x, r, mode=1)
+    torch.cuda.synchronize()
+    rr = r.cpu().numpy().reshape(-1, 3)
+    interior = np.setdiff1d(np.arange(m.num_nodes), groups["boundary"])
+    assert np.abs(rr[interior]).max() <= 1e-12 * np.abs(rr).max()
+
+
 def test_q1_two_materials(oracle):
     m = mg.hex_box(8, (0, 0, 0), (800.0, 800.0, 800.0), brick=4)
     m.labels = (m.coords[m.cone[:, 0], 2] >= 400).astype(np.int32) + 1
