@@ -461,30 +461,39 @@ static int true_residual(plx_solver s, const double *b, const double *x, double 
 // out = M^-1 rhs on the fault (Lagrange space, three components), Jacobi-preconditioned CG in
 // FP64 to round-off (M is the well-conditioned fault-face mass matrix).
 static int fault_mass_solve(plx_solver s, const double *rhs, double *out) {
+    // Jacobi-preconditioned CG with its scalars on the device (finish_* kernels); the host
+    // reads the residual norm every 8 iterations only.
     plx_dm dm = s->dm;
     const int64_t n = 3 * (int64_t)dm->fault->num_lag;
     int err = 0;
     hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, n, out, 0.0);
     hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, rhs, 0.0, s->lr);
     if ((err = launch_fault_jacobi(dm, s->lr, s->lz))) return err;
-    hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, s->lz, 0.0, s->lp);
-    double rz = 0.0, bb = 0.0;
-    if ((err = dot64n(s, n, s->lr, s->lz, &rz)) || (err = dot64n(s, n, rhs, rhs, &bb))) return err;
-    for (int it = 0; it < 500 && bb > 0.0; ++it) {
-        double pq = 0.0, rr = 0.0, rz_new = 0.0;
-        if ((err = launch_fault_mass(dm, s->lp, s->lq)) || (err = dot64n(s, n, s->lp, s->lq, &pq))) return err;
-        if (pq <= 0.0) break;
-        const double alpha = rz / pq;
-        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, alpha, s->lp, 1.0, out);
-        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, -alpha, s->lq, 1.0, s->lr);
-        if ((err = dot64n(s, n, s->lr, s->lr, &rr))) return err;
-        if (rr <= 1e-30 * bb) break;
-        if ((err = launch_fault_jacobi(dm, s->lr, s->lz)) || (err = dot64n(s, n, s->lr, s->lz, &rz_new)))
-            return err;
-        hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, s->lz, rz_new / rz, s->lp);
-        rz = rz_new;
-    }
+    // p = z; r.z and r.r (= b.b)
+    hipLaunchKernelGGL(rz_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lr, s->lz, s->lp,
+                       s->part_a, s->part_b);
+    hipLaunchKernelGGL(finish_start, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
     PLX_HIP(dm, hipGetLastError());
+    double bb = 0.0;
+    if ((err = read_scalar(s, S_RR, &bb))) return err;
+    const int check = 8;
+    double rr = bb;
+    for (int it = 0; it < 500 && bb > 0.0 && rr > 1e-30 * bb;) {
+        for (int k = 0; k < check; ++k, ++it) {
+            if ((err = launch_fault_mass(dm, s->lp, s->lq))) return err;
+            hipLaunchKernelGGL(dot_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lp, s->lq, s->part_a);
+            hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
+            hipLaunchKernelGGL(cg_update_xr<double>, red_grid(), dim3(256), 0, dm->stream, n, out, s->lr, s->lp, s->lq,
+                               s->scal, s->part_b);
+            if ((err = launch_fault_jacobi(dm, s->lr, s->lz))) return err;
+            hipLaunchKernelGGL(rz_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lr, s->lz,
+                               (double *)nullptr, s->part_a, (double *)nullptr);
+            hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+            hipLaunchKernelGGL(p_update<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lz, s->lp, s->scal);
+        }
+        PLX_HIP(dm, hipGetLastError());
+        if ((err = read_scalar(s, S_RR, &rr))) return err;
+    }
     return 0;
 }
 
