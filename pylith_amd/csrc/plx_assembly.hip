@@ -403,8 +403,8 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
                        (size_t)kp->max_nodes * 3 * (sizeof(Real) + (atomic ? sizeof(double) : sizeof(Real)));
     // persistent grid: WG_PER_CU workgroups per CU (a multiple of 8 keeps each workgroup on
     // the XCD whose tile range it walks), never more than there are tiles
-    int ncu = 0;
-    PLX_HIP(dm, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
+    if (dm->num_cu <= 0) PLX_HIP(dm, hipDeviceGetAttribute(&dm->num_cu, hipDeviceAttributeMultiprocessorCount, 0));
+    const int ncu = dm->num_cu;
     const int resident = ((std::max(ncu, 8) * dm->wg_per_cu + 7) / 8) * 8;
     {
     ScopedEvent ev(dm, "plx:apply_tiles");

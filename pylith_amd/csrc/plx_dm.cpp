@@ -304,6 +304,7 @@ int plx_dm_destroy(plx_dm dm) {
 }
 
 int plx_dm_set_stream(plx_dm dm, void *stream) {
+    if (dm) dm->epoch++;
     if (!dm) return 1;
     dm->stream = (hipStream_t)stream;
     return 0;
@@ -312,6 +313,7 @@ int plx_dm_set_stream(plx_dm dm, void *stream) {
 const char *plx_dm_last_error(plx_dm dm) { return dm ? dm->last_error.c_str() : "null handle"; }
 
 int plx_dm_set_kernel_options(plx_dm dm, int algorithm, int accumulate, int min_waves_per_simd) {
+    if (dm) dm->epoch++;
     if (!dm) return 1;
     if (algorithm != PLX_ALGO_AUTO && algorithm != PLX_ALGO_QUADRATURE_POINTS &&
         algorithm != PLX_ALGO_FACTORED)
@@ -334,6 +336,7 @@ int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size) {
 }
 
 int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs) {
+    if (dm) dm->epoch++;
     if (!dm || num_constrained < 0 || (num_constrained > 0 && !dofs)) return set_error(dm, 1, "bad constraint arguments");
     std::vector<char> is_con(dm->local_size, 0);
     for (int i = 0; i < num_constrained; ++i) {
@@ -358,6 +361,7 @@ int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs) 
 }
 
 int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1) {
+    if (dm) dm->epoch++;
     if (!dm) return 1;
     if (!known_residual_pair(f0, f1)) return set_error(dm, 1, "no device twin for this residual kernel pair");
     KeyPlan *kp = nullptr;
@@ -369,6 +373,7 @@ int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1) {
 }
 
 int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, int jf2, int jf3) {
+    if (dm) dm->epoch++;
     if (!dm) return 1;
     if (jf0 || jf1 || jf2 ||
         (jf3 != PLX_JF3VU_ISOLINEAR_ELASTICITY_3D && jf3 != PLX_JF3VU_GENMAXWELL_3D))
@@ -384,6 +389,7 @@ int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, 
 }
 
 int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const double *constants) {
+    if (dm) dm->epoch++;
     if (!dm || n < 0 || (n > 0 && !constants)) return 1;
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp) return set_error(dm, 1, "unknown key");
@@ -394,6 +400,7 @@ int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const doub
 int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux, const int32_t *aux_sizes,
                          int aux_per_qp, const double *aux) {
     if (!dm || num_aux < 2 || !aux_sizes || !aux) return set_error(dm, 1, "bad auxiliary arguments");
+    dm->epoch++;
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp) return set_error(dm, 1, "unknown key");
     int naux_tot = 0;

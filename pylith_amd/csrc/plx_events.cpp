@@ -45,6 +45,7 @@ using namespace plx;
 extern "C" {
 
 int plx_dm_set_event_logging(plx_dm dm, int on) {
+    if (dm) dm->epoch++;
     if (!dm) return 1;
     fold_pending(dm);
     dm->events.on = on != 0;

@@ -145,6 +145,10 @@ struct plx_dm_s {
     int accumulate = PLX_ACCUMULATE_AUTO;
     int min_waves = 2;
     int wg_per_cu = 2;               // resident workgroups per CU of the persistent cell kernels
+    int num_cu = 0;                  // compute units of the device (queried once)
+    // bumped by every call that changes device state or launch choices (buffers, kernels,
+    // options, stream): captured graphs recorded at another epoch are stale
+    uint64_t epoch = 0;
     std::string last_error;
 };
 

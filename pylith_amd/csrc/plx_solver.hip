@@ -19,6 +19,8 @@
 // node's value to its positive twin.  U^T K U is symmetric positive definite and is what the
 // Krylov methods above see (corrections w carry zeros at the positive nodes; "fold" is U^T).
 // Finally the positive-side rows give M lambda = (A u)_+ - b_+.
+#include <array>
+#include <map>
 #include <vector>
 
 #include "plx_element.hpp"
@@ -296,6 +298,13 @@ struct plx_solver_s {
     plx_solve_stats last{};
     // multigrid preconditioner: the assembled merged operator and its hierarchy (Q1 keys)
     plx_gf amg = nullptr;
+    // solves run on this stream; chunks of Krylov iterations are captured as hipGraphs, keyed by
+    // (kind, buffers, preconditioner) and valid for one DM epoch
+    hipStream_t stream = nullptr;
+    bool use_graph = true;
+    uint64_t graph_epoch = ~0ull;
+    std::map<std::array<uintptr_t, 4>, hipGraphExec_t> graphs;
+    double *lsol = nullptr;  // fault-mass CG iterate
     // fault (null-space method): jump across split nodes and Lagrange-space work vectors
     bool fault = false;
     double *jump = nullptr, *lr = nullptr, *lp = nullptr, *lq = nullptr, *lz = nullptr, *lx = nullptr;
@@ -328,6 +337,42 @@ static int masked_apply(plx_solver s, Real *x, Real *y) {
 static int read_scalar(plx_solver s, int idx, double *v) {
     PLX_HIP(s->dm, hipMemcpyAsync(v, s->scal + idx, sizeof(double), hipMemcpyDeviceToHost, s->dm->stream));
     PLX_HIP(s->dm, hipStreamSynchronize(s->dm->stream));
+    return 0;
+}
+
+static void clear_graphs(plx_solver s) {
+    for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second);
+    s->graphs.clear();
+}
+
+// Run body() (a fixed chunk of iterations on fixed buffers, no host synchronisation) through a
+// captured graph; eager when graphs are off or the handle records events.
+template <typename F>
+static int run_chunk(plx_solver s, uintptr_t kind, const void *a, const void *b, F &&body) {
+    plx_dm dm = s->dm;
+    if (!s->use_graph || dm->events.on || !s->stream || dm->stream != s->stream) return body();
+    if (s->graph_epoch != dm->epoch) {
+        clear_graphs(s);
+        s->graph_epoch = dm->epoch;
+    }
+    const std::array<uintptr_t, 4> key{kind, (uintptr_t)a, (uintptr_t)b, (uintptr_t)s->amg};
+    auto it = s->graphs.find(key);
+    if (it == s->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        PLX_HIP(dm, hipStreamBeginCapture(dm->stream, hipStreamCaptureModeThreadLocal));
+        const int err = body();
+        const hipError_t e_end = hipStreamEndCapture(dm->stream, &graph);
+        if (err || e_end != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return err ? err : set_error(dm, 76, std::string("hipStreamEndCapture: ") + hipGetErrorString(e_end));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e_inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PLX_HIP(dm, e_inst);
+        it = s->graphs.emplace(key, exec).first;
+    }
+    PLX_HIP(dm, hipGraphLaunch(it->second, dm->stream));
     return 0;
 }
 
@@ -379,20 +424,34 @@ static int pcg_amg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, do
     int it = 0;
     double rr = rr0;
     const int check = 2;  // a V-cycle iteration is expensive next to a convergence check
-    while (it < maxit && rr > target && rr0 > 0.0) {
-        for (int k = 0; k < check && it < maxit; ++k, ++it) {
-            if ((err = masked_apply<Real>(s, p, q))) return err;
-            hipLaunchKernelGGL(dot_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, p, q, s->part_a);
-            hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
-            hipLaunchKernelGGL(cg_update_xr<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, x, r, p, q, s->scal,
-                               s->part_b);
-            if ((err = amg_precondition<Real>(s, r, z))) return err;
-            hipLaunchKernelGGL(rz_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, r, z, (Real *)nullptr,
-                               s->part_a, (double *)nullptr);
-            hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
-            hipLaunchKernelGGL(p_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, z, p, s->scal);
-        }
+    auto iteration = [&]() -> int {
+        int e = masked_apply<Real>(s, p, q);
+        if (e) return e;
+        hipLaunchKernelGGL(dot_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, p, q, s->part_a);
+        hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
+        hipLaunchKernelGGL(cg_update_xr<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, x, r, p, q, s->scal,
+                           s->part_b);
+        if ((e = amg_precondition<Real>(s, r, z))) return e;
+        hipLaunchKernelGGL(rz_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, r, z, (Real *)nullptr,
+                           s->part_a, (double *)nullptr);
+        hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+        hipLaunchKernelGGL(p_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, z, p, s->scal);
         PLX_HIP(dm, hipGetLastError());
+        return 0;
+    };
+    while (it < maxit && rr > target && rr0 > 0.0) {
+        if (it + check <= maxit) {
+            err = run_chunk(s, 10 * sizeof(Real) + 2, x, r, [&]() -> int {
+                for (int k = 0; k < check; ++k)
+                    if (int e = iteration()) return e;
+                return 0;
+            });
+            if (err) return err;
+            it += check;
+        } else {
+            for (; it < maxit; ++it)
+                if ((err = iteration())) return err;
+        }
         if ((err = read_scalar(s, S_RR, &rr))) return err;
     }
     *its = it;
@@ -419,17 +478,31 @@ static int pcg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, const 
     int it = 0;
     double rr = rr0;
     const int check = 10;
-    while (it < maxit && rr > target && rr0 > 0.0) {
-        for (int k = 0; k < check && it < maxit; ++k, ++it) {
-            if ((err = masked_apply<Real>(s, p, q))) return err;
-            hipLaunchKernelGGL(dot_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, p, q, s->part_a);
-            hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
-            hipLaunchKernelGGL(cg_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->nnodes, Binv, x, r,
-                               p, q, z, s->scal, s->part_a, s->part_b);
-            hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
-            hipLaunchKernelGGL(p_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, z, p, s->scal);
-        }
+    auto iteration = [&]() -> int {
+        int e = masked_apply<Real>(s, p, q);
+        if (e) return e;
+        hipLaunchKernelGGL(dot_partials<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, p, q, s->part_a);
+        hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
+        hipLaunchKernelGGL(cg_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->nnodes, Binv, x, r, p, q, z,
+                           s->scal, s->part_a, s->part_b);
+        hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+        hipLaunchKernelGGL(p_update<Real>, red_grid(), dim3(256), 0, dm->stream, s->n, z, p, s->scal);
         PLX_HIP(dm, hipGetLastError());
+        return 0;
+    };
+    while (it < maxit && rr > target && rr0 > 0.0) {
+        if (it + check <= maxit) {
+            err = run_chunk(s, 10 * sizeof(Real) + 1, x, r, [&]() -> int {
+                for (int k = 0; k < check; ++k)
+                    if (int e = iteration()) return e;
+                return 0;
+            });
+            if (err) return err;
+            it += check;
+        } else {
+            for (; it < maxit; ++it)
+                if ((err = iteration())) return err;
+        }
         if ((err = read_scalar(s, S_RR, &rr))) return err;
     }
     *its = it;
@@ -466,7 +539,8 @@ static int fault_mass_solve(plx_solver s, const double *rhs, double *out) {
     plx_dm dm = s->dm;
     const int64_t n = 3 * (int64_t)dm->fault->num_lag;
     int err = 0;
-    hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, n, out, 0.0);
+    double *xs = s->lsol;  // fixed iterate buffer (the chunks are captured)
+    hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, n, xs, 0.0);
     hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, rhs, 0.0, s->lr);
     if ((err = launch_fault_jacobi(dm, s->lr, s->lz))) return err;
     // p = z; r.z and r.r (= b.b)
@@ -478,22 +552,30 @@ static int fault_mass_solve(plx_solver s, const double *rhs, double *out) {
     if ((err = read_scalar(s, S_RR, &bb))) return err;
     const int check = 8;
     double rr = bb;
-    for (int it = 0; it < 500 && bb > 0.0 && rr > 1e-30 * bb;) {
-        for (int k = 0; k < check; ++k, ++it) {
-            if ((err = launch_fault_mass(dm, s->lp, s->lq))) return err;
-            hipLaunchKernelGGL(dot_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lp, s->lq, s->part_a);
-            hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
-            hipLaunchKernelGGL(cg_update_xr<double>, red_grid(), dim3(256), 0, dm->stream, n, out, s->lr, s->lp, s->lq,
-                               s->scal, s->part_b);
-            if ((err = launch_fault_jacobi(dm, s->lr, s->lz))) return err;
-            hipLaunchKernelGGL(rz_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lr, s->lz,
-                               (double *)nullptr, s->part_a, (double *)nullptr);
-            hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
-            hipLaunchKernelGGL(p_update<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lz, s->lp, s->scal);
-        }
-        PLX_HIP(dm, hipGetLastError());
+    for (int it = 0; it < 500 && bb > 0.0 && rr > 1e-30 * bb; it += check) {
+        err = run_chunk(s, 3, xs, s->lr, [&]() -> int {
+            for (int k = 0; k < check; ++k) {
+                int e = launch_fault_mass(dm, s->lp, s->lq);
+                if (e) return e;
+                hipLaunchKernelGGL(dot_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lp, s->lq,
+                                   s->part_a);
+                hipLaunchKernelGGL(finish_alpha, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->scal);
+                hipLaunchKernelGGL(cg_update_xr<double>, red_grid(), dim3(256), 0, dm->stream, n, xs, s->lr, s->lp,
+                                   s->lq, s->scal, s->part_b);
+                if ((e = launch_fault_jacobi(dm, s->lr, s->lz))) return e;
+                hipLaunchKernelGGL(rz_partials<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lr, s->lz,
+                                   (double *)nullptr, s->part_a, (double *)nullptr);
+                hipLaunchKernelGGL(finish_beta, dim3(1), dim3(256), 0, dm->stream, s->part_a, s->part_b, s->scal);
+                hipLaunchKernelGGL(p_update<double>, red_grid(), dim3(256), 0, dm->stream, n, s->lz, s->lp, s->scal);
+            }
+            PLX_HIP(dm, hipGetLastError());
+            return 0;
+        });
+        if (err) return err;
         if ((err = read_scalar(s, S_RR, &rr))) return err;
     }
+    hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, n, 1.0, xs, 0.0, out);
+    PLX_HIP(dm, hipGetLastError());
     return 0;
 }
 
@@ -599,9 +681,10 @@ int plx_solver_create(plx_solver *out, plx_dm dm, int label_value, int part) {
         ok = ok && hipMalloc(v, n * sizeof(double)) == hipSuccess;
     if (s->fault) {
         const size_t nl = 3 * (size_t)dm->fault->num_lag;
-        for (double **v : {&s->jump, &s->lr, &s->lp, &s->lq, &s->lz, &s->lx})
+        for (double **v : {&s->jump, &s->lr, &s->lp, &s->lq, &s->lz, &s->lx, &s->lsol})
             ok = ok && hipMalloc(v, nl * sizeof(double)) == hipSuccess;
     }
+    ok = ok && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
         plx_solver_destroy(s);
         return set_error(dm, 76, "solver allocation failed");
@@ -616,8 +699,10 @@ int plx_solver_destroy(plx_solver s) {
                     s->q32,    s->z32,    s->r64,  s->p64,    s->q64,    s->z64, s->t64};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (double *p : {s->jump, s->lr, s->lp, s->lq, s->lz, s->lx})
+    for (double *p : {s->jump, s->lr, s->lp, s->lq, s->lz, s->lx, s->lsol})
         if (p) (void)hipFree(p);
+    clear_graphs(s);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     for (double *p : s->V) (void)hipFree(p);
     for (double *p : s->Z) (void)hipFree(p);
     if (s->amg) plx_gf_destroy(s->amg);
@@ -654,6 +739,7 @@ int plx_solver_setup(plx_solver s) {
     PLX_HIP(dm, hipStreamSynchronize(dm->stream));
     (void)hipFree(blocks);
     (void)hipFree(mask);
+    clear_graphs(s);
     if (s->amg) {  // rebuild the hierarchy from the current operator
         plx_gf_destroy(s->amg);
         s->amg = nullptr;
@@ -665,6 +751,7 @@ int plx_solver_setup(plx_solver s) {
 int plx_solver_set_preconditioner(plx_solver s, int kind) {
     if (!s) return 1;
     plx_dm dm = s->dm;
+    clear_graphs(s);
     if (kind == PLX_PC_PBJACOBI) {
         if (s->amg) plx_gf_destroy(s->amg);
         s->amg = nullptr;
@@ -693,9 +780,12 @@ int plx_solver_set_inner(plx_solver s, double rtol, int maxit, int restart) {
     return 0;
 }
 
-int plx_solver_solve(plx_solver s, int method, const double *b, double *x, double rtol, int maxit,
-                     plx_solve_stats *stats) {
-    if (!s || !x || rtol <= 0.0 || maxit <= 0) return 1;
+}  // extern "C"
+
+namespace plx {
+
+static int solve_impl(plx_solver s, int method, const double *b, double *x, double rtol, int maxit,
+                      plx_solve_stats *stats) {
     plx_dm dm = s->dm;
     plx_solve_stats st{};
     int err = 0;
@@ -831,6 +921,33 @@ int plx_solver_solve(plx_solver s, int method, const double *b, double *x, doubl
     st.converged = st.final_residual <= rtol * r0 * (method == PLX_KSP_CG ? 10.0 : 1.0) ? 1 : 0;
     s->last = st;
     if (stats) *stats = st;
+    return 0;
+}
+
+}  // namespace plx
+
+extern "C" {
+
+// The solve runs on the solver's own stream (ordered after the caller's queued work and before
+// anything the caller queues afterwards), where its iteration chunks can be graph-captured.
+int plx_solver_solve(plx_solver s, int method, const double *b, double *x, double rtol, int maxit,
+                     plx_solve_stats *stats) {
+    if (!s || !x || rtol <= 0.0 || maxit <= 0) return 1;
+    plx_dm dm = s->dm;
+    hipStream_t caller = dm->stream;
+    hipEvent_t ev = nullptr;
+    PLX_HIP(dm, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (hipEventRecord(ev, caller) != hipSuccess || hipStreamWaitEvent(s->stream, ev, 0) != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        return set_error(dm, 76, "solver stream ordering failed");
+    }
+    dm->stream = s->stream;
+    const int err = solve_impl(s, method, b, x, rtol, maxit, stats);
+    dm->stream = caller;
+    const bool ordered = hipEventRecord(ev, s->stream) == hipSuccess && hipStreamWaitEvent(caller, ev, 0) == hipSuccess;
+    (void)hipEventDestroy(ev);
+    if (err) return err;
+    if (!ordered) return set_error(dm, 76, "solver stream ordering failed");
     return 0;
 }
 
