@@ -77,10 +77,11 @@ def setup(mesh, aux, sizes):
     return dm
 
 
-def cpu_baseline(problem, u, colors, budget_s=10.0):
-    """Oracle residual on the host: colored OpenMP over all usable cores for ~budget_s."""
+def cpu_baseline(problem, u, colors, budget_s=10.0, nthreads=None):
+    """Oracle residual on the host: colored OpenMP over all usable cores (or nthreads) for
+    ~budget_s (at least one apply)."""
     import numpy as np
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
     out = None
     n = 0
     t0 = time.perf_counter()
@@ -349,13 +350,13 @@ def bench_c4(torch, mg, L, n_impulses=512):
     return out
 
 
-def bench_c5(torch, n_requests=120):
+def bench_c5(torch, n_requests=120, max_block=32):
     """C5: Poisson stream of fault-slip solves on 16^3/32^3/64^3/96^3 meshes (40/30/20/10%),
     64 patches with N(0, 0.5 m) slip, lognormal Vs per cell (per mesh size), 200 GB budget;
     the arrival rate is 70% of the capacity measured with all requests queued at once."""
     from pylith_amd import serving
     sizes, probs = (16, 32, 64, 96), (0.4, 0.3, 0.2, 0.1)
-    eng = serving.ServingEngine(sizes, block=32, rtol=1e-8)
+    eng = serving.ServingEngine(sizes, block=max_block, rtol=1e-8)
     t0 = time.perf_counter()
     eng.warm()
     t_warm = time.perf_counter() - t0
@@ -394,9 +395,26 @@ def bench_c5(torch, n_requests=120):
     return {"workload": "Poisson stream, meshes 16^3/32^3/64^3/96^3 at 40/30/20/10%, 64 fault patches",
             "requests": n_requests, "capacity_solves_per_s": cap, "arrival_rate_per_s": rate,
             "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "solves_per_s": stats["solves_per_s"],
-            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-8, "max_block": 32,
+            "per_size": per, "warm_seconds": t_warm, "rtol": 1e-8, "max_block": max_block,
             "fp64_single_seconds": t64, "fp64_solves_per_s_mix": fp64_rate,
             "capacity_speedup_vs_fp64": cap / fp64_rate}
+
+
+def bench_p2(torch, mg, n_edge, steps, warmup):
+    """P2 FP64 residual apply on an n^3 x 6 box (same synthetic inputs as the headline)."""
+    mesh = mg.tet_p2_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(mesh.num_cells, seed=SEED, vs_sigma=0.2, vp_vs=1.732)
+    dm = setup(mesh, aux, sizes)
+    x = torch.from_numpy(mg.sinusoid_field(mesh.node_coords, SEED)).cuda()
+    r = torch.zeros_like(x)
+    f = lambda: dm.compute_residual(0, 0, x, r, mode=1)  # noqa: E731
+    sec = time_applies(f, steps, warmup, torch)
+    kms, _ = kernel_avg_ms(dm, f, steps, torch)
+    B = p2_bytes(mesh.num_nodes, mesh.num_vertices, mesh.num_cells, 8)
+    dm.close()
+    return {"cells": mesh.num_cells, "dofs": 3 * mesh.num_nodes, "gdofs": 3 * mesh.num_nodes / sec / 1e9,
+            "ms": sec * 1e3, "kernel_ms": kms, "algorithmic_bytes": B,
+            "hbm_frac_kernel": B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def bench_q1(torch, mg, n_edge, steps, warmup):
@@ -432,7 +450,8 @@ def main():
     ap.add_argument("--q1-large", type=int, default=192, help="edge of the large Q1 cube")
     ap.add_argument("--no-systems", action="store_true",
                     help="skip the solver / viscoelastic / Green's-function / serving measurements")
-    ap.add_argument("--only", choices=["c1solve", "c2solve", "c3", "c3amg", "c4", "c5", "q1"], default=None,
+    ap.add_argument("--only", choices=["c1solve", "c2solve", "c3", "c3amg", "c4", "c5", "c5b64", "q1", "p2large"],
+                    default=None,
                     help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -454,7 +473,9 @@ def main():
               "c2solve": lambda: bench_c2_solve(torch, mg, L), "c3": lambda: bench_c3(torch, mg, L),
               "c3amg": lambda: bench_c3(torch, mg, L, "amg"),
               "c4": lambda: bench_c4(torch, mg, L), "c5": lambda: bench_c5(torch),
-              "q1": lambda: bench_q1(torch, mg, args.q1_large, args.steps, args.warmup)}[args.only]
+              "c5b64": lambda: bench_c5(torch, max_block=64),
+              "q1": lambda: bench_q1(torch, mg, args.q1_large, args.steps, args.warmup),
+              "p2large": lambda: bench_p2(torch, mg, 96, args.steps, args.warmup)}[args.only]
         print(f"[bench] {args.only}", file=sys.stderr, flush=True)
         print(json.dumps({args.only: fn()}))
         return
@@ -523,10 +544,14 @@ def main():
         apply64()
         torch.cuda.synchronize()
         err = float(np.abs(r.cpu().numpy() - ref).max() / np.abs(ref).max())
+        print("[bench] CPU baseline, one core", file=sys.stderr, flush=True)
+        _, dt1, n1, _ = cpu_baseline(problem, u, mesh.colors, budget_s=4.0, nthreads=1)
         result["cpu_baseline"] = {
             "value": ndofs / dt / 1e9, "unit": "GDoF/s", "cores": nthreads, "kind": "port",
             "sample": f"{n} full applies of the same workload, oracle cell loop (PetscPointFn "
                       f"call per quadrature point), colored OpenMP over {nthreads} threads",
+            "single_core": {"value": ndofs / dt1 / 1e9, "unit": "GDoF/s", "cores": 1,
+                            "sample": f"{n1} full applies on one thread"},
         }
         result["parity"] = {"rel_err_inf_vs_oracle": err, "tol": 1e-12, "ok": err <= 1e-12}
 
@@ -560,6 +585,9 @@ def main():
         extra["p2_fp64_jacobian_action"] = {"gdofs": ndofs / secj / 1e9, "ms": secj * 1e3}
         dm.close()
         del x, r, x32, r32, jv
+        torch.cuda.empty_cache()
+        print("[bench] P2 96^3 x 6 (working set beyond the Infinity Cache)", file=sys.stderr, flush=True)
+        extra["p2_fp64_residual_96cube"] = bench_p2(torch, mg, 96, max(10, args.steps // 2), args.warmup)
         torch.cuda.empty_cache()
         for n_edge in (32, args.q1_large):
             print(f"[bench] Q1 hex {n_edge}^3", file=sys.stderr, flush=True)
