@@ -29,6 +29,10 @@ DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "integrator_driver.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "integrator_driver")
 
 
+AMG_PROBE_SRC = os.path.join(ROOT, "tests", "cpp", "amg_probe.cpp")
+AMG_PROBE = os.path.join(ROOT, "tests", "cpp", "libamg_probe.so")
+
+
 def build(verbose=False):
     cmd = [HIPCC, *COMMON_FLAGS, *DEVICE_FLAGS, "-x", "hip", *sources(), "-o", LIB + ".tmp"]
     if verbose:
@@ -36,7 +40,20 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     build_driver(verbose)
+    build_amg_probe(verbose)
     return LIB
+
+
+def build_amg_probe(verbose=False):
+    """Host-only test shim over the multigrid set-up (g++; no device code)."""
+    if not os.path.exists(AMG_PROBE_SRC):
+        return None
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-I", CSRC, AMG_PROBE_SRC,
+           os.path.join(CSRC, "plx_amg.cpp"), "-lpthread", "-o", AMG_PROBE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return AMG_PROBE
 
 
 def build_driver(verbose=False):

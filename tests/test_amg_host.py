@@ -1,0 +1,239 @@
+"""Host set-up of the smoothed-aggregation multigrid (pylith_amd/csrc/plx_amg.cpp, the GAMG role of
+materials/Elasticity.cc:307-318) on the merged operator of a faulted Q1 box assembled from the
+oracle's element matrices, through the g++-built shim tests/cpp/libamg_probe.so.  No GPU.
+
+Pins (each fails for a plausible set-up mistake):
+* R is P transposed and every coarse operator is the Galerkin product R A P (symmetric);
+* the stored inverse diagonal blocks invert the diagonal blocks; lambda_max bounds D^-1 A;
+* the prolongator reproduces all six rigid-body modes on rows away from the Dirichlet nodes
+  (a wrong sign or index in the rotation modes, or a broken per-aggregate QR, fails this);
+* the V-cycle is symmetric positive definite and cuts PCG iterations several-fold.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+scipy_sparse = pytest.importorskip("scipy.sparse")
+sp = scipy_sparse
+
+from pylith_amd import meshgen as mg  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "tests", "cpp", "libamg_probe.so")
+JF3 = 2
+
+
+@pytest.fixture(scope="module")
+def probe():
+    if not os.path.exists(LIB):
+        from pylith_amd import build
+        build.build_amg_probe()
+    lib = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    lib.amg_probe_build.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.c_int, ctypes.c_int]
+    lib.amg_probe_level.argtypes = [ctypes.c_int, P, P]
+    lib.amg_probe_array.argtypes = [ctypes.c_int, ctypes.c_int, P]
+    lib.amg_probe_coarse.argtypes = [P]
+    lib.amg_probe_vcycle.argtypes = [P, P]
+    lib.amg_probe_error.restype = ctypes.c_char_p
+    return lib
+
+
+@pytest.fixture(scope="module")
+def problem(oracle):
+    """Merged operator of a sheared 10^3 box with a dipping fault (positive split nodes and the
+    x-face nodes are identity rows), as the Green's-function assembly builds it."""
+    n, L = 10, 6000.0
+    shear = 0.5
+    m, f = mg.hex_box_with_fault(n, (0, 0, -L), (L, L, 0), n // 2, brick=4, shear=shear)
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=3, vs_sigma=0.2, vp_vs=1.732)
+    nn = m.num_nodes
+    merged = np.arange(nn)
+    merged[f.pos_node] = f.neg_node
+    cl = merged[m.closure]
+    # the cells are congruent parallelepipeds: K_cell = lambda K_lambda + mu K_mu
+    Pmu = oracle.Problem(m, np.tile([2500.0, 1.0, 2.0 / 3.0], (m.num_cells, 1)), sizes)
+    Plam = oracle.Problem(m, np.tile([2500.0, 0.0, 1.0], (m.num_cells, 1)), sizes)
+    Kmu, Klam = -Pmu.element_matrix(0, JF3), -Plam.element_matrix(0, JF3)
+    mu = aux[:, 1]
+    lam = aux[:, 2] - 2.0 / 3.0 * mu
+    d = (3 * cl[:, :, None] + np.arange(3)).reshape(m.num_cells, 24)
+    rows = np.repeat(d, 24, axis=1).ravel()
+    cols = np.tile(d, (1, 24)).ravel()
+    vals = (lam[:, None] * Klam.ravel()[None, :] + mu[:, None] * Kmu.ravel()[None, :]).ravel()
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(3 * nn, 3 * nn))
+    x0 = m.node_coords[:, 0] - shear * m.node_coords[:, 2]
+    fixed_nodes = np.nonzero(np.isclose(x0, 0.0) | np.isclose(x0, L))[0]
+    fixed = np.zeros(3 * nn, bool)
+    fixed[(3 * fixed_nodes[:, None] + np.arange(3)).ravel()] = True
+    fixed[(3 * f.pos_node[:, None] + np.arange(3)).ravel()] = True
+    keep = sp.diags((~fixed).astype(float))
+    A = (keep @ A @ keep + sp.diags(fixed.astype(float))).tocsr()
+    # 3x3 block pattern of the node graph (the identity rows keep their diagonal block only)
+    Ab = sp.bsr_matrix(A, blocksize=(3, 3))
+    Ab.sort_indices()
+    return dict(m=m, f=f, A=A, Ab=Ab, fixed=fixed, nn=nn, fixed_nodes=fixed_nodes)
+
+
+@pytest.fixture(scope="module")
+def hierarchy(probe, problem):
+    Ab, m = problem["Ab"], problem["m"]
+    ptr = Ab.indptr.astype(np.int32)
+    col = Ab.indices.astype(np.int32)
+    val = np.ascontiguousarray(Ab.data, np.float64)
+    coords = np.ascontiguousarray(m.node_coords, np.float64)
+    fx = problem["fixed"].astype(np.uint8)
+    nl = probe.amg_probe_build(problem["nn"], ptr.ctypes.data, col.ctypes.data, val.ctypes.data,
+                               coords.ctypes.data, fx.ctypes.data, 300, 8)
+    assert nl >= 1, probe.amg_probe_error()
+
+    def arr(l, which, dtype):
+        cnt = probe.amg_probe_array(l, which, None)
+        out = np.zeros(cnt, dtype)
+        probe.amg_probe_array(l, which, out.ctypes.data)
+        return out
+
+    levels = []
+    for l in range(nl):
+        info = np.zeros(3, np.int32)
+        lmax = ctypes.c_double()
+        probe.amg_probe_level(l, info.ctypes.data, ctypes.byref(lmax))
+        n, b, nc = (int(v) for v in info)
+        lv = dict(n=n, b=b, nc=nc, lmax=lmax.value)
+        lv["P"] = sp.bsr_matrix((arr(l, 5, np.float64).reshape(-1, b, 6), arr(l, 4, np.int32),
+                                 arr(l, 3, np.int32)), shape=(n * b, nc * 6)).tocsr()
+        lv["R"] = sp.bsr_matrix((arr(l, 8, np.float64).reshape(-1, 6, b), arr(l, 7, np.int32),
+                                 arr(l, 6, np.int32)), shape=(nc * 6, n * b)).tocsr()
+        lv["Dinv"] = arr(l, 9, np.float64).reshape(n, b, b)
+        if l == 0:
+            lv["A"] = problem["A"]
+        else:
+            lv["A"] = sp.bsr_matrix((arr(l, 2, np.float64).reshape(-1, 6, 6), arr(l, 1, np.int32),
+                                     arr(l, 0, np.int32)), shape=(n * 6, n * 6)).tocsr()
+        levels.append(lv)
+    cn = probe.amg_probe_coarse_n()
+    cinv = np.zeros(cn * cn)
+    probe.amg_probe_coarse(cinv.ctypes.data)
+    return dict(levels=levels, coarse_inv=cinv.reshape(cn, cn))
+
+
+def test_galerkin_coarse_operators(hierarchy):
+    lv = hierarchy["levels"]
+    for l, L in enumerate(lv):
+        assert abs(L["R"] - L["P"].T).max() == 0.0
+        Ac = (L["R"] @ L["A"] @ L["P"]).toarray()
+        nxt = lv[l + 1]["A"].toarray() if l + 1 < len(lv) else np.linalg.inv(hierarchy["coarse_inv"])
+        # the set-up writes 1 on the diagonal of dummy (rank-deficient) coarse dofs
+        dummy = np.abs(Ac).sum(1) == 0.0
+        Ac[dummy, dummy] = 1.0
+        scale = np.abs(Ac).max()
+        assert np.abs(nxt - Ac).max() <= 1e-9 * scale, l
+        assert np.abs(nxt - nxt.T).max() <= 1e-9 * scale, l
+
+
+def test_inverse_diagonal_blocks_and_spectrum_bound(hierarchy):
+    for L in hierarchy["levels"]:
+        b = L["b"]
+        A = sp.bsr_matrix(L["A"], blocksize=(b, b))
+        D = np.zeros((L["n"], b, b))
+        for i in range(L["n"]):
+            for k in range(A.indptr[i], A.indptr[i + 1]):
+                if A.indices[k] == i:
+                    D[i] = A.data[k]
+        prod = np.einsum("nij,njk->nik", L["Dinv"], D)
+        ok = np.abs(np.linalg.det(D)) > 1e-12 * np.abs(D).max()
+        assert np.allclose(prod[ok], np.eye(b), atol=1e-10)
+        # lambda_max(D^-1 A) <= lmax <= 1.2 x it (the set-up takes 1.1 x a power-iteration estimate)
+        Dinv = sp.block_diag(list(L["Dinv"])).tocsr()
+        ev = np.linalg.eigvals((Dinv @ L["A"]).toarray())
+        top = ev.real.max()
+        assert top <= L["lmax"] <= 1.2 * top
+
+
+def test_prolongator_reproduces_rigid_body_modes(problem, hierarchy):
+    m, fixed = problem["m"], problem["fixed"]
+    P = hierarchy["levels"][0]["P"]
+    x = m.node_coords - m.node_coords.mean(0)
+    nn = problem["nn"]
+    modes = []
+    for c in range(3):  # translations
+        t = np.zeros((nn, 3))
+        t[:, c] = 1.0
+        modes.append(t.ravel())
+    for w in np.eye(3):  # rotations w x x
+        modes.append(np.cross(w, x).ravel())
+    # rows whose stencil holds no fixed dof (there A m = 0 for a rigid mode m, so the smoothed
+    # prolongator keeps the tentative one's exact reproduction); positive split nodes are merged
+    A = problem["A"]
+    merged = np.arange(nn)
+    merged[problem["f"].pos_node] = problem["f"].neg_node
+    cl = merged[m.closure]
+    is_fixed_node = np.zeros(nn, bool)
+    is_fixed_node[problem["fixed_nodes"]] = True
+    near = np.zeros(nn, bool)
+    near[np.unique(cl[is_fixed_node[cl].any(1)])] = True
+    touched = np.repeat(near, 3)
+    rows = np.nonzero(~fixed & ~touched)[0]
+    assert np.abs(A[rows] @ (modes_check := np.tile([1.0, 0.0, 0.0], nn) * ~fixed)).max() <= 1e-6 * abs(A).max()
+    assert rows.size > 0.3 * A.shape[0]
+    Pd = P.toarray()
+    for k, mode in enumerate(modes):
+        mvec = mode * (~fixed)
+        c, *_ = np.linalg.lstsq(Pd[rows], mvec[rows], rcond=None)
+        err = np.abs(Pd[rows] @ c - mvec[rows]).max() / np.abs(mvec[rows]).max()
+        assert err <= 1e-10, (k, err)
+
+
+def test_vcycle_is_spd_and_cuts_iterations(probe, problem, hierarchy):
+    A, fixed = problem["A"], problem["fixed"]
+    n = A.shape[0]
+
+    def M(r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        probe.amg_probe_vcycle(r.ctypes.data, z.ctypes.data)
+        return z
+
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal(n) * ~fixed, rng.standard_normal(n) * ~fixed
+    assert abs(u @ M(v) - v @ M(u)) <= 1e-11 * abs(u @ M(u))
+    for _ in range(5):
+        w = rng.standard_normal(n) * ~fixed
+        assert w @ M(w) > 0.0
+
+    def pcg(prec, b, rtol=1e-8, maxit=2000):
+        x = np.zeros_like(b)
+        r = b.copy()
+        z = prec(r)
+        p = z.copy()
+        rz = r @ z
+        b0 = np.linalg.norm(b)
+        for it in range(maxit):
+            q = A @ p
+            a = rz / (p @ q)
+            x += a * p
+            r -= a * q
+            if np.linalg.norm(r) <= rtol * b0:
+                return x, it + 1
+            z = prec(r)
+            rzn = r @ z
+            p = z + rzn / rz * p
+            rz = rzn
+        return x, maxit
+
+    b = rng.standard_normal(n) * ~fixed
+    Ab = sp.bsr_matrix(A, blocksize=(3, 3))
+    D = np.zeros((n // 3, 3, 3))
+    for i in range(n // 3):
+        for k in range(Ab.indptr[i], Ab.indptr[i + 1]):
+            if Ab.indices[k] == i:
+                D[i] = Ab.data[k]
+    Dinv = np.linalg.inv(D)
+    jac = lambda r: np.einsum("nij,nj->ni", Dinv, r.reshape(-1, 3)).ravel()  # noqa: E731
+    xa, ita = pcg(M, b)
+    xj, itj = pcg(jac, b)
+    assert np.linalg.norm(A @ xa - b) <= 1e-8 * np.linalg.norm(b)
+    assert ita * 3 <= itj, (ita, itj)
+    assert ita <= 25, ita
