@@ -240,10 +240,12 @@ int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, 
 /* ---------------------------------------------------------------------------------------------
  * Batched Green's functions: GreensFns (problems/GreensFns.cc:236-331) with the impulses of
  * FaultCohesiveImpulses (faults/FaultCohesiveImpulses.cc:263-300) -- one linear solve per
- * fault-slip impulse, all sharing the operator.  The operator of the key (Q1, every cell, planar
- * fault) is assembled once at create, with the DM's constrained dofs; impulses are solved k at
- * a time (k in {8, 16, 32, 64}, autotuned when 0) by FP64 iterative refinement around an FP32
- * block PCG, each block solved to ||b - A x|| <= rtol ||b|| per impulse in FP64.
+ * fault-slip impulse, all sharing the operator.  The operator of the key (Q1, the key covering
+ * every cell) is assembled once at create as 3x3-block CSR over the nodes, with the DM's
+ * constrained dofs and, when the DM has a fault, the split nodes merged (the constraint-
+ * eliminated operator); plx_gf_solve needs a planar fault.  Impulses are solved k at a time
+ * (k in {8, 16, 32, 64}, autotuned when 0) by FP64 iterative refinement around an FP32 block
+ * PCG, each block solved to ||b - A x|| <= rtol ||b|| per impulse in FP64.
  *   slip [num_impulses][num_lagrange][3] (device, fault frame), u [num_impulses][local size]
  *   (device): the displacement of each impulse (Lagrange-multiplier entries set to zero). */
 typedef struct plx_gf_s *plx_gf;
