@@ -238,10 +238,12 @@ __global__ void __launch_bounds__(256) fill_kernel(int64_t n, Real *a, Real v) {
 }
 
 // y = a*x + b*y (FP64)
+// y = a x + b y; b = 0 assigns (y is not read: it may be uninitialised memory, where 0 * NaN would
+// propagate), a = 0 scales y alone
 __global__ void __launch_bounds__(256) axpby_kernel(int64_t n, double a, const double *__restrict__ x,
                                                     double b, double *y) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        y[i] = a * x[i] + b * y[i];
+        y[i] = b == 0.0 ? a * x[i] : (a == 0.0 ? b * y[i] : a * x[i] + b * y[i]);
 }
 
 // Invert the 3x3 point blocks of A = -J; rows/columns of constrained dofs become identity.
