@@ -249,6 +249,9 @@ int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, 
  *   slip [num_impulses][num_lagrange][3] (device, fault frame), u [num_impulses][local size]
  *   (device): the displacement of each impulse (Lagrange-multiplier entries set to zero). */
 typedef struct plx_gf_s *plx_gf;
+/* label_value of plx_solver_create / plx_gf_create selecting every key of the part that has
+ * Jacobian kernels and an auxiliary field (one key per material): the operator is their sum. */
+#define PLX_ALL_KEYS (-1)
 typedef struct {
     int block;                     /* block width used */
     int num_blocks;

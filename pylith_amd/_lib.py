@@ -11,6 +11,7 @@ HEX_Q1, TET_P2 = 1, 2
 FP64, FP32 = 0, 1
 GF_PC_JACOBI, GF_PC_AMG = 0, 1
 PC_PBJACOBI, PC_AMG = 0, 1
+ALL_KEYS = -1  # label value selecting every key of a part (one per material)
 ADD_VALUES, INSERT_VALUES = 0, 1
 LHS, RHS, LHS_LUMPED_INV, LHS_WEIGHTED = 0, 1, 2, 3
 FN_NONE = 0

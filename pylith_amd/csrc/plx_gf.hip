@@ -689,7 +689,8 @@ using namespace plx;
 
 struct plx_gf_s {
     plx_dm dm = nullptr;
-    KeyPlan *kp = nullptr;
+    KeyPlan *kp = nullptr;        // the first key
+    std::vector<KeyPlan *> keys;  // every key of the operator (one per material)
     hipStream_t stream = nullptr;
     int nrows = 0;
     int64_t ndof = 0, nnzb = 0;
@@ -1122,10 +1123,21 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     if (!out || !dm) return 1;
     *out = nullptr;
     if (dm->cell_type != PLX_HEX_Q1) return set_error(dm, 1, "Green's functions are implemented for Q1 hexes");
-    KeyPlan *kp = find_key(dm, label_value, part);
-    if (!kp || !kp->jf3 || !kp->has_aux) return set_error(dm, 1, "the key needs Jacobian kernels and aux");
-    if (kp->cells.size() != (size_t)dm->num_cells) return set_error(dm, 1, "the key must cover every cell");
-    const int nn = dm->num_nodes, nc = (int)kp->cells.size();
+    std::vector<KeyPlan *> keys;
+    if (label_value == PLX_ALL_KEYS) {
+        for (KeyPlan *k : dm->keys)
+            if (k->part == part && k->jf3 && k->has_aux) keys.push_back(k);
+    } else if (KeyPlan *k = find_key(dm, label_value, part)) {
+        keys.push_back(k);
+    }
+    if (keys.empty()) return set_error(dm, 1, "no key with Jacobian kernels and aux");
+    std::vector<int32_t> allcells;  // the keys' cells, key after key
+    for (KeyPlan *k : keys) {
+        if (!k->jf3 || !k->has_aux) return set_error(dm, 1, "the key needs Jacobian kernels and aux");
+        allcells.insert(allcells.end(), k->cells.begin(), k->cells.end());
+    }
+    if (allcells.size() != (size_t)dm->num_cells) return set_error(dm, 1, "the keys must cover every cell");
+    const int nn = dm->num_nodes, nc = (int)allcells.size();
     std::vector<int32_t> merged(nn);
     std::vector<uint8_t> is_pos(nn, 0);
     for (int i = 0; i < nn; ++i) merged[i] = i;
@@ -1147,7 +1159,7 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     std::vector<int32_t> inc_ptr(nn + 1, 0), inc;
     for (int p = 0; p < nc; ++p)
         for (int a = 0; a < 8; ++a) {
-            mcl[8 * (size_t)p + a] = merged[dm->closure[8 * (size_t)kp->cells[p] + a]];
+            mcl[8 * (size_t)p + a] = merged[dm->closure[8 * (size_t)allcells[p] + a]];
             inc_ptr[mcl[8 * (size_t)p + a] + 1]++;
         }
     for (int i = 0; i < nn; ++i) inc_ptr[i + 1] += inc_ptr[i];
@@ -1178,7 +1190,8 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     }
     plx_gf g = new plx_gf_s();
     g->dm = dm;
-    g->kp = kp;
+    g->kp = keys[0];
+    g->keys = keys;
     g->nrows = nn;
     g->ndof = 3 * (int64_t)nn;
     g->nnzb = (int64_t)col.size();
@@ -1200,7 +1213,6 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
              const std::vector<int32_t> &mcl, const std::vector<int32_t> &diag_pos,
              const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos) {
     plx_dm dm = g->dm;
-    KeyPlan *kp = g->kp;
     const int nn = g->nrows;
     auto up = [&](auto **d, const auto &h) -> int {
         using T = typename std::decay<decltype(h)>::type::value_type;
@@ -1223,8 +1235,15 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
     PLX_HIP(dm, hipMalloc(&g->d_val32, (size_t)g->nnzb * 9 * sizeof(float)));
     PLX_HIP(dm, hipMalloc(&g->d_Binv64, 9 * (size_t)nn * sizeof(double)));
     PLX_HIP(dm, hipMalloc(&g->d_Binv32, 9 * (size_t)nn * sizeof(float)));
-    hipLaunchKernelGGL(gf_element_kernel, dim3((8 * nc + 255) / 256), dim3(256), 0, g->stream, nc, kp->d_cells,
-                       dm->d_closure, dm->d_coords, kp->d_mat64, d_Ke);
+    {
+        size_t off = 0;  // element matrices of each key's cells with that key's moduli, key after key
+        for (KeyPlan *kp : g->keys) {
+            const int nk = (int)kp->cells.size();
+            hipLaunchKernelGGL(gf_element_kernel, dim3((8 * nk + 255) / 256), dim3(256), 0, g->stream, nk, kp->d_cells,
+                               dm->d_closure, dm->d_coords, kp->d_mat64, d_Ke + off * 64 * 9);
+            off += (size_t)nk;
+        }
+    }
     hipLaunchKernelGGL(gf_assemble_kernel, dim3(nn), dim3(256), 0, g->stream, nn, g->d_row_ptr, g->d_col, d_inc_ptr,
                        d_inc, d_mcl, d_Ke, d_con, d_pos, g->d_val64);
     hipLaunchKernelGGL(gf_diag_kernel, dim3((nn + 255) / 256), dim3(256), 0, g->stream, nn, d_diag, g->d_val64,
@@ -1388,7 +1407,14 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
             hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, g->v1, 0.0);
             hipLaunchKernelGGL(gf_scatter_jump, dim3((F.num_lag + 255) / 256), dim3(256), 0, g->stream, F.num_lag,
                                F.d_pos_node, g->jump, g->v1);
-            if ((err = launch_apply(dm, g->kp, PLX_FP64, true, g->v1, g->v2, PLX_INSERT_VALUES, 1.0))) break;
+            if (g->keys.size() == 1) {
+                if ((err = launch_apply(dm, g->kp, PLX_FP64, true, g->v1, g->v2, PLX_INSERT_VALUES, 1.0))) break;
+            } else {
+                hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, g->v2, 0.0);
+                for (KeyPlan *kp : g->keys)
+                    if ((err = launch_apply(dm, kp, PLX_FP64, true, g->v1, g->v2, PLX_ADD_VALUES, 1.0))) break;
+                if (err) break;
+            }
             if ((err = launch_fold(dm, PLX_FP64, g->v2, kFold))) break;
             if (nc > 0)
                 hipLaunchKernelGGL(gf_mask, dim3((nc + 255) / 256), dim3(256), 0, g->stream, nc, dm->con.d_dofs, g->v2);

@@ -282,7 +282,9 @@ using namespace plx;
 
 struct plx_solver_s {
     plx_dm dm = nullptr;
-    KeyPlan *kp = nullptr;
+    KeyPlan *kp = nullptr;           // the first key
+    std::vector<KeyPlan *> keys;     // every key of the operator (one per material)
+    int label_value = 0, part = 0;
     int64_t n = 0;
     int nnodes = 0;
     double *Binv64 = nullptr;
@@ -316,6 +318,21 @@ namespace plx {
 
 static inline dim3 red_grid() { return dim3(kRedBlocks); }
 
+// y = scale * J x summed over the solver's keys (the dofs of their closures are written: one key
+// inserts; several keys zero y first and accumulate)
+static int apply_keys(plx_solver s, int st, const void *x, void *y, double scale) {
+    plx_dm dm = s->dm;
+    if (s->keys.size() == 1) return launch_apply(dm, s->keys[0], st, true, x, y, PLX_INSERT_VALUES, scale);
+    if (st == PLX_FP64)
+        hipLaunchKernelGGL(fill_kernel<double>, red_grid(), dim3(256), 0, dm->stream, s->n, (double *)y, 0.0);
+    else
+        hipLaunchKernelGGL(fill_kernel<float>, red_grid(), dim3(256), 0, dm->stream, s->n, (float *)y, 0.0f);
+    PLX_HIP(dm, hipGetLastError());
+    for (KeyPlan *kp : s->keys)
+        if (int err = launch_apply(dm, kp, st, true, x, y, PLX_ADD_VALUES, scale)) return err;
+    return 0;
+}
+
 // y = A x on the unconstrained dofs.  With a fault x is a merged vector (zeros at positive split
 // nodes) and y = U^T A U x; x's positive entries are filled for the apply and cleared again.
 template <typename Real>
@@ -324,7 +341,7 @@ static int masked_apply(plx_solver s, Real *x, Real *y) {
     const int st = std::is_same<Real, double>::value ? PLX_FP64 : PLX_FP32;
     int err = 0;
     if (s->fault && (err = launch_fold(dm, st, x, kUnfold))) return err;
-    if ((err = launch_apply(dm, s->kp, st, true, x, y, PLX_INSERT_VALUES, -1.0))) return err;
+    if ((err = apply_keys(s, st, x, y, -1.0))) return err;
     if (s->fault && ((err = launch_fold(dm, st, x, kClearPos)) || (err = launch_fold(dm, st, y, kFold))))
         return err;
     const int nc = (int)dm->con.dofs.size();
@@ -516,7 +533,7 @@ static int pcg(plx_solver s, Real *x, Real *r, Real *p, Real *q, Real *z, const 
 // fault x is the full displacement, positive nodes included, and r is folded).
 static int true_residual(plx_solver s, const double *b, const double *x, double *r) {
     plx_dm dm = s->dm;
-    int err = launch_apply(dm, s->kp, PLX_FP64, true, x, r, PLX_INSERT_VALUES, -1.0);
+    int err = apply_keys(s, PLX_FP64, x, r, -1.0);
     if (err) return err;
     if (b)
         hipLaunchKernelGGL(axpby_kernel, red_grid(), dim3(256), 0, dm->stream, s->n, 1.0, b, -1.0, r);
@@ -601,7 +618,7 @@ static int fault_jump(plx_solver s, const double *b) {
 static int fault_lambda(plx_solver s, const double *b, double *x) {
     plx_dm dm = s->dm;
     const int64_t n = 3 * (int64_t)dm->fault->num_lag;
-    int err = launch_apply(dm, s->kp, PLX_FP64, true, x, s->t64, PLX_INSERT_VALUES, -1.0);
+    int err = apply_keys(s, PLX_FP64, x, s->t64, -1.0);
     if (err || (err = launch_gather_pos(dm, s->t64, s->lx))) return err;
     if (b) {
         if ((err = launch_gather_pos(dm, b, s->lq))) return err;
@@ -649,11 +666,23 @@ extern "C" {
 int plx_solver_create(plx_solver *out, plx_dm dm, int label_value, int part) {
     if (!out || !dm) return 1;
     *out = nullptr;
-    KeyPlan *kp = find_key(dm, label_value, part);
-    if (!kp || !kp->jf3 || !kp->has_aux) return set_error(dm, 1, "solver needs a key with Jacobian kernels and aux");
+    std::vector<KeyPlan *> keys;
+    if (label_value == PLX_ALL_KEYS) {
+        for (KeyPlan *k : dm->keys)
+            if (k->part == part && k->jf3 && k->has_aux) keys.push_back(k);
+        if (keys.empty()) return set_error(dm, 1, "no key of this part has Jacobian kernels and aux");
+    } else {
+        KeyPlan *kp = find_key(dm, label_value, part);
+        if (!kp || !kp->jf3 || !kp->has_aux)
+            return set_error(dm, 1, "solver needs a key with Jacobian kernels and aux");
+        keys.push_back(kp);
+    }
     plx_solver s = new plx_solver_s();
     s->dm = dm;
-    s->kp = kp;
+    s->kp = keys[0];
+    s->keys = keys;
+    s->label_value = label_value;
+    s->part = part;
     s->n = 3 * (int64_t)dm->num_nodes;  // the displacement block; Lagrange dofs are eliminated
     s->nnodes = dm->num_nodes;
     const size_t n = (size_t)s->n;
@@ -719,8 +748,9 @@ int plx_solver_setup(plx_solver s) {
     uint8_t *mask = nullptr;
     PLX_HIP(dm, hipMalloc(&blocks, 9 * (size_t)s->nnodes * sizeof(double)));
     PLX_HIP(dm, hipMemsetAsync(blocks, 0, 9 * (size_t)s->nnodes * sizeof(double), dm->stream));
-    int err = launch_pbdiag(dm, s->kp, blocks);
-    if (err) return err;
+    int err = 0;
+    for (KeyPlan *kp : s->keys)
+        if ((err = launch_pbdiag(dm, kp, blocks))) return err;
     std::vector<uint8_t> hmask(s->n, 0);
     for (int32_t d : dm->con.dofs) hmask[d] = 1;
     if (s->fault) {
@@ -762,7 +792,7 @@ int plx_solver_set_preconditioner(plx_solver s, int kind) {
     if (kind != PLX_PC_AMG) return set_error(dm, 1, "preconditioner must be PLX_PC_PBJACOBI or PLX_PC_AMG");
     if (s->amg) return 0;
     plx_gf g = nullptr;
-    int err = plx_gf_create(&g, dm, s->kp->label_value, s->kp->part);
+    int err = plx_gf_create(&g, dm, s->label_value, s->part);
     if (!err) err = plx_gf_set_preconditioner(g, PLX_GF_PC_AMG);
     if (err) {
         if (g) plx_gf_destroy(g);
