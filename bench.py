@@ -57,6 +57,33 @@ def time_applies(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
+def apply_percentiles(fn, steps, torch):
+    """Per-apply wall times (torch events on the handle's stream, each apply bracketed): median
+    and p99 in ms."""
+    import numpy as np
+    ts = []
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    return float(np.median(ts)), float(np.percentile(ts, 99))
+
+
+def cpu_description():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
 def kernel_avg_ms(dm, fn, steps, torch, name="plx:apply_tiles"):
     dm.reset_event_stats()
     dm.set_event_logging(True)
@@ -432,10 +459,15 @@ def bench_q1(torch, mg, n_edge, steps, warmup):
     s = time_applies(f, steps, warmup, torch)
     km, _ = kernel_avg_ms(hdm, f, steps, torch)
     hm_ms, _ = kernel_avg_ms(hdm, f, steps, torch, "plx:reduce_halo")
+    med, p99 = apply_percentiles(f, steps, torch)
     hb = q1_bytes(hm.num_nodes, hm.num_cells, 8)
-    out = {"gdofs": 3 * hm.num_nodes / s / 1e9, "ms": s * 1e3, "kernel_ms": km, "halo_reduce_ms": hm_ms,
+    jv = torch.zeros_like(hx)
+    sj = time_applies(lambda: hdm.compute_jacobian_action(0, 0, hx, jv, mode=1), steps, warmup, torch)
+    out = {"gdofs": 3 * hm.num_nodes / s / 1e9, "ms": s * 1e3, "ms_median": med, "ms_p99": p99,
+           "kernel_ms": km, "halo_reduce_ms": hm_ms,
            "algorithmic_bytes": hb, "hbm_frac_kernel": hb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS,
-           "hbm_frac_apply": hb / s / 1e9 / HBM_PEAK_GBS}
+           "hbm_frac_apply": hb / s / 1e9 / HBM_PEAK_GBS,
+           "jacobian_action_gdofs": 3 * hm.num_nodes / sj / 1e9}
     hdm.close()
     return out
 
@@ -495,6 +527,7 @@ def main():
     sec = time_applies(apply64, args.steps, args.warmup, torch)
     kms, kcount = kernel_avg_ms(dm, apply64, args.steps, torch)
     hms, _ = kernel_avg_ms(dm, apply64, args.steps, torch, "plx:reduce_halo")
+    med, p99 = apply_percentiles(apply64, args.steps, torch)
     B = p2_bytes(mesh.num_nodes, mesh.num_vertices, mesh.num_cells, 8)
     achieved = B / (kms * 1e-3) / 1e9
     traffic = None
@@ -509,6 +542,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": sec * 1e3,
+        "ms_per_step_median": med,
+        "ms_per_step_p99": p99,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -552,6 +587,7 @@ def main():
                       f"call per quadrature point), colored OpenMP over {nthreads} threads",
             "single_core": {"value": ndofs / dt1 / 1e9, "unit": "GDoF/s", "cores": 1,
                             "sample": f"{n1} full applies on one thread"},
+            "host": cpu_description(),
         }
         result["parity"] = {"rel_err_inf_vs_oracle": err, "tol": 1e-12, "ok": err <= 1e-12}
 
