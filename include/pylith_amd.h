@@ -51,7 +51,13 @@ enum plx_pointfn {
     PLX_F1V_GENMAXWELL_3D = 3,                /* IsotropicLinearGenMaxwell3D::f1v_infinitesimalStrain  (IsotropicLinearGenMaxwell.hh:1054) */
     PLX_JF3VU_GENMAXWELL_3D = 4,              /* IsotropicLinearGenMaxwell3D::Jf3vu_infinitesimalStrain (IsotropicLinearGenMaxwell.hh:1143) */
     PLX_VISCOUS_STRAIN_GENMAXWELL_3D = 5,     /* IsotropicLinearGenMaxwell3D::viscousStrain_infinitesimalStrain_asVector (:1507) */
-    PLX_TOTAL_STRAIN_3D = 6                   /* Elasticity3D::infinitesimalStrain_asVector (Elasticity.hh:586) */
+    PLX_TOTAL_STRAIN_3D = 6,                  /* Elasticity3D::infinitesimalStrain_asVector (Elasticity.hh:586) */
+    /* f0 of the quasi-static displacement equation (materials/Elasticity.cc:455-475); aux
+     * subfields density(1), then body_force(3) and/or gravitational_acceleration(3), then the
+     * rheology's subfields (materials/Elasticity.cc:234-241). */
+    PLX_F0V_BODYFORCE = 7,                    /* Elasticity::g0v_bodyforce     (Elasticity.hh:300) */
+    PLX_F0V_GRAV = 8,                         /* Elasticity::g0v_grav          (Elasticity.hh:249) */
+    PLX_F0V_GRAVBODYFORCE = 9                 /* Elasticity::g0v_gravbodyforce (Elasticity.hh:346) */
 };
 
 /* ---------------------------------------------------------------------------------------------

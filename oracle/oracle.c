@@ -210,8 +210,51 @@ static void total_strain_3d(int dim, int numS, int numA, const int sOff[], const
     strain_from_grad(&s_x[sOff_x[0]], out);
 }
 
+/* Body force and gravity, the f0 of the quasi-static displacement equation (Elasticity.hh:249-385,
+ * chosen in materials/Elasticity.cc:455-475).  Aux subfields: 0 density, then body_force and/or
+ * gravitational_acceleration (3 each); f0 += density * g (g0v_grav, Elasticity.hh:249-284),
+ * f0 += b (g0v_bodyforce, :300-329), or both, gravity first (g0v_gravbodyforce, :346-373). */
+static void f0v_grav(int dim, int numS, int numA, const int sOff[], const int sOff_x[],
+                     const double s[], const double s_t[], const double s_x[], const int aOff[],
+                     const int aOff_x[], const double a[], const double a_t[], const double a_x[],
+                     double t, const double x[], int numConstants, const double constants[],
+                     double f0[]) {
+    (void)numS; (void)numA; (void)sOff; (void)sOff_x; (void)s; (void)s_t; (void)s_x; (void)aOff_x;
+    (void)a_t; (void)a_x; (void)t; (void)x; (void)numConstants; (void)constants;
+    const double rho = a[aOff[0]];
+    const double *g = &a[aOff[1]];
+    for (int i = 0; i < dim; ++i) f0[i] += rho * g[i];
+}
+
+static void f0v_bodyforce(int dim, int numS, int numA, const int sOff[], const int sOff_x[],
+                          const double s[], const double s_t[], const double s_x[], const int aOff[],
+                          const int aOff_x[], const double a[], const double a_t[], const double a_x[],
+                          double t, const double x[], int numConstants, const double constants[],
+                          double f0[]) {
+    (void)numS; (void)numA; (void)sOff; (void)sOff_x; (void)s; (void)s_t; (void)s_x; (void)aOff_x;
+    (void)a_t; (void)a_x; (void)t; (void)x; (void)numConstants; (void)constants;
+    const double *b = &a[aOff[1]];
+    for (int i = 0; i < dim; ++i) f0[i] += b[i];
+}
+
+static void f0v_gravbodyforce(int dim, int numS, int numA, const int sOff[], const int sOff_x[],
+                              const double s[], const double s_t[], const double s_x[],
+                              const int aOff[], const int aOff_x[], const double a[],
+                              const double a_t[], const double a_x[], double t, const double x[],
+                              int numConstants, const double constants[], double f0[]) {
+    const int offGrav[2] = {aOff[0], aOff[2]};
+    const int offBody[2] = {aOff[0], aOff[1]};
+    f0v_grav(dim, numS, numA, sOff, sOff_x, s, s_t, s_x, offGrav, aOff_x, a, a_t, a_x, t, x,
+             numConstants, constants, f0);
+    f0v_bodyforce(dim, numS, numA, sOff, sOff_x, s, s_t, s_x, offBody, aOff_x, a, a_t, a_x, t, x,
+                  numConstants, constants, f0);
+}
+
 oracle_point_fn oracle_point_kernel(int id) {
     switch (id) {
+    case ORACLE_F0V_BODYFORCE: return f0v_bodyforce;
+    case ORACLE_F0V_GRAV: return f0v_grav;
+    case ORACLE_F0V_GRAVBODYFORCE: return f0v_gravbodyforce;
     case ORACLE_F1V_ISOLINEAR_ELASTICITY_3D: return f1v_isolinear_elasticity_3d;
     case ORACLE_F1V_GENMAXWELL_3D: return f1v_genmaxwell_3d;
     case ORACLE_VISCOUS_STRAIN_GENMAXWELL_3D: return viscous_strain_genmaxwell_3d;

@@ -5,7 +5,8 @@
  *
  * What it restates (reference file:line):
  *   - pointwise kernels with the PetscPointFn / PetscPointJacFn / PetscBdPointFn
- *     signature (libsrc/pylith/fekernels/ headers, cited per function in oracle.c);
+ *     signature (libsrc/pylith/fekernels/ headers, cited per function in oracle.c): elastic and
+ *     generalized Maxwell f1 / Jf3, body force and gravity f0, state projections, fault rows;
  *   - PETSc's DMPlexComputeResidualByKey / DMPlexComputeJacobianActionByKey cell
  *     loop as PyLith drives it (libsrc/pylith/feassemble/IntegratorDomain.cc:448-449,
  *     :540-541): closure gather, per-quadrature-point geometry, pointwise kernel
@@ -64,7 +65,10 @@ enum {
     ORACLE_F1V_GENMAXWELL_3D = 3,
     ORACLE_JF3VU_GENMAXWELL_3D = 4,
     ORACLE_VISCOUS_STRAIN_GENMAXWELL_3D = 5,
-    ORACLE_TOTAL_STRAIN_3D = 6
+    ORACLE_TOTAL_STRAIN_3D = 6,
+    ORACLE_F0V_BODYFORCE = 7,
+    ORACLE_F0V_GRAV = 8,
+    ORACLE_F0V_GRAVBODYFORCE = 9
 };
 
 /* Mesh + discretization + auxiliary field of one integrator key. */

@@ -21,6 +21,9 @@ F1V_GENMAXWELL_3D = 3
 JF3VU_GENMAXWELL_3D = 4
 VISCOUS_STRAIN_GENMAXWELL_3D = 5
 TOTAL_STRAIN_3D = 6
+F0V_BODYFORCE = 7
+F0V_GRAV = 8
+F0V_GRAVBODYFORCE = 9
 KSP_CG, KSP_FGMRES_IR, KSP_IR = 0, 1, 2
 ALGO_AUTO, ALGO_QUADRATURE_POINTS, ALGO_FACTORED = 0, 1, 2
 ACCUMULATE_AUTO, ACCUMULATE_COLORED, ACCUMULATE_LDS_ATOMIC = 0, 1, 2

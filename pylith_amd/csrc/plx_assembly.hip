@@ -475,6 +475,30 @@ int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const v
     return set_error(dm, 1, "bad scalar type");
 }
 
+// Body force / gravity load (fixed per node): r[node] += load.
+template <typename Real>
+__global__ void __launch_bounds__(256) add_load(int n, const int32_t *__restrict__ node,
+                                                const Real *__restrict__ load, Real *r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t g = 3 * (size_t)node[i];
+    r[g + 0] += load[3 * (size_t)i + 0];
+    r[g + 1] += load[3 * (size_t)i + 1];
+    r[g + 2] += load[3 * (size_t)i + 2];
+}
+
+int launch_add_load(plx_dm dm, KeyPlan *kp, int scalar_type, void *r) {
+    const int n = kp->num_load;
+    if (scalar_type == PLX_FP64)
+        hipLaunchKernelGGL(add_load<double>, dim3((n + 255) / 256), dim3(256), 0, dm->stream, n,
+                           kp->d_load_node, kp->d_load64, static_cast<double *>(r));
+    else
+        hipLaunchKernelGGL(add_load<float>, dim3((n + 255) / 256), dim3(256), 0, dm->stream, n,
+                           kp->d_load_node, kp->d_load32, static_cast<float *>(r));
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
 // --------------------------------------------------------------------------------------------
 // Point-block diagonal of J (FP64): D_b[f][g] = -sum_q w|J| [(lambda+mu) dphi_f dphi_g
 //   + mu delta_fg |grad phi|^2] for isotropic C, i.e. the diagonal blocks of the element

@@ -39,7 +39,8 @@ static void free_plan_arrays(KeyPlan *k) {
                     k->d_lclosure,    k->d_lcone,         k->d_shared_node,   k->d_color,
                     k->d_tile_ncolor,   k->d_records,       k->d_shared_ptr,    k->d_tiles_affine,  k->d_tiles_general,
                     k->d_shared_slot, k->d_halo,          k->d_mat64,         k->d_mat32,
-                    k->d_state,       k->d_gm_mat,        k->d_prestress};
+                    k->d_state,       k->d_gm_mat,        k->d_prestress,     k->d_load_node,
+                    k->d_load64,      k->d_load32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     KeyPlan fresh;
@@ -154,6 +155,116 @@ static int build_plan(plx_dm dm, KeyPlan *kp) {
 // Elastic moduli are the last two subfields (IsotropicLinearElasticity::setContext,
 // IsotropicLinearElasticity.hh:116-117) or, for generalized Maxwell, subfields numA-6 and
 // numA-5 (IsotropicLinearGenMaxwell.hh:133-134); per-qp aux must be uniform in a cell there.
+// Body force / gravity (Elasticity::g0v_bodyforce, g0v_grav, g0v_gravbodyforce,
+// fekernels/Elasticity.hh:249-385).  These f0 read the auxiliary field only, so their part of the
+// residual, sum_q w_q |J_q| phi_b(q) f0(q) per closure node b, is a fixed vector: computed here
+// (host, FP64, in the key's cell order) whenever the kernels or the aux field change, and added
+// after the cell pass of every residual (not of the Jacobian action: d f0 / d u = 0).
+static int build_load(plx_dm dm, KeyPlan *kp) {
+    if (kp->d_load_node) (void)hipFree(kp->d_load_node);
+    if (kp->d_load64) (void)hipFree(kp->d_load64);
+    if (kp->d_load32) (void)hipFree(kp->d_load32);
+    kp->d_load_node = nullptr;
+    kp->d_load64 = nullptr;
+    kp->d_load32 = nullptr;
+    kp->num_load = 0;
+    if (kp->f0 == PLX_FN_NONE || !kp->has_aux) return 0;
+    // aux subfields the kernel reads (Elasticity.cc:234-241 order)
+    const bool grav = kp->f0 == PLX_F0V_GRAV || kp->f0 == PLX_F0V_GRAVBODYFORCE;
+    const bool body = kp->f0 == PLX_F0V_BODYFORCE || kp->f0 == PLX_F0V_GRAVBODYFORCE;
+    const int i_body = 1, i_grav = kp->f0 == PLX_F0V_GRAVBODYFORCE ? 2 : 1;
+    const int nsub = kp->f0 == PLX_F0V_GRAVBODYFORCE ? 3 : 2;
+    if (kp->num_aux < nsub + 2 || kp->aux_sizes[0] != 1 || kp->aux_sizes[1] != 3 ||
+        (nsub == 3 && kp->aux_sizes[2] != 3))
+        return set_error(dm, 1, "body force / gravity need aux subfields density(1), then 3-vectors");
+    std::vector<int> off(kp->num_aux);
+    int naux_tot = 0;
+    for (int i = 0; i < kp->num_aux; ++i) {
+        off[i] = naux_tot;
+        naux_tot += kp->aux_sizes[i];
+    }
+    const int per = kp->aux_per_qp ? dm->nq : 1;
+    // quadrature weights and basis values (Q1: 2x2x2 Gauss on [-1,1]^3, PyLith vertex order;
+    // P2: the 14-point rule, vertices then edges (0,1),(0,2),(0,3),(1,2),(1,3),(2,3))
+    const int nb = dm->nb, nq = dm->nq;
+    std::vector<double> w(nq), phi((size_t)nq * nb), dq((size_t)nq * nb * 3);
+    const double ref[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                              {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    if (dm->cell_type == PLX_HEX_Q1) {
+        const double g = 0.57735026918962576451;
+        for (int q = 0; q < 8; ++q) {
+            const double xi[3] = {(q & 1) ? g : -g, (q & 2) ? g : -g, (q & 4) ? g : -g};
+            w[q] = 1.0;
+            for (int b = 0; b < 8; ++b) {
+                const double a0 = 1 + ref[b][0] * xi[0], a1 = 1 + ref[b][1] * xi[1], a2 = 1 + ref[b][2] * xi[2];
+                phi[(size_t)q * 8 + b] = 0.125 * a0 * a1 * a2;
+                dq[((size_t)q * 8 + b) * 3 + 0] = 0.125 * ref[b][0] * a1 * a2;
+                dq[((size_t)q * 8 + b) * 3 + 1] = 0.125 * a0 * ref[b][1] * a2;
+                dq[((size_t)q * 8 + b) * 3 + 2] = 0.125 * a0 * a1 * ref[b][2];
+            }
+        }
+    } else {
+        P2Tables t;
+        p2_fill_tables(&t);
+        const int e[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+        for (int q = 0; q < nq; ++q) {
+            const double *l = t.lam[q];
+            w[q] = t.w[q];
+            for (int v = 0; v < 4; ++v) phi[(size_t)q * 10 + v] = l[v] * (2.0 * l[v] - 1.0);
+            for (int k = 0; k < 6; ++k) phi[(size_t)q * 10 + 4 + k] = 4.0 * l[e[k][0]] * l[e[k][1]];
+        }
+    }
+    std::vector<double> load(3 * (size_t)dm->num_nodes, 0.0);
+    std::vector<char> touched(dm->num_nodes, 0);
+    for (int32_t cell : kp->cells) {
+        const int32_t *cl = dm->closure.data() + (size_t)cell * nb;
+        const int32_t *cn = dm->cone.data() + (size_t)cell * dm->nv;
+        double X[8][3];
+        for (int v = 0; v < dm->nv; ++v)
+            for (int d = 0; d < 3; ++d) X[v][d] = dm->coords[3 * (size_t)cn[v] + d];
+        const double *a = kp->aux_host.data() + (size_t)cell * per * naux_tot;
+        for (int q = 0; q < nq; ++q) {
+            double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (dm->cell_type == PLX_HEX_Q1) {
+                for (int v = 0; v < 8; ++v)
+                    for (int d = 0; d < 3; ++d)
+                        for (int k = 0; k < 3; ++k) J[3 * d + k] += X[v][d] * dq[((size_t)q * 8 + v) * 3 + k];
+            } else {
+                for (int d = 0; d < 3; ++d)
+                    for (int k = 0; k < 3; ++k) J[3 * d + k] = X[k + 1][d] - X[0][d];
+            }
+            const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                               J[2] * (J[3] * J[7] - J[4] * J[6]);
+            const double *aq = a + (per > 1 ? (size_t)q * naux_tot : 0);
+            double f[3] = {0.0, 0.0, 0.0};
+            if (grav)
+                for (int c = 0; c < 3; ++c) f[c] += aq[off[0]] * aq[off[i_grav] + c];
+            if (body)
+                for (int c = 0; c < 3; ++c) f[c] += aq[off[i_body] + c];
+            const double wd = w[q] * det;
+            for (int b = 0; b < nb; ++b)
+                for (int c = 0; c < 3; ++c) load[3 * (size_t)cl[b] + c] += wd * phi[(size_t)q * nb + b] * f[c];
+        }
+        for (int b = 0; b < nb; ++b) touched[cl[b]] = 1;
+    }
+    std::vector<int32_t> nodes;
+    for (int n = 0; n < dm->num_nodes; ++n)
+        if (touched[n]) nodes.push_back(n);
+    std::vector<double> v64(3 * nodes.size());
+    std::vector<float> v32(3 * nodes.size());
+    for (size_t i = 0; i < nodes.size(); ++i)
+        for (int c = 0; c < 3; ++c) {
+            v64[3 * i + c] = load[3 * (size_t)nodes[i] + c];
+            v32[3 * i + c] = (float)v64[3 * i + c];
+        }
+    int err = 0;
+    if ((err = upload(dm, &kp->d_load_node, nodes.data(), nodes.size()))) return err;
+    if ((err = upload(dm, &kp->d_load64, v64.data(), v64.size()))) return err;
+    if ((err = upload(dm, &kp->d_load32, v32.data(), v32.size()))) return err;
+    kp->num_load = (int)nodes.size();
+    return 0;
+}
+
 static int repack_aux(plx_dm dm, KeyPlan *kp) {
     const int num_aux = kp->num_aux;
     std::vector<int> off(num_aux);
@@ -165,7 +276,10 @@ static int repack_aux(plx_dm dm, KeyPlan *kp) {
     const int per = kp->aux_per_qp ? dm->nq : 1;
     const int n = (int)kp->cells.size();
     const bool genmax = kp->f1 == PLX_F1V_GENMAXWELL_3D || kp->jf3 == PLX_JF3VU_GENMAXWELL_3D;
-    if (genmax) return genmax_capture(dm, kp);
+    if (genmax) {
+        const int err = genmax_capture(dm, kp);
+        return err ? err : build_load(dm, kp);
+    }
     kp->genmax = false;
     const int i_mu = num_aux - 2;
     const int i_k = num_aux - 1;
@@ -191,7 +305,7 @@ static int repack_aux(plx_dm dm, KeyPlan *kp) {
     if ((err = upload(dm, &kp->d_mat64, m64.data(), m64.size()))) return err;
     if ((err = upload(dm, &kp->d_mat32, m32.data(), m32.size()))) return err;
     kp->has_aux = true;
-    return 0;
+    return build_load(dm, kp);
 }
 
 static int get_or_create_key(plx_dm dm, int label_value, int part, KeyPlan **out) {
@@ -212,7 +326,8 @@ static int get_or_create_key(plx_dm dm, int label_value, int part, KeyPlan **out
 }
 
 static bool known_residual_pair(int f0, int f1) {
-    return f0 == PLX_FN_NONE && (f1 == PLX_F1V_ISOLINEAR_ELASTICITY_3D || f1 == PLX_F1V_GENMAXWELL_3D);
+    return (f0 == PLX_FN_NONE || f0 == PLX_F0V_BODYFORCE || f0 == PLX_F0V_GRAV || f0 == PLX_F0V_GRAVBODYFORCE) &&
+           (f1 == PLX_F1V_ISOLINEAR_ELASTICITY_3D || f1 == PLX_F1V_GENMAXWELL_3D);
 }
 
 }  // namespace plx
@@ -369,7 +484,7 @@ int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1) {
     if (err) return err;
     kp->f0 = f0;
     kp->f1 = f1;
-    return 0;
+    return build_load(dm, kp);
 }
 
 int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, int jf2, int jf3) {
@@ -428,7 +543,11 @@ int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, 
         return set_error(dm, 1, "residual kernel not supported by this entry point");
     if ((kp->f1 == PLX_F1V_GENMAXWELL_3D) != kp->genmax)
         return set_error(dm, 1, "residual kernel and auxiliary layout disagree");
-    return launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
+    if (kp->f0 != PLX_FN_NONE && kp->num_load == 0)
+        return set_error(dm, 1, "body force / gravity: the auxiliary layout does not match the f0 kernel");
+    const int err = launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
+    if (err || kp->num_load == 0) return err;
+    return launch_add_load(dm, kp, scalar_type, locF);
 }
 
 int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar_type, double t,

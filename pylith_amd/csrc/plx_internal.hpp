@@ -81,6 +81,12 @@ struct KeyPlan {
     double *d_prestress = nullptr;   // [6][8][stride]
     std::vector<double> aux_host;    // captured aux (original layout) for state kernels
     std::vector<double> constants;
+    // f0 load term (body force / gravity: independent of the solution), added to the residual
+    // after the cell pass: the key's closure nodes and their values (FP64 and FP32 copies)
+    int num_load = 0;
+    int32_t *d_load_node = nullptr;
+    double *d_load64 = nullptr;
+    float *d_load32 = nullptr;
 };
 
 struct EventLog {
@@ -184,6 +190,7 @@ void events_release(plx_dm dm);
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
                  void *r, int mode, double scale = 1.0);
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
+int launch_add_load(plx_dm dm, KeyPlan *kp, int scalar_type, void *r);
 int ensure_mesh_indices(plx_dm dm);  // uploads d_closure and d_cone (each if absent)
 // Generalized Maxwell state (plx_genmax.hip).
 int launch_genmax_prestress(plx_dm dm, KeyPlan *kp);

@@ -1,0 +1,142 @@
+"""Body force and gravity on the device (f0 = Elasticity::g0v_bodyforce / g0v_grav /
+g0v_gravbodyforce, fekernels/Elasticity.hh:249-385) against the oracle, through the C ABI.
+
+Bar: FP64 within 1e-12 relative, FP32 within 2e-5; the Jacobian action carries no f0 term; the
+reference's Gravity3D / BodyForce3D MMS cases (TetP2 on its tet.msh) leave the interior rows of
+the device residual at zero.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from pylith_amd import meshgen as mg  # noqa: E402
+from test_oracle_f0 import F0_BODY, F0_GRAV, F0_GRAVBODY, f0_aux, mms_case  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+F1, JF3 = 1, 2
+TOL64, TOL32 = 1e-12, 2e-5
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def mesh_case(name):
+    if name == "q1_nonaffine":
+        m = mg.hex_box((9, 7, 5), (0, 0, 0), (900.0, 700.0, 500.0), ordered=False)
+        rng = np.random.default_rng(11)
+        x = m.coords.copy()
+        inner = np.all((x > 1) & (x < np.array([899, 699, 499])), axis=1)
+        x[inner] += rng.uniform(-25, 25, (inner.sum(), 3))
+        m.coords = m.node_coords = x
+        return m
+    return mg.tet_p2_box(6, (0, 0, 0), (600.0, 600.0, 600.0))
+
+
+def random_f0_aux(m, kind, seed, per_qp=False, nq=1):
+    """Per-cell moduli; density, body force and gravity per cell, or per point when per_qp."""
+    rng = np.random.default_rng(seed)
+    base, _ = mg.elastic_aux(m.num_cells, seed=seed, vs_sigma=0.2, vp_vs=1.732)
+    n = m.num_cells * (nq if per_qp else 1)
+    b = np.repeat(base, nq, 0) if per_qp else base
+    b = b.copy()
+    b[:, 0] = rng.uniform(2000.0, 3000.0, n)
+    body = rng.normal(scale=1e4, size=(n, 3)) if kind in (F0_BODY, F0_GRAVBODY) else None
+    grav = rng.normal(scale=10.0, size=(n, 3)) if kind in (F0_GRAV, F0_GRAVBODY) else None
+    return f0_aux(b, body=body, grav=grav)
+
+
+@pytest.mark.parametrize("kind", [F0_BODY, F0_GRAV, F0_GRAVBODY])
+@pytest.mark.parametrize("mesh_name", ["q1_nonaffine", "p2"])
+def test_residual_with_f0_matches_oracle(oracle, kind, mesh_name):
+    from pylith_amd.dm import DM
+    m = mesh_case(mesh_name)
+    aux, sizes = random_f0_aux(m, kind, seed=kind)
+    u = mg.random_field(m.num_nodes, 3) * 1e-3
+    P = oracle.Problem(m, aux, sizes)
+    ref = P.residual(u, F1, f0=kind)
+    ref_j = P.residual(u, F1)  # J v = F(v) - F(0): no f0 term
+    print(f"[f0] {mesh_name} kind={kind} cells={m.num_cells}", flush=True)
+    for order in ("residual_first", "aux_first"):
+        dm = DM(m)
+        if order == "residual_first":
+            dm.add_residual(0, 0, kind, F1)
+            dm.add_jacobian(0, 0, 0, 0, 0, JF3)
+            dm.set_auxiliary(0, 0, aux, sizes)
+        else:
+            dm.add_jacobian(0, 0, 0, 0, 0, JF3)
+            dm.set_auxiliary(0, 0, aux, sizes)
+            dm.add_residual(0, 0, kind, F1)
+        for dt, tol in ((torch.float64, TOL64), (torch.float32, TOL32)):
+            x = dev(u, dt)
+            r = torch.zeros_like(x)
+            dm.compute_residual(0, 0, x, r, mode=1)
+            torch.cuda.synchronize()
+            assert relerr(r.double().cpu().numpy(), ref) <= tol, (order, dt)
+            r2 = torch.full_like(x, 1.0)
+            dm.compute_residual(0, 0, x, r2, mode=0)
+            torch.cuda.synchronize()
+            assert relerr(r2.double().cpu().numpy() - 1.0, ref) <= 10 * tol, (order, dt)
+            jv = torch.zeros_like(x)
+            dm.compute_jacobian_action(0, 0, x, jv, mode=1)
+            torch.cuda.synchronize()
+            assert relerr(jv.double().cpu().numpy(), ref_j) <= tol, (order, dt)
+        dm.close()
+
+
+def test_body_force_per_quadrature_point(oracle):
+    """Aux given per quadrature point (Q1, 8 points): the body force varies inside a cell."""
+    from pylith_amd.dm import DM
+    m = mesh_case("q1_nonaffine")
+    aux, sizes = random_f0_aux(m, F0_GRAVBODY, seed=4, per_qp=True, nq=8)
+    u = mg.random_field(m.num_nodes, 5) * 1e-3
+    ref = oracle.Problem(m, aux, sizes, per_qp=True).residual(u, F1, f0=F0_GRAVBODY)
+    dm = DM(m)
+    dm.add_residual(0, 0, F0_GRAVBODY, F1)
+    dm.add_jacobian(0, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(0, 0, aux, sizes, per_qp=True)
+    x = dev(u)
+    r = torch.zeros_like(x)
+    dm.compute_residual(0, 0, x, r, mode=1)
+    torch.cuda.synchronize()
+    assert relerr(r.cpu().numpy(), ref) <= TOL64
+    dm.close()
+
+
+@pytest.mark.parametrize("kind", [F0_GRAV, F0_BODY, F0_GRAVBODY])
+def test_mms_interior_rows_vanish_on_device(oracle, kind):
+    from pylith_amd.dm import DM
+    m, (aux, sizes), u, interior = mms_case(kind)
+    P = oracle.Problem(m, aux, sizes)
+    scale = np.abs(P.residual(np.zeros_like(u), 0, f0=kind).reshape(-1, 3)[interior]).max()
+    dm = DM(m)
+    dm.add_residual(24, 0, kind, F1)
+    dm.add_jacobian(24, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(24, 0, aux, sizes)
+    x = dev(u)
+    r = torch.zeros_like(x)
+    dm.compute_residual(24, 0, x, r, mode=1)
+    torch.cuda.synchronize()
+    rr = r.cpu().numpy().reshape(-1, 3)
+    assert np.abs(rr[interior]).max() <= 1e-10 * scale
+    assert relerr(r.cpu().numpy(), P.residual(u, F1, f0=kind)) <= TOL64
+    dm.close()
+
+
+def test_f0_layout_errors():
+    from pylith_amd.dm import DM, PlxError
+    m = mesh_case("p2")
+    base, sizes = mg.elastic_aux(m.num_cells)
+    dm = DM(m)
+    dm.add_residual(0, 0, F0_GRAV, F1)
+    dm.add_jacobian(0, 0, 0, 0, 0, JF3)
+    with pytest.raises(PlxError):
+        dm.set_auxiliary(0, 0, base, sizes)  # no gravity subfield
+    with pytest.raises(PlxError):
+        dm.add_residual(0, 0, 42, F1)  # no device twin
+    dm.close()
