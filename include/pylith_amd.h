@@ -57,7 +57,11 @@ enum plx_pointfn {
      * rheology's subfields (materials/Elasticity.cc:234-241). */
     PLX_F0V_BODYFORCE = 7,                    /* Elasticity::g0v_bodyforce     (Elasticity.hh:300) */
     PLX_F0V_GRAV = 8,                         /* Elasticity::g0v_grav          (Elasticity.hh:249) */
-    PLX_F0V_GRAVBODYFORCE = 9                 /* Elasticity::g0v_gravbodyforce (Elasticity.hh:346) */
+    PLX_F0V_GRAVBODYFORCE = 9,                /* Elasticity::g0v_gravbodyforce (Elasticity.hh:346) */
+    /* f1 with reference stress and strain (IsotropicLinearElasticity::useReferenceState); aux
+     * [..., reference_stress(6), reference_strain(6), shear_modulus, bulk_modulus]; its Jacobian
+     * is PLX_JF3VU_ISOLINEAR_ELASTICITY_3D (materials/IsotropicLinearElasticity.cc:120-131). */
+    PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D = 10 /* IsotropicLinearElasticity3D::f1v_infinitesimalStrain_refState (IsotropicLinearElasticity.hh:848) */
 };
 
 /* ---------------------------------------------------------------------------------------------

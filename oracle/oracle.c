@@ -77,6 +77,41 @@ static void f1v_isolinear_elasticity_3d(int dim, int numS, int numA, const int s
     for (int i = 0; i < 9; ++i) f1[i] -= T[i];
 }
 
+/* IsotropicLinearElasticity3D::f1v_infinitesimalStrain_refState (IsotropicLinearElasticity.hh:848-880)
+ * with cauchyStress_refState = meanStress_refState + deviatoricStress_refState (:204-216, :261-304).
+ * Aux: [..., reference_stress(6), reference_strain(6), shear_modulus, bulk_modulus], vectors in the
+ * order xx, yy, zz, xy, yz, xz (setContext_refState :131-169, TensorOps::_fromVector3D,
+ * Tensor.hh:197-207). */
+static void f1v_isolinear_elasticity_refstate_3d(int dim, int numS, int numA, const int sOff[],
+                                                 const int sOff_x[], const double s[],
+                                                 const double s_t[], const double s_x[],
+                                                 const int aOff[], const int aOff_x[],
+                                                 const double a[], const double a_t[],
+                                                 const double a_x[], double t, const double x[],
+                                                 int numConstants, const double constants[],
+                                                 double f1[]) {
+    (void)dim; (void)numS; (void)sOff; (void)s; (void)s_t; (void)aOff_x; (void)a_t; (void)a_x;
+    (void)t; (void)x; (void)numConstants; (void)constants;
+    const double *refStress = &a[aOff[numA - 4]];
+    const double *refStrain = &a[aOff[numA - 3]];
+    const double mu = a[aOff[numA - 2]];
+    const double K = a[aOff[numA - 1]];
+    double e[6];
+    strain_from_grad(&s_x[sOff_x[0]], e);
+    const double tr = e[0] + e[1] + e[2];
+    const double trRef = refStrain[0] + refStrain[1] + refStrain[2];
+    const double meanRef = (refStress[0] + refStress[1] + refStress[2]) / 3.0;
+    double sig[6] = {0, 0, 0, 0, 0, 0};
+    const double mean = meanRef + K * (tr - trRef);
+    sig[0] += mean; sig[1] += mean; sig[2] += mean;
+    const double traceTerm = -2.0 / 3.0 * mu * (tr - trRef);
+    for (int i = 0; i < 3; ++i) sig[i] += refStress[i] - meanRef + 2.0 * mu * (e[i] - refStrain[i]) + traceTerm;
+    for (int i = 3; i < 6; ++i) sig[i] += refStress[i] + 2.0 * mu * (e[i] - refStrain[i]);
+    double T[9];
+    sym_to_tensor(sig, T);
+    for (int i = 0; i < 9; ++i) f1[i] -= T[i];
+}
+
 /* Fill the 21 nonzeros of -C (isotropic) at index ((f*3+g)*3+df)*3+dg,
  * IsotropicLinearElasticity.hh:1014-1034 / IsotropicLinearGenMaxwell.hh:1284-1304. */
 static void jf3_isotropic(double C1111, double C1122, double C1212, double J[]) {
@@ -252,6 +287,7 @@ static void f0v_gravbodyforce(int dim, int numS, int numA, const int sOff[], con
 
 oracle_point_fn oracle_point_kernel(int id) {
     switch (id) {
+    case ORACLE_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D: return f1v_isolinear_elasticity_refstate_3d;
     case ORACLE_F0V_BODYFORCE: return f0v_bodyforce;
     case ORACLE_F0V_GRAV: return f0v_grav;
     case ORACLE_F0V_GRAVBODYFORCE: return f0v_gravbodyforce;

@@ -68,7 +68,8 @@ enum {
     ORACLE_TOTAL_STRAIN_3D = 6,
     ORACLE_F0V_BODYFORCE = 7,
     ORACLE_F0V_GRAV = 8,
-    ORACLE_F0V_GRAVBODYFORCE = 9
+    ORACLE_F0V_GRAVBODYFORCE = 9,
+    ORACLE_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D = 10
 };
 
 /* Mesh + discretization + auxiliary field of one integrator key. */

@@ -24,6 +24,7 @@ TOTAL_STRAIN_3D = 6
 F0V_BODYFORCE = 7
 F0V_GRAV = 8
 F0V_GRAVBODYFORCE = 9
+F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D = 10
 KSP_CG, KSP_FGMRES_IR, KSP_IR = 0, 1, 2
 ALGO_AUTO, ALGO_QUADRATURE_POINTS, ALGO_FACTORED = 0, 1, 2
 ACCUMULATE_AUTO, ACCUMULATE_COLORED, ACCUMULATE_LDS_ATOMIC = 0, 1, 2

@@ -155,11 +155,16 @@ static int build_plan(plx_dm dm, KeyPlan *kp) {
 // Elastic moduli are the last two subfields (IsotropicLinearElasticity::setContext,
 // IsotropicLinearElasticity.hh:116-117) or, for generalized Maxwell, subfields numA-6 and
 // numA-5 (IsotropicLinearGenMaxwell.hh:133-134); per-qp aux must be uniform in a cell there.
-// Body force / gravity (Elasticity::g0v_bodyforce, g0v_grav, g0v_gravbodyforce,
-// fekernels/Elasticity.hh:249-385).  These f0 read the auxiliary field only, so their part of the
-// residual, sum_q w_q |J_q| phi_b(q) f0(q) per closure node b, is a fixed vector: computed here
-// (host, FP64, in the key's cell order) whenever the kernels or the aux field change, and added
-// after the cell pass of every residual (not of the Jacobian action: d f0 / d u = 0).
+// The residual terms that do not depend on the solution, as one fixed per-node vector:
+//  * body force / gravity (Elasticity::g0v_bodyforce, g0v_grav, g0v_gravbodyforce,
+//    fekernels/Elasticity.hh:249-385): sum_q w_q |J_q| phi_b(q) f0(q);
+//  * the reference state of IsotropicLinearElasticity3D::f1v_infinitesimalStrain_refState
+//    (IsotropicLinearElasticity.hh:848-880, :261-304): sigma = C : eps + (sigma_ref - C : eps_ref),
+//    so f1 = -C : eps (the cell pass) - S with S = sigma_ref - C : eps_ref, whose part is
+//    -sum_q w_q |J_q| S(q) grad phi_b(q).
+// Computed here (host, FP64, in the key's cell order) whenever the kernels or the aux field
+// change, and added after the cell pass of every residual (not of the Jacobian action, which
+// these terms do not enter).
 static int build_load(plx_dm dm, KeyPlan *kp) {
     if (kp->d_load_node) (void)hipFree(kp->d_load_node);
     if (kp->d_load64) (void)hipFree(kp->d_load64);
@@ -168,15 +173,20 @@ static int build_load(plx_dm dm, KeyPlan *kp) {
     kp->d_load64 = nullptr;
     kp->d_load32 = nullptr;
     kp->num_load = 0;
-    if (kp->f0 == PLX_FN_NONE || !kp->has_aux) return 0;
+    const bool refstate = kp->f1 == PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D;
+    if ((kp->f0 == PLX_FN_NONE && !refstate) || !kp->has_aux) return 0;
     // aux subfields the kernel reads (Elasticity.cc:234-241 order)
     const bool grav = kp->f0 == PLX_F0V_GRAV || kp->f0 == PLX_F0V_GRAVBODYFORCE;
     const bool body = kp->f0 == PLX_F0V_BODYFORCE || kp->f0 == PLX_F0V_GRAVBODYFORCE;
     const int i_body = 1, i_grav = kp->f0 == PLX_F0V_GRAVBODYFORCE ? 2 : 1;
     const int nsub = kp->f0 == PLX_F0V_GRAVBODYFORCE ? 3 : 2;
-    if (kp->num_aux < nsub + 2 || kp->aux_sizes[0] != 1 || kp->aux_sizes[1] != 3 ||
-        (nsub == 3 && kp->aux_sizes[2] != 3))
+    if (kp->f0 != PLX_FN_NONE &&
+        (kp->num_aux < nsub + (refstate ? 4 : 2) || kp->aux_sizes[0] != 1 || kp->aux_sizes[1] != 3 ||
+         (nsub == 3 && kp->aux_sizes[2] != 3)))
         return set_error(dm, 1, "body force / gravity need aux subfields density(1), then 3-vectors");
+    const int i_rs = kp->num_aux - 4, i_re = kp->num_aux - 3;
+    if (refstate && (kp->num_aux < 5 || kp->aux_sizes[i_rs] != 6 || kp->aux_sizes[i_re] != 6))
+        return set_error(dm, 1, "reference state needs aux subfields reference_stress(6), reference_strain(6) before the moduli");
     std::vector<int> off(kp->num_aux);
     int naux_tot = 0;
     for (int i = 0; i < kp->num_aux; ++i) {
@@ -210,6 +220,8 @@ static int build_load(plx_dm dm, KeyPlan *kp) {
         for (int q = 0; q < nq; ++q) {
             const double *l = t.lam[q];
             w[q] = t.w[q];
+            for (int b = 0; b < 10; ++b)
+                for (int k = 0; k < 3; ++k) dq[((size_t)q * 10 + b) * 3 + k] = t.dphi[q][b][k];
             for (int v = 0; v < 4; ++v) phi[(size_t)q * 10 + v] = l[v] * (2.0 * l[v] - 1.0);
             for (int k = 0; k < 6; ++k) phi[(size_t)q * 10 + 4 + k] = 4.0 * l[e[k][0]] * l[e[k][1]];
         }
@@ -233,9 +245,28 @@ static int build_load(plx_dm dm, KeyPlan *kp) {
                 for (int d = 0; d < 3; ++d)
                     for (int k = 0; k < 3; ++k) J[3 * d + k] = X[k + 1][d] - X[0][d];
             }
-            const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
-                               J[2] * (J[3] * J[7] - J[4] * J[6]);
+            double Ji[9];
+            const double det = inv3(J, Ji);
             const double *aq = a + (per > 1 ? (size_t)q * naux_tot : 0);
+            if (refstate) {
+                // S = sigma_ref - C : eps_ref (vectors xx, yy, zz, xy, yz, xz)
+                const double *rs = aq + off[i_rs], *re = aq + off[i_re];
+                const double mu = aq[off[kp->num_aux - 2]], K = aq[off[kp->num_aux - 1]];
+                const double tr = re[0] + re[1] + re[2];
+                double S6[6];
+                for (int i = 0; i < 3; ++i) S6[i] = rs[i] - K * tr - 2.0 * mu * (re[i] - tr / 3.0);
+                for (int i = 3; i < 6; ++i) S6[i] = rs[i] - 2.0 * mu * re[i];
+                const double S[3][3] = {{S6[0], S6[3], S6[5]}, {S6[3], S6[1], S6[4]}, {S6[5], S6[4], S6[2]}};
+                const double wd = w[q] * det;
+                for (int b = 0; b < nb; ++b) {
+                    const double *dr = &dq[((size_t)q * nb + b) * 3];
+                    double gp[3];
+                    for (int d = 0; d < 3; ++d) gp[d] = dr[0] * Ji[0 * 3 + d] + dr[1] * Ji[1 * 3 + d] + dr[2] * Ji[2 * 3 + d];
+                    for (int c = 0; c < 3; ++c)
+                        load[3 * (size_t)cl[b] + c] -= wd * (S[c][0] * gp[0] + S[c][1] * gp[1] + S[c][2] * gp[2]);
+                }
+            }
+            if (kp->f0 == PLX_FN_NONE) continue;
             double f[3] = {0.0, 0.0, 0.0};
             if (grav)
                 for (int c = 0; c < 3; ++c) f[c] += aq[off[0]] * aq[off[i_grav] + c];
@@ -327,7 +358,8 @@ static int get_or_create_key(plx_dm dm, int label_value, int part, KeyPlan **out
 
 static bool known_residual_pair(int f0, int f1) {
     return (f0 == PLX_FN_NONE || f0 == PLX_F0V_BODYFORCE || f0 == PLX_F0V_GRAV || f0 == PLX_F0V_GRAVBODYFORCE) &&
-           (f1 == PLX_F1V_ISOLINEAR_ELASTICITY_3D || f1 == PLX_F1V_GENMAXWELL_3D);
+           (f1 == PLX_F1V_ISOLINEAR_ELASTICITY_3D || f1 == PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D ||
+            f1 == PLX_F1V_GENMAXWELL_3D);
 }
 
 }  // namespace plx
@@ -539,12 +571,13 @@ int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, 
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp || !kp->f1) return set_error(dm, 1, "no residual kernels registered for key");
     if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
-    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D && kp->f1 != PLX_F1V_GENMAXWELL_3D)
+    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D && kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D &&
+        kp->f1 != PLX_F1V_GENMAXWELL_3D)
         return set_error(dm, 1, "residual kernel not supported by this entry point");
     if ((kp->f1 == PLX_F1V_GENMAXWELL_3D) != kp->genmax)
         return set_error(dm, 1, "residual kernel and auxiliary layout disagree");
-    if (kp->f0 != PLX_FN_NONE && kp->num_load == 0)
-        return set_error(dm, 1, "body force / gravity: the auxiliary layout does not match the f0 kernel");
+    if ((kp->f0 != PLX_FN_NONE || kp->f1 == PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D) && kp->num_load == 0)
+        return set_error(dm, 1, "body force / gravity / reference state: the auxiliary layout does not match the kernels");
     const int err = launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
     if (err || kp->num_load == 0) return err;
     return launch_add_load(dm, kp, scalar_type, locF);

@@ -1,9 +1,11 @@
 """Body force and gravity on the device (f0 = Elasticity::g0v_bodyforce / g0v_grav /
-g0v_gravbodyforce, fekernels/Elasticity.hh:249-385) against the oracle, through the C ABI.
+g0v_gravbodyforce, fekernels/Elasticity.hh:249-385) and the reference-state stress
+(IsotropicLinearElasticity3D::f1v_infinitesimalStrain_refState, IsotropicLinearElasticity.hh:848)
+against the oracle, through the C ABI.
 
-Bar: FP64 within 1e-12 relative, FP32 within 2e-5; the Jacobian action carries no f0 term; the
-reference's Gravity3D / BodyForce3D MMS cases (TetP2 on its tet.msh) leave the interior rows of
-the device residual at zero.
+Bar: FP64 within 1e-12 relative, FP32 within 2e-5; the Jacobian action carries neither term; the
+reference's Gravity3D / BodyForce3D (TetP2 on its tet.msh) and GravityRefState3D (HexQ1 and
+TetP2) MMS cases leave the interior rows of the device residual at zero.
 """
 import numpy as np
 import pytest
@@ -139,4 +141,68 @@ def test_f0_layout_errors():
         dm.set_auxiliary(0, 0, base, sizes)  # no gravity subfield
     with pytest.raises(PlxError):
         dm.add_residual(0, 0, 42, F1)  # no device twin
+    dm.close()
+
+
+F1_REF = 10
+
+
+@pytest.mark.parametrize("mesh_name", ["q1_nonaffine", "p2"])
+@pytest.mark.parametrize("with_gravity", [False, True])
+def test_reference_state_matches_oracle(oracle, mesh_name, with_gravity):
+    """f1 with reference stress and strain per quadrature point (and gravity): device = oracle;
+    the Jacobian action is the plain elastic one."""
+    from pylith_amd.dm import DM
+    from test_oracle_f0 import refstate_aux
+    m = mesh_case(mesh_name)
+    nq = 8 if m.cell_type == mg.HEX_Q1 else 14
+    rng = np.random.default_rng(21)
+    base, _ = mg.elastic_aux(m.num_cells, seed=3, vs_sigma=0.2, vp_vs=1.732)
+    n = m.num_cells * nq
+    rs = rng.normal(scale=1e6, size=(n, 6))
+    re = rng.normal(scale=1e-4, size=(n, 6))
+    grav = (0.1, -0.2, -9.8) if with_gravity else None
+    aux, sizes = refstate_aux(base, nq, rs, re, grav=grav)
+    f0 = F0_GRAV if with_gravity else 0
+    u = mg.random_field(m.num_nodes, 6) * 1e-3
+    P = oracle.Problem(m, aux, sizes, per_qp=True)
+    ref = P.residual(u, F1_REF, f0=f0)
+    ref_j = oracle.Problem(m, *mg.elastic_aux(m.num_cells, seed=3, vs_sigma=0.2, vp_vs=1.732)).residual(u, F1)
+    print(f"[refstate] {mesh_name} gravity={with_gravity}", flush=True)
+    dm = DM(m)
+    dm.add_residual(0, 0, f0, F1_REF)
+    dm.add_jacobian(0, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(0, 0, aux, sizes, per_qp=True)
+    for dt, tol in ((torch.float64, TOL64), (torch.float32, TOL32)):
+        x = dev(u, dt)
+        r = torch.zeros_like(x)
+        dm.compute_residual(0, 0, x, r, mode=1)
+        torch.cuda.synchronize()
+        assert relerr(r.double().cpu().numpy(), ref) <= tol, dt
+        jv = torch.zeros_like(x)
+        dm.compute_jacobian_action(0, 0, x, jv, mode=1)
+        torch.cuda.synchronize()
+        assert relerr(jv.double().cpu().numpy(), ref_j) <= tol, dt
+    dm.close()
+
+
+@pytest.mark.parametrize("name", ["mms_hex.msh", "mms_tet.msh"])
+def test_gravity_reference_state_mms_on_device(oracle, name):
+    from pylith_amd.dm import DM
+    from test_oracle_f0 import gravity_refstate_case
+    m, aux, sizes, interior = gravity_refstate_case(oracle, name)
+    P = oracle.Problem(m, aux, sizes, per_qp=True)
+    u = np.zeros(3 * m.num_nodes)
+    scale = np.abs(P.residual(u, 0, f0=F0_GRAV).reshape(-1, 3)[interior]).max()
+    dm = DM(m)
+    dm.add_residual(24, 0, F0_GRAV, F1_REF)
+    dm.add_jacobian(24, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(24, 0, aux, sizes, per_qp=True)
+    x = dev(u)
+    r = torch.zeros_like(x)
+    dm.compute_residual(24, 0, x, r, mode=1)
+    torch.cuda.synchronize()
+    rr = r.cpu().numpy()
+    assert np.abs(rr.reshape(-1, 3)[interior]).max() <= 1e-10 * scale
+    assert relerr(rr, P.residual(u, F1_REF, f0=F0_GRAV)) <= TOL64
     dm.close()

@@ -1,6 +1,7 @@
 """Body force and gravity, the f0 of the quasi-static displacement equation
-(Elasticity::g0v_bodyforce / g0v_grav / g0v_gravbodyforce, fekernels/Elasticity.hh:249-385), in
-the oracle.
+(Elasticity::g0v_bodyforce / g0v_grav / g0v_gravbodyforce, fekernels/Elasticity.hh:249-385), and
+the reference-state stress (IsotropicLinearElasticity3D::f1v_infinitesimalStrain_refState,
+IsotropicLinearElasticity.hh:848-880), in the oracle.
 
 Pins:
 * the reference's MMS cases Gravity3D and BodyForce3D (tests/mmstests/linearelasticity/
@@ -10,7 +11,11 @@ Pins:
   factor, those rows do not vanish;
 * closed-form moments of the basis on affine P2 tets (vertex functions integrate to -V/20,
   edge functions to V/5) and the partition of unity on non-affine Q1 hexes (the rows of a
-  constant f0 sum to f0 times the volume).
+  constant f0 sum to f0 times the volume);
+* the reference state: the pointwise stress against sigma_ref + lambda tr(e - e_ref) I +
+  2 mu (e - e_ref) written with lambda (the kernel uses K and the deviator), a consistent
+  reference state reducing to the plain kernel, and the reference's GravityRefState3D MMS case
+  (HexQ1 and TetP2: a lithostatic reference stress balances gravity at u = 0).
 """
 import math
 import os
@@ -151,3 +156,86 @@ def test_q1_partition_of_unity_nonaffine(oracle):
                                      m.node_coords, None, m.labels[c:c + 1])) for c in range(m.num_cells)])
     f = body + base[:, :1] * grav
     np.testing.assert_allclose(r.sum(0), (vols[:, None] * f).sum(0), rtol=1e-12)
+
+
+# --- reference stress / strain (IsotropicLinearElasticity3D::f1v_infinitesimalStrain_refState) ---
+F1_REF = 10
+
+
+def qp_coords(m, oracle):
+    """Physical quadrature points [num_cells, nq, 3] of the oracle's rule (affine P2 geometry from
+    the vertices, trilinear Q1 geometry)."""
+    p, _ = oracle.quadrature(m.cell_type)
+    if m.cell_type == mg.TET_P2:
+        lam = np.stack([1.0 - p.sum(1), p[:, 0], p[:, 1], p[:, 2]], 1)  # [nq, 4]
+        return np.einsum("qv,cvd->cqd", lam, m.coords[m.cone])
+    psi = np.stack([oracle.tabulate(m.cell_type, xi)[0] for xi in p])  # [nq, 8]
+    return np.einsum("qv,cvd->cqd", psi, m.coords[m.closure])
+
+
+def refstate_aux(base, nq, ref_stress, ref_strain, grav=None):
+    """Per quadrature point [density, (gravity), reference_stress(6), reference_strain(6), mu, K]
+    (Elasticity.cc:234-241 with IsotropicLinearElasticity.cc:88-93)."""
+    n = ref_stress.shape[0]
+    b = np.repeat(base, nq, 0)
+    cols, sizes = [b[:, :1]], [1]
+    if grav is not None:
+        cols.append(np.broadcast_to(np.asarray(grav, float), (n, 3)))
+        sizes.append(3)
+    cols += [ref_stress, ref_strain, b[:, 1:3]]
+    sizes += [6, 6, 1, 1]
+    return np.ascontiguousarray(np.concatenate(cols, 1)), np.array(sizes, np.int32)
+
+
+def gravity_refstate_case(oracle, name, zmax=4.0e3):
+    """GravityRefState3D.cc: u = 0, gravity (0, 0, -g), reference stress rho g (z - zmax) on the
+    diagonal (basis order 1: exact at the quadrature points), reference strain 0."""
+    m, groups = meshio_gmsh.read(os.path.join(GOLDEN, name))
+    base, rho, _ = mms_material(m)
+    xq = qp_coords(m, oracle).reshape(-1, 3)
+    nq = oracle.quadrature(m.cell_type)[1].size
+    rs = np.zeros((xq.shape[0], 6))
+    rs[:, :3] = (rho * G_ACC * (xq[:, 2] - zmax))[:, None]
+    aux, sizes = refstate_aux(base, nq, rs, np.zeros_like(rs), grav=(0.0, 0.0, -G_ACC))
+    interior = np.setdiff1d(np.arange(m.num_nodes), groups["boundary"])
+    return m, aux, sizes, interior
+
+
+def test_refstate_point_kernel_closed_form(oracle):
+    """sigma = sigma_ref + lambda tr(e - e_ref) I + 2 mu (e - e_ref); f1 = -sigma."""
+    rng = np.random.default_rng(12)
+    mu, K = 3.0e10, 5.0e10
+    lam = K - 2.0 / 3.0 * mu
+    for _ in range(5):
+        g = rng.normal(size=(3, 3)) * 1e-3
+        rs, re = rng.normal(size=6) * 1e6, rng.normal(size=6) * 1e-4
+        aux = np.concatenate([[2500.0], rs, re, [mu, K]])
+        f1 = oracle.point_f1(F1_REF, g.ravel(), aux, [1, 6, 6, 1, 1])[:9].reshape(3, 3)
+        e = 0.5 * (g + g.T)
+        E = np.array([[re[0], re[3], re[5]], [re[3], re[1], re[4]], [re[5], re[4], re[2]]])
+        S = np.array([[rs[0], rs[3], rs[5]], [rs[3], rs[1], rs[4]], [rs[5], rs[4], rs[2]]])
+        d = e - E
+        sigma = S + lam * np.trace(d) * np.eye(3) + 2.0 * mu * d
+        np.testing.assert_allclose(f1, -sigma, rtol=1e-13, atol=1e-13 * np.abs(sigma).max())
+        # consistent reference state (sigma_ref = C : e_ref): the plain kernel's stress
+        Sc = lam * np.trace(E) * np.eye(3) + 2.0 * mu * E
+        rsc = np.array([Sc[0, 0], Sc[1, 1], Sc[2, 2], Sc[0, 1], Sc[1, 2], Sc[0, 2]])
+        f1c = oracle.point_f1(F1_REF, g.ravel(), np.concatenate([[2500.0], rsc, re, [mu, K]]), [1, 6, 6, 1, 1])
+        f1p = oracle.point_f1(F1, g.ravel(), [2500.0, mu, K], [1, 1, 1])
+        np.testing.assert_allclose(f1c[:9], f1p[:9], rtol=1e-9, atol=1e-9 * np.abs(f1p[:9]).max())
+
+
+@pytest.mark.parametrize("name", ["mms_hex.msh", "mms_tet.msh"])
+def test_gravity_refstate_mms(oracle, name):
+    """GravityRefState3D (HexQ1 and TetP2 cases): the lithostatic reference stress balances
+    gravity at u = 0, so every interior row vanishes; without gravity they do not."""
+    m, aux, sizes, interior = gravity_refstate_case(oracle, name)
+    P = oracle.Problem(m, aux, sizes, per_qp=True)
+    u = np.zeros(3 * m.num_nodes)
+    r = P.residual(u, F1_REF, f0=F0_GRAV).reshape(-1, 3)
+    rg = P.residual(u, 0, f0=F0_GRAV).reshape(-1, 3)
+    scale = np.abs(rg[interior]).max()
+    assert scale > 0.0
+    assert np.abs(r[interior]).max() <= 1e-10 * scale
+    r_nog = P.residual(u, F1_REF).reshape(-1, 3)
+    assert np.abs(r_nog[interior]).max() > 0.5 * scale
