@@ -66,6 +66,11 @@ class DM:
             stream = torch.cuda.current_stream()
         self._check(self.lib.plx_dm_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)))
 
+    def set_stream(self, stream):
+        """Launch on `stream` (a torch.cuda.Stream) from now on, e.g. the capture stream of a
+        HIP graph."""
+        self._check(self.lib.plx_dm_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)))
+
     def _check(self, err):
         if err:
             msg = self.lib.plx_dm_last_error(self.h)

@@ -223,3 +223,44 @@ def test_bad_geometry_rejected():
     m.cone = m.closure.copy()
     with pytest.raises(PlxError):
         DM(m)
+
+
+@pytest.mark.parametrize("ct", ["q1", "p2"])
+def test_graph_captured_applies_match_eager(oracle, ct):
+    """Residual and Jacobian action captured into one HIP graph on the handle's stream replay to
+    the eager bits (colour-ordered accumulation), replay after replay, with eager work between."""
+    from pylith_amd import _lib as L
+    if ct == "q1":
+        m = mg.hex_box((10, 9, 7), (0, 0, 0), (1000.0, 900.0, 700.0))
+    else:
+        m = mg.tet_p2_box(5, (0, 0, 0), (500.0, 500.0, 500.0))
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=2, vs_sigma=0.2, vp_vs=1.732)
+    dm = make_dm(m, aux, sizes)
+    dm.set_kernel_options(L.ALGO_AUTO, L.ACCUMULATE_COLORED, 0)
+    x = dev(mg.random_field(m.num_nodes, 17))
+    r_e, j_e = torch.zeros_like(x), torch.zeros_like(x)
+    dm.compute_residual(0, 0, x, r_e, mode=1)
+    dm.compute_jacobian_action(0, 0, x, j_e, mode=1)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    dm.set_stream(s)
+    r, jv = torch.zeros_like(x), torch.zeros_like(x)
+    with torch.cuda.stream(s):
+        dm.compute_residual(0, 0, x, r, mode=1)
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        dm.compute_residual(0, 0, x, r, mode=1)
+        dm.compute_jacobian_action(0, 0, x, jv, mode=1)
+    for k in range(4):
+        r.fill_(float("nan"))
+        jv.fill_(float("nan"))
+        y = (x * 2.0).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(r, r_e), k
+        assert torch.equal(jv, j_e), k
+        assert torch.isfinite(y)
+    dm.set_stream(torch.cuda.current_stream())
+    dm.close()
