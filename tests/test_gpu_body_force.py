@@ -206,3 +206,32 @@ def test_gravity_reference_state_mms_on_device(oracle, name):
     assert np.abs(rr.reshape(-1, 3)[interior]).max() <= 1e-10 * scale
     assert relerr(rr, P.residual(u, F1_REF, f0=F0_GRAV)) <= TOL64
     dm.close()
+
+
+@pytest.mark.parametrize("kind", [F0_GRAV, F0_BODY, F0_GRAVBODY])
+def test_solve_recovers_mms_solution(kind):
+    """Dirichlet values of the exact MMS displacement on the boundary, right-hand side F(0) (the
+    load): the FP64 CG solve reproduces the quadratic exact solution, which P2 represents."""
+    from pylith_amd import _lib as L
+    from pylith_amd.dm import DM, Solver
+    m, (aux, sizes), u_exact, interior = mms_case(kind)
+    boundary = np.setdiff1d(np.arange(m.num_nodes), interior)
+    dofs = (3 * boundary[:, None] + np.arange(3)).ravel().astype(np.int32)
+    dm = DM(m)
+    dm.add_residual(24, 0, kind, F1)
+    dm.add_jacobian(24, 0, 0, 0, 0, JF3)
+    dm.set_auxiliary(24, 0, aux, sizes)
+    dm.set_constraints(dofs)
+    solver = Solver(dm, label_value=24)
+    zero = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+    b = torch.zeros_like(zero)
+    dm.compute_residual(24, 0, zero, b, mode=1)
+    x = torch.zeros_like(zero)
+    dm.insert_boundary_values(dev(u_exact[dofs]), x)
+    st = solver.solve(x, b=b, method=L.KSP_CG, rtol=1e-13, maxit=20000)
+    torch.cuda.synchronize()
+    assert st["converged"], st
+    err = np.abs(x.cpu().numpy() - u_exact).max() / np.abs(u_exact).max()
+    assert err <= 1e-8, (err, st)
+    solver.close()
+    dm.close()
