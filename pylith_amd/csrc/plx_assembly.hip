@@ -109,7 +109,7 @@ __device__ __forceinline__ int rec_get(const uint4 (&r)[W], int k) {
 template <typename Real, int CT>
 struct Stage {
     using S = Shape<CT>;
-    int tile, c0, nc, n0, nn, v0, nvt, nown;
+    int tile, c0, nc, n0, nn, v0, nvt, nown, hoff;
     Real pu[S::PFN][3];
     double px[CT == PLX_HEX_Q1 ? S::PFN : S::PFV][3];
     uint4 rec[S::CPT][S::RECW];  // packed 16-bit local closure (and cone) of each cell
@@ -130,20 +130,36 @@ __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, in
     T.v0 = P.tile_vert_ptr[tile];
     T.nvt = P.tile_vert_ptr[tile + 1] - T.v0;
     T.nown = P.tile_nown[tile];
+    T.hoff = P.tile_halo_off[tile];
+    if constexpr (CT == PLX_HEX_Q1) {
+        // every node id first, then the values: one wait between the two, not one per slot
+        int gn[S::PFN];
 #pragma unroll
-    for (int j = 0; j < S::PFN; ++j) {
-        const int k = tid + j * S::NT;
-        if (k < T.nn) {
-            const size_t g = (size_t)P.tile_nodes[T.n0 + k];
+        for (int j = 0; j < S::PFN; ++j) {
+            const int k = tid + j * S::NT;
+            gn[j] = k < T.nn ? P.tile_nodes[T.n0 + k] : -1;
+        }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) T.pu[j][c] = u[3 * g + c];
-            if constexpr (CT == PLX_HEX_Q1) {
+        for (int j = 0; j < S::PFN; ++j) {
+            if (gn[j] >= 0) {
+                const size_t g = (size_t)gn[j];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) T.pu[j][c] = u[3 * g + c];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
             }
         }
-    }
-    if constexpr (CT != PLX_HEX_Q1) {
+    } else {
+        // P2: id and values slot by slot (measured faster here than all ids first)
+#pragma unroll
+        for (int j = 0; j < S::PFN; ++j) {
+            const int k = tid + j * S::NT;
+            if (k < T.nn) {
+                const size_t g = (size_t)P.tile_nodes[T.n0 + k];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) T.pu[j][c] = u[3 * g + c];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < S::PFV; ++j) {
             const int k = tid + j * S::NT;
@@ -228,6 +244,19 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
         }
         __syncthreads();
 
+        // node ids of this tile's owned write-out, loaded before the next tile's gathers so that
+        // waiting for them does not wait for those (not for the FP64 Q1 kernel: its registers
+        // are full, and the extra ids spill)
+        constexpr bool EARLY = CT != PLX_HEX_Q1 || sizeof(Real) == 4;
+        constexpr int NW = EARLY ? S::PFN : 1;
+        int wn[NW];
+        if constexpr (EARLY) {
+#pragma unroll
+            for (int j = 0; j < NW; ++j) {
+                const int k = tid + j * NT;
+                wn[j] = k < cur.nown ? A.p.tile_nodes[cur.n0 + k] : 0;
+            }
+        }
         const int next_slot = slot + G;
         const bool has_next = next_slot < A.num_list;
         Stage<Real, CT> nxt;
@@ -315,15 +344,30 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
         }
         __syncthreads();
 
-        for (int k = tid; k < cur.nown; k += NT) {
-            const size_t g = (size_t)A.p.tile_nodes[cur.n0 + k];
+        if constexpr (EARLY) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const Real v = (Real)sR[3 * k + c];
-                r[3 * g + c] = insert ? v : r[3 * g + c] + v;
+            for (int j = 0; j < NW; ++j) {
+                const int k = tid + j * NT;
+                if (k < cur.nown) {
+                    const size_t g = (size_t)wn[j];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const Real v = (Real)sR[3 * k + c];
+                        r[3 * g + c] = insert ? v : r[3 * g + c] + v;
+                    }
+                }
+            }
+        } else {
+            for (int k = tid; k < cur.nown; k += NT) {
+                const size_t g = (size_t)A.p.tile_nodes[cur.n0 + k];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const Real v = (Real)sR[3 * k + c];
+                    r[3 * g + c] = insert ? v : r[3 * g + c] + v;
+                }
             }
         }
-        const size_t hoff = (size_t)A.p.tile_halo_off[cur.tile];
+        const size_t hoff = (size_t)cur.hoff;
         for (int k = cur.nown + tid; k < cur.nn; k += NT) {
             const size_t s = hoff + (size_t)(k - cur.nown);
 #pragma unroll
