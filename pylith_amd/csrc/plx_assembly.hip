@@ -388,14 +388,28 @@ __global__ void __launch_bounds__(256) reduce_halo(int nshared, const int32_t *_
                                                    const Real *__restrict__ halo, Real *r, int mode) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nshared) return;
-    Real a0 = Real(0), a1 = Real(0), a2 = Real(0);
-    for (int j = ptr[s]; j < ptr[s + 1]; ++j) {
-        const size_t h = 3 * (size_t)slot[j];
-        a0 += halo[h + 0];
-        a1 += halo[h + 1];
-        a2 += halo[h + 2];
-    }
+    const int j0 = ptr[s], j1 = ptr[s + 1];
     const size_t g = 3 * (size_t)node[s];
+    Real a0 = Real(0), a1 = Real(0), a2 = Real(0);
+    // slots four at a time: their ids, then their values, then the sums in slot order (the
+    // ids are read at clamped positions inside this node's range; extra lanes are not summed)
+    for (int jb = j0; jb < j1; jb += 4) {
+        int sl[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sl[q] = slot[min(jb + q, j1 - 1)];
+        Real v[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[q][c] = halo[3 * (size_t)sl[q] + c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (jb + q < j1) {
+                a0 += v[q][0];
+                a1 += v[q][1];
+                a2 += v[q][2];
+            }
+    }
     if (mode == PLX_INSERT_VALUES) {
         r[g + 0] = a0;
         r[g + 1] = a1;
