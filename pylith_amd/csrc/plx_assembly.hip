@@ -179,8 +179,9 @@ __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, in
             const uint4 *recs = reinterpret_cast<const uint4 *>(A.records);
 #pragma unroll
             for (int w = 0; w < S::RECW; ++w) T.rec[i][w] = recs[(size_t)pos * S::RECW + w];
-            T.lam[i] = (Real)A.scale * mat[pos];
-            T.mu[i] = (Real)A.scale * mat[P.stride + pos];
+            // unscaled here (scaled where used): the fetch then has no use of these loads to wait for
+            T.lam[i] = mat[pos];
+            T.mu[i] = mat[P.stride + pos];
             if (ACC == kAccColored) T.color[i] = A.color[pos];
         }
     }
@@ -270,7 +271,7 @@ __global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) 
             for (int b = 0; b < NB; ++b) lc[b] = rec_get(cur.rec[i], b);
             Real re[3][NB];
             if (active) {
-                const Real lam = cur.lam[i], mu = cur.mu[i];
+                const Real lam = (Real)A.scale * cur.lam[i], mu = (Real)A.scale * cur.mu[i];
                 if constexpr (CT == PLX_HEX_Q1) {
                     Real rl[3][8];
                     if constexpr (LINEAR) {
