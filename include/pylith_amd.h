@@ -112,7 +112,11 @@ int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs);
 /* ---------------------------------------------------------------------------------------------
  * Weak form: PetscWeakFormAddResidual / PetscWeakFormAddJacobian as called by
  * IntegratorDomain::setKernelsResidual / setKernelsJacobian (IntegratorDomain.cc:145-146,183-184).
- * The key is (label_value, part) of the cells' material label. */
+ * The key is (label_value, part) of the cells' material label.  f0 is PLX_FN_NONE or one of the
+ * PLX_F0V_* body force / gravity kernels; f1 an elastic, reference-state or generalized Maxwell
+ * f1.  Terms that do not depend on the solution (the f0 load, sigma_ref - C : eps_ref) are
+ * integrated once on the host when the kernels or the aux field change and added after the cell
+ * pass of every residual; the Jacobian action does not see them. */
 int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1);
 int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, int jf2, int jf3);
 
