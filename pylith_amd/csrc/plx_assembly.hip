@@ -79,18 +79,24 @@ __device__ __forceinline__ int xcd_tile(int i, int n) {
     return x * per + min(x, rem) + j;
 }
 
-// Workgroup shape per cell type: NT threads, CPT cells per thread, PFN node slots and PFV
-// vertex slots per thread for the register prefetch of the next tile.
-template <int CT>
+// Workgroup shape per cell type and vector type: NT threads, CPT cells per thread, PFN node
+// slots and PFV vertex slots per thread for the register prefetch of the next tile.
+template <int CT, typename Real>
 struct Shape;
-template <>
-struct Shape<PLX_HEX_Q1> {
+template <typename Real>
+struct Shape<PLX_HEX_Q1, Real> {
     // RECW: uint4 words per cell record (8 x uint16 closure)
     static constexpr int NT = 256, CPT = 2, NB = 8, NV = 8, PFN = 3, PFV = 0, RECW = 1;
 };
+// P2 FP64: 3-wave workgroups with two cells per thread, so that two workgroups fit a CU at the
+// kernel's register count (2 waves per SIMD); FP32 (4 waves per SIMD) keeps 6-wave workgroups.
+// record: 10 x uint16 closure, 4 x uint16 cone, 2 padding = 2 uint4 words
 template <>
-struct Shape<PLX_TET_P2> {
-    // record: 10 x uint16 closure, 4 x uint16 cone, 2 padding = 2 uint4 words
+struct Shape<PLX_TET_P2, double> {
+    static constexpr int NT = 192, CPT = 2, NB = 10, NV = 4, PFN = 4, PFV = 1, RECW = 2;
+};
+template <>
+struct Shape<PLX_TET_P2, float> {
     static constexpr int NT = 384, CPT = 1, NB = 10, NV = 4, PFN = 2, PFV = 1, RECW = 2;
 };
 
@@ -108,7 +114,7 @@ __device__ __forceinline__ int rec_get(const uint4 (&r)[W], int k) {
 // tile's arithmetic).
 template <typename Real, int CT>
 struct Stage {
-    using S = Shape<CT>;
+    using S = Shape<CT, Real>;
     int tile, c0, nc, n0, nn, v0, nvt, nown, hoff;
     Real pu[S::PFN][3];
     double px[CT == PLX_HEX_Q1 ? S::PFN : S::PFV][3];
@@ -119,7 +125,7 @@ struct Stage {
 
 template <typename Real, int CT, int ACC>
 __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, int tile, int tid) {
-    using S = Shape<CT>;
+    using S = Shape<CT, Real>;
     const TilePlanDev &P = A.p;
     const Real *u = static_cast<const Real *>(A.u);
     T.tile = tile;
@@ -190,8 +196,8 @@ __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, in
 // Persistent, software-pipelined tile kernel: each workgroup walks its slots; while it works
 // on one tile, the gathers of its next tile are in flight.
 template <typename Real, int CT, bool LINEAR, int ACC, int MINW>
-__global__ void __launch_bounds__(Shape<CT>::NT, MINW) apply_tiles(ApplyArgs A) {
-    using S = Shape<CT>;
+__global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(ApplyArgs A) {
+    using S = Shape<CT, Real>;
     constexpr int NT = S::NT, NB = S::NB, NV = S::NV;
     static_assert(S::PFN * NT >= 768 && S::CPT * NT >= (CT == PLX_HEX_Q1 ? 512 : 384), "tile limits");
     static_assert(CT == PLX_HEX_Q1 || S::PFV * NT >= 192, "vertex limit");
@@ -469,7 +475,7 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     ScopedEvent ev(dm, "plx:apply_tiles");
 #define PLX_LAUNCH_W(CT, LIN, ACC, W, N)                                                             \
     hipLaunchKernelGGL((apply_tiles<Real, CT, LIN, ACC, W>), dim3(std::min(N, resident)),              \
-                       dim3(Shape<CT>::NT), lds, dm->stream, A)
+                       dim3(Shape<CT, Real>::NT), lds, dm->stream, A)
 #define PLX_LAUNCH(CT, LIN, ACC, LIST, N)                                                            \
     do {                                                                                             \
         A.tile_list = LIST;                                                                          \
