@@ -12,6 +12,8 @@
 //      leave as partial sums in the halo buffer, reduced by a node-ordered second kernel.
 // The result is deterministic: no global atomics, fixed order everywhere.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "plx_element.hpp"
 #include "plx_internal.hpp"
@@ -445,6 +447,22 @@ static TilePlanDev plan_dev(plx_dm dm, const KeyPlan *kp) {
     return p;
 }
 
+// Resident workgroups per CU of a persistent tile kernel at this LDS size, from the occupancy
+// calculator (registers, LDS and waves), cached per kernel and size; at most 4.
+static int resident_per_cu(const void *kernel, int nt, size_t lds, int fallback) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, size_t>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(kernel, lds);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, nt, lds) != hipSuccess || n < 1) n = fallback;
+    n = std::min(n, 4);
+    cache[key] = n;
+    return n;
+}
+
 template <typename Real>
 static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode, double scale,
                        const double *prestress) {
@@ -466,16 +484,21 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     const bool atomic = dm->accumulate != PLX_ACCUMULATE_COLORED;
     const size_t lds = (size_t)kp->max_verts * 3 * sizeof(double) +
                        (size_t)kp->max_nodes * 3 * (sizeof(Real) + (atomic ? sizeof(double) : sizeof(Real)));
-    // persistent grid: WG_PER_CU workgroups per CU (a multiple of 8 keeps each workgroup on
-    // the XCD whose tile range it walks), never more than there are tiles
+    // persistent grid: as many workgroups per CU as the kernel's registers and LDS let reside
+    // (a multiple of 8 in all keeps each workgroup on the XCD whose tile range it walks), never
+    // more than there are tiles
     if (dm->num_cu <= 0) PLX_HIP(dm, hipDeviceGetAttribute(&dm->num_cu, hipDeviceAttributeMultiprocessorCount, 0));
-    const int ncu = dm->num_cu;
-    const int resident = ((std::max(ncu, 8) * dm->wg_per_cu + 7) / 8) * 8;
+    const int ncu = std::max(dm->num_cu, 8);
     {
     ScopedEvent ev(dm, "plx:apply_tiles");
 #define PLX_LAUNCH_W(CT, LIN, ACC, W, N)                                                             \
-    hipLaunchKernelGGL((apply_tiles<Real, CT, LIN, ACC, W>), dim3(std::min(N, resident)),              \
-                       dim3(Shape<CT, Real>::NT), lds, dm->stream, A)
+    do {                                                                                             \
+        const void *kfn = reinterpret_cast<const void *>(&apply_tiles<Real, CT, LIN, ACC, W>);        \
+        const int per_cu = resident_per_cu(kfn, Shape<CT, Real>::NT, lds, dm->wg_per_cu);            \
+        const int resident = ((ncu * per_cu + 7) / 8) * 8;                                           \
+        hipLaunchKernelGGL((apply_tiles<Real, CT, LIN, ACC, W>), dim3(std::min(N, resident)),          \
+                           dim3(Shape<CT, Real>::NT), lds, dm->stream, A);                           \
+    } while (0)
 #define PLX_LAUNCH(CT, LIN, ACC, LIST, N)                                                            \
     do {                                                                                             \
         A.tile_list = LIST;                                                                          \
