@@ -296,7 +296,12 @@ int plx_gf_solve(plx_gf gf, int num_impulses, const double *slip, double *u, dou
 /* ---------------------------------------------------------------------------------------------
  * Event timers (the device counterpart of pylith::petsc::EventLogger, petsc/EventLogger.icc:23-47):
  * when on, every kernel launch of the handle is bracketed by hipEvents on its stream.
- * Events: "plx:apply_tiles", "plx:reduce_halo".  get_event_stats synchronises the stream. */
+ * Events: the kernels "plx:apply_tiles", "plx:reduce_halo", and the reference's integrator
+ * events around whole entry points (IntegratorDomain.cc:77-85): "PL:IntegratorDomain:
+ * computeLHSResidual" / "computeRHSResidual" (plx_compute_residual by part),
+ * "PL:IntegratorDomain:computeLHSJacobian" (plx_compute_jacobian) and
+ * "PL:IntegratorDomain:updateStateVars" (plx_update_state_vars).  get_event_stats synchronises
+ * the stream. */
 int plx_dm_set_event_logging(plx_dm dm, int on);
 int plx_dm_get_event_stats(plx_dm dm, const char *name, int *count, double *total_ms);
 int plx_dm_reset_event_stats(plx_dm dm);

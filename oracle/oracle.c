@@ -138,8 +138,15 @@ static void jf3vu_isolinear_elasticity_3d(int dim, int numS, int numA, const int
     jf3_isotropic(lambda2mu, lambda, mu, J);
 }
 
+/* Test hook: a known mistake planted in the viscous-strain update, so that the tests can show
+ * their pins catch it (0: none; 1: the coefficient without its tau/dt factor; 2: dev e in place
+ * of dev e - dev e_t).  Only tests set it, and they reset it to 0. */
+static int g_plant_maxwell = 0;
+void oracle_plant_maxwell_error(int kind) { g_plant_maxwell = kind; }
+
 /* IsotropicLinearMaxwell::viscousStrainCoeff (IsotropicLinearMaxwell.hh:220-223). */
 static double viscous_strain_coeff(double dt, double tau) {
+    if (g_plant_maxwell == 1) return 1.0 - exp(-dt / tau);
     return tau * (1.0 - exp(-dt / tau)) / dt;
 }
 
@@ -149,6 +156,8 @@ static void maxwell_viscous_strain(double tau, const double prev[6], const doubl
     double de[6], dtot[6];
     deviatoric(e, de);
     deviatoric(total, dtot);
+    if (g_plant_maxwell == 2)
+        for (int i = 0; i < 6; ++i) dtot[i] = 0.0;
     const double dq = viscous_strain_coeff(dt, tau);
     const double expFac = exp(-dt / tau);
     for (int i = 0; i < 6; ++i) out[i] = expFac * prev[i] + dq * (de[i] - dtot[i]);

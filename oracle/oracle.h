@@ -87,6 +87,9 @@ typedef struct {
     const double *constants;
 } oracle_mesh;
 
+/* Test hook: plant a known mistake in the viscous-strain update (0 resets; see oracle.c). */
+void oracle_plant_maxwell_error(int kind);
+
 int oracle_nodes_per_cell(int cell_type);
 int oracle_verts_per_cell(int cell_type);
 int oracle_num_quad(int cell_type);

@@ -46,6 +46,12 @@ def lib():
     return _lib
 
 
+def plant_maxwell_error(kind):
+    """Test hook: 1 drops tau/dt from the viscous-strain coefficient, 2 uses dev(e) in place of
+    dev(e) - dev(e_t) in the update, 0 restores the reference update."""
+    lib().oracle_plant_maxwell_error(ctypes.c_int(kind))
+
+
 def _p(a, kind):
     return a.ctypes.data_as(_i32p if kind == "i" else _dblp)
 

@@ -1,16 +1,18 @@
 // plx_assembly.hip -- gfx950 kernels of the residual / Jacobian-action apply.
 //
-// One workgroup per tile of cells, one cell per thread:
+// One persistent workgroup walks a sequence of tiles of cells (one or two cells per thread):
 //   1. the tile's nodal field and vertex coordinates are staged in LDS through the tile's
-//      node / vertex lists (coalesced where node numbers run contiguously);
-//   2. each thread evaluates its element vector (Q1 hex: sum-factorized 2x2x2 Gauss;
-//      P2 tet: 14-point rule) calling the device twin of the pointwise kernel per
-//      quadrature point;
-//   3. element vectors are accumulated into an LDS node buffer, colour by colour, so the
-//      summation order is fixed;
-//   4. nodes owned by the tile are written straight to the output vector; shared nodes
-//      leave as partial sums in the halo buffer, reduced by a node-ordered second kernel.
-// The result is deterministic: no global atomics, fixed order everywhere.
+//      node / vertex lists (coalesced where node numbers run contiguously); the next tile's
+//      gathers are issued into registers before the current tile's arithmetic;
+//   2. each thread evaluates its element vectors (Q1 hex: sum-factorized 2x2x2 Gauss or the
+//      modal form on affine cells; P2 tet: 14-point rule or its P1-moment form) from the
+//      device twin of the pointwise kernel;
+//   3. element vectors are accumulated into an LDS node buffer either colour by colour (fixed
+//      summation order, bitwise reproducible) or with LDS atomic adds into an FP64 accumulator
+//      (the default; the order of the additions into one node, and so the last bits, may vary);
+//   4. nodes owned by the tile are written straight to the output vector; shared nodes leave
+//      as partial sums in the halo buffer, reduced in slot order by a node-ordered second kernel.
+// No global atomics are used.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -20,22 +22,34 @@
 
 namespace plx {
 
-// Reference tables of the P2 element in constant memory.
+// Reference tables of the P2 element in constant memory (each device holds its own copy).
 __constant__ P2Tables c_p2;
-static bool g_tables_ready = false;
+constexpr int kMaxDevices = 64;
+static bool g_tables_ready[kMaxDevices] = {};
+static std::mutex g_device_mu;
 
 struct DevTab {
     __device__ static double w(int q) { return c_p2.w[q]; }
     __device__ static double dphi(int q, int b, int k) { return c_p2.dphi[q][b][k]; }
 };
 
+// Called with the DM's device current (plx_dm_create).
 static int ensure_tables(plx_dm dm) {
-    if (g_tables_ready) return 0;
+    std::lock_guard<std::mutex> lock(g_device_mu);
+    if (g_tables_ready[dm->device]) return 0;
     P2Tables t;
     p2_fill_tables(&t);
     PLX_HIP(dm, hipMemcpyToSymbol(HIP_SYMBOL(c_p2), &t, sizeof(t)));
-    g_tables_ready = true;
+    g_tables_ready[dm->device] = true;
     return 0;
+}
+
+int device_setup(plx_dm dm) {
+    int dev = 0;
+    PLX_HIP(dm, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(dm, 1, "device index beyond the supported range");
+    dm->device = dev;
+    return ensure_tables(dm);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -465,7 +479,7 @@ static int resident_per_cu(const void *kernel, int nt, size_t lds, int fallback)
 
 template <typename Real>
 static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode, double scale,
-                       const double *prestress) {
+                       const double *prestress, void *halo, bool log) {
     ApplyArgs A;
     A.scale = scale;
     A.prestress = prestress;
@@ -475,7 +489,7 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     A.mat = std::is_same<Real, double>::value ? (const void *)kp->d_mat64 : (const void *)kp->d_mat32;
     A.u = u;
     A.r = r;
-    A.halo = kp->d_halo;
+    A.halo = halo ? halo : kp->d_halo;
     A.mode = mode;
     A.records = kp->d_records;
     // a per-point prestress needs the quadrature-point path
@@ -490,7 +504,7 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     if (dm->num_cu <= 0) PLX_HIP(dm, hipDeviceGetAttribute(&dm->num_cu, hipDeviceAttributeMultiprocessorCount, 0));
     const int ncu = std::max(dm->num_cu, 8);
     {
-    ScopedEvent ev(dm, "plx:apply_tiles");
+    ScopedEvent ev(dm, log ? "plx:apply_tiles" : nullptr);
 #define PLX_LAUNCH_W(CT, LIN, ACC, W, N)                                                             \
     do {                                                                                             \
         const void *kfn = reinterpret_cast<const void *>(&apply_tiles<Real, CT, LIN, ACC, W>);        \
@@ -538,10 +552,10 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     }
     PLX_HIP(dm, hipGetLastError());
     if (kp->num_shared > 0) {
-        ScopedEvent ev(dm, "plx:reduce_halo");
+        ScopedEvent ev(dm, log ? "plx:reduce_halo" : nullptr);
         hipLaunchKernelGGL(reduce_halo<Real>, dim3((kp->num_shared + 255) / 256), dim3(256), 0,
                            dm->stream, kp->num_shared, kp->d_shared_node, kp->d_shared_ptr,
-                           kp->d_shared_slot, static_cast<const Real *>(kp->d_halo),
+                           kp->d_shared_slot, static_cast<const Real *>(A.halo),
                            static_cast<Real *>(r), mode);
         PLX_HIP(dm, hipGetLastError());
     }
@@ -552,14 +566,12 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
 // J v = F(v); for generalized Maxwell F(u) = J u + (the state term), with J the elastic operator
 // at the effective shear modulus (the residual adds the per-point state stress).
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u, void *r,
-                 int mode, double scale) {
-    int err = ensure_tables(dm);
-    if (err) return err;
+                 int mode, double scale, void *halo, bool log) {
     if (mode != PLX_ADD_VALUES && mode != PLX_INSERT_VALUES) return set_error(dm, 1, "bad insert mode");
     const double *pre = (!jacobian && kp->genmax) ? kp->d_prestress : nullptr;
     if (pre && scalar_type != PLX_FP64) return set_error(dm, 1, "the viscoelastic residual is FP64");
-    if (scalar_type == PLX_FP64) return apply_typed<double>(dm, kp, u, r, mode, scale, pre);
-    if (scalar_type == PLX_FP32) return apply_typed<float>(dm, kp, u, r, mode, scale, pre);
+    if (scalar_type == PLX_FP64) return apply_typed<double>(dm, kp, u, r, mode, scale, pre, halo, log);
+    if (scalar_type == PLX_FP32) return apply_typed<float>(dm, kp, u, r, mode, scale, pre, halo, log);
     return set_error(dm, 1, "bad scalar type");
 }
 
@@ -679,8 +691,8 @@ int ensure_mesh_indices(plx_dm dm) {
 }
 
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks) {
-    int err = ensure_tables(dm);
-    if (err || (err = ensure_mesh_indices(dm))) return err;
+    const int err = ensure_mesh_indices(dm);
+    if (err) return err;
     const int n = (int)kp->cells.size();
     const TilePlanDev P = plan_dev(dm, kp);
     if (dm->cell_type == PLX_HEX_Q1)
@@ -752,3 +764,4 @@ int validate_geometry(plx_dm dm, int *num_bad) {
 }
 
 }  // namespace plx
+

@@ -356,6 +356,22 @@ static int get_or_create_key(plx_dm dm, int label_value, int part, KeyPlan **out
     return 0;
 }
 
+int residual_key(plx_dm dm, int label_value, int part, const void *locX, void *locF, KeyPlan **out) {
+    if (!dm || !locX || !locF) return set_error(dm, 1, "null vector");
+    KeyPlan *kp = find_key(dm, label_value, part);
+    if (!kp || !kp->f1) return set_error(dm, 1, "no residual kernels registered for key");
+    if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
+    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D && kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D &&
+        kp->f1 != PLX_F1V_GENMAXWELL_3D)
+        return set_error(dm, 1, "residual kernel not supported by this entry point");
+    if ((kp->f1 == PLX_F1V_GENMAXWELL_3D) != kp->genmax)
+        return set_error(dm, 1, "residual kernel and auxiliary layout disagree");
+    if ((kp->f0 != PLX_FN_NONE || kp->f1 == PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D) && kp->num_load == 0)
+        return set_error(dm, 1, "body force / gravity / reference state: the auxiliary layout does not match the kernels");
+    *out = kp;
+    return 0;
+}
+
 static bool known_residual_pair(int f0, int f1) {
     return (f0 == PLX_FN_NONE || f0 == PLX_F0V_BODYFORCE || f0 == PLX_F0V_GRAV || f0 == PLX_F0V_GRAVBODYFORCE) &&
            (f1 == PLX_F1V_ISOLINEAR_ELASTICITY_3D || f1 == PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D ||
@@ -416,6 +432,11 @@ int plx_dm_create(plx_dm *out, int cell_type, int num_cells, int num_vertices, i
         hipMemcpy(dm->d_coords, dm->coords.data(), dm->coords.size() * sizeof(double),
                   hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&dm->d_scalar, 8 * sizeof(double)) != hipSuccess) {
+        plx_dm_destroy(dm);
+        return 76;
+    }
+    // element tables and the scratch address on the DM's device
+    if (device_setup(dm)) {
         plx_dm_destroy(dm);
         return 76;
     }
@@ -483,7 +504,7 @@ int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size) {
 }
 
 int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs) {
-    if (dm) dm->epoch++;
+    if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm || num_constrained < 0 || (num_constrained > 0 && !dofs)) return set_error(dm, 1, "bad constraint arguments");
     std::vector<char> is_con(dm->local_size, 0);
     for (int i = 0; i < num_constrained; ++i) {
@@ -508,7 +529,7 @@ int plx_dm_set_constraints(plx_dm dm, int num_constrained, const int32_t *dofs) 
 }
 
 int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1) {
-    if (dm) dm->epoch++;
+    if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm) return 1;
     if (!known_residual_pair(f0, f1)) return set_error(dm, 1, "no device twin for this residual kernel pair");
     KeyPlan *kp = nullptr;
@@ -520,7 +541,7 @@ int plx_dm_add_residual(plx_dm dm, int label_value, int part, int f0, int f1) {
 }
 
 int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, int jf2, int jf3) {
-    if (dm) dm->epoch++;
+    if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm) return 1;
     if (jf0 || jf1 || jf2 ||
         (jf3 != PLX_JF3VU_ISOLINEAR_ELASTICITY_3D && jf3 != PLX_JF3VU_GENMAXWELL_3D))
@@ -536,7 +557,7 @@ int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, 
 }
 
 int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const double *constants) {
-    if (dm) dm->epoch++;
+    if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm || n < 0 || (n > 0 && !constants)) return 1;
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp) return set_error(dm, 1, "unknown key");
@@ -547,7 +568,7 @@ int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const doub
 int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux, const int32_t *aux_sizes,
                          int aux_per_qp, const double *aux) {
     if (!dm || num_aux < 2 || !aux_sizes || !aux) return set_error(dm, 1, "bad auxiliary arguments");
-    dm->epoch++;
+    dm->epoch++, dm->op_epoch++;
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp) return set_error(dm, 1, "unknown key");
     int naux_tot = 0;
@@ -567,17 +588,10 @@ int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, 
                          const void *locX, const void *locX_t, void *locF, int mode) {
     (void)t;
     (void)locX_t;
-    if (!dm || !locX || !locF) return set_error(dm, 1, "null vector");
-    KeyPlan *kp = find_key(dm, label_value, part);
-    if (!kp || !kp->f1) return set_error(dm, 1, "no residual kernels registered for key");
-    if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
-    if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D && kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D &&
-        kp->f1 != PLX_F1V_GENMAXWELL_3D)
-        return set_error(dm, 1, "residual kernel not supported by this entry point");
-    if ((kp->f1 == PLX_F1V_GENMAXWELL_3D) != kp->genmax)
-        return set_error(dm, 1, "residual kernel and auxiliary layout disagree");
-    if ((kp->f0 != PLX_FN_NONE || kp->f1 == PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D) && kp->num_load == 0)
-        return set_error(dm, 1, "body force / gravity / reference state: the auxiliary layout does not match the kernels");
+    KeyPlan *kp = nullptr;
+    if (const int e = residual_key(dm, label_value, part, locX, locF, &kp)) return e;
+    // the event of the reference's integrator around the same call (IntegratorDomain.cc:80-81)
+    ScopedEvent ev(dm, part == PLX_RHS ? "PL:IntegratorDomain:computeRHSResidual" : "PL:IntegratorDomain:computeLHSResidual");
     const int err = launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
     if (err || kp->num_load == 0) return err;
     return launch_add_load(dm, kp, scalar_type, locF);

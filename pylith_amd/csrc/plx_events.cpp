@@ -10,7 +10,7 @@
 namespace plx {
 
 ScopedEvent::ScopedEvent(plx_dm dm, const char *name) : dm_(dm), name_(name) {
-    if (!dm_->events.on) return;
+    if (!dm_->events.on || !name_) return;
     if (hipEventCreate(&a_) == hipSuccess && hipEventRecord(a_, dm_->stream) == hipSuccess) active_ = true;
 }
 

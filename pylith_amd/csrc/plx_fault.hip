@@ -412,7 +412,7 @@ extern "C" {
 int plx_dm_add_fault(plx_dm dm, int num_cohesive, const int32_t *closure_neg, const int32_t *closure_pos,
                      const int32_t *closure_lag, const int32_t *face_cone, int num_lagrange,
                      const double ref_dir_1[3], const double ref_dir_2[3]) {
-    if (dm) dm->epoch++;
+    if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm || num_cohesive <= 0 || num_lagrange <= 0 || !closure_neg || !closure_pos || !closure_lag ||
         !face_cone || !ref_dir_1 || !ref_dir_2)
         return set_error(dm, 1, "bad fault arguments");

@@ -289,6 +289,7 @@ int plx_update_state_vars(plx_dm dm, int label_value, int part, double t, const 
     if (!kp) return set_error(dm, 1, "unknown key");
     if (!kp->genmax) return 0;  // no state variables
     if (!kp->has_aux) return set_error(dm, 1, "auxiliary field or time step not set");
+    ScopedEvent ev(dm, "PL:IntegratorDomain:updateStateVars");
     return launch_genmax_update(dm, kp, locX);
 }
 

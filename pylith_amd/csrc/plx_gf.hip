@@ -726,6 +726,7 @@ struct plx_gf_s {
     int coarse_m = 0;
     float *coarse_inv = nullptr, *cb = nullptr, *cx = nullptr;
     int amg_kmax = 0;
+    uint64_t op_epoch = 0;  // the DM's operator epoch the assembled copies were made at
 };
 
 namespace plx {
@@ -1195,6 +1196,7 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     g->nrows = nn;
     g->ndof = 3 * (int64_t)nn;
     g->nnzb = (int64_t)col.size();
+    g->op_epoch = dm->op_epoch;
     const int err = gf_setup(g, nc, row_ptr, col, inc_ptr, inc, mcl, diag_pos, conmask, is_pos);
     if (err) {
         plx_gf_destroy(g);
@@ -1320,6 +1322,8 @@ int plx_gf_apply(plx_gf g, int k, int scalar_type, const void *X, void *Y) {
     if (!g || !X || !Y) return 1;
     if (k <= 0 || 256 % k != 0 || k > 64) return set_error(g->dm, 1, "k must divide 256 and be <= 64");
     plx_dm dm = g->dm;
+    if (g->op_epoch != dm->op_epoch)
+        return set_error(dm, 1, "the DM's operator changed after plx_gf_create: destroy and recreate the handle");
     if (scalar_type == PLX_FP64)
         gf_spmm_launch<double>(g, k, g->d_val64, (const double *)X, (double *)Y);
     else
@@ -1375,6 +1379,10 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
     if (!g || num_impulses <= 0 || !slip || !u || !(rtol > 0.0) || maxit <= 0) return 1;
     plx_dm dm = g->dm;
     if (!dm->fault) return set_error(dm, 1, "Green's-function impulses need a fault");
+    // the right-hand sides use the live matrix-free operator, the block PCG the copy assembled at
+    // create: both must be the same operator
+    if (g->op_epoch != dm->op_epoch)
+        return set_error(dm, 1, "the DM's operator changed after plx_gf_create: destroy and recreate the handle");
     FaultData &F = *dm->fault;
     int err = 0;
     if (g->block == 0 && (err = plx_gf_autotune(g, nullptr))) return err;
@@ -1386,8 +1394,10 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
     // after the work already queued on the handle's stream
     hipEvent_t ev;
     PLX_HIP(dm, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    PLX_HIP(dm, hipEventRecord(ev, dm->stream));
-    PLX_HIP(dm, hipStreamWaitEvent(g->stream, ev, 0));
+    if (hipEventRecord(ev, dm->stream) != hipSuccess || hipStreamWaitEvent(g->stream, ev, 0) != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        return set_error(dm, 76, "Green's-function stream ordering failed");
+    }
     hipStream_t saved = dm->stream;
     dm->stream = g->stream;
     const int64_t nloc = dm->local_size, ndof = g->ndof, nl3 = 3 * (int64_t)F.num_lag;
@@ -1421,7 +1431,10 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
             hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->v2, g->B64);
         }
         if (err) break;
-        PLX_HIP(dm, hipGetLastError());
+        if (hipGetLastError() != hipSuccess) {
+            err = set_error(dm, 76, "Green's-function right-hand side launch failed");
+            break;
+        }
         if ((err = gf_block_solve(g, k, rtol, maxit, &st))) break;
         // solutions: u = U w + u_jump (Lagrange multipliers left at zero)
         for (int c = 0; c < kb && !err; ++c) {
@@ -1434,10 +1447,8 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
         st.num_blocks++;
     }
     dm->stream = saved;
-    if (!err) {
-        PLX_HIP(dm, hipEventRecord(ev, g->stream));
-        PLX_HIP(dm, hipStreamWaitEvent(dm->stream, ev, 0));
-    }
+    if (!err && (hipEventRecord(ev, g->stream) != hipSuccess || hipStreamWaitEvent(dm->stream, ev, 0) != hipSuccess))
+        err = set_error(dm, 76, "Green's-function stream ordering failed");
     (void)hipEventDestroy(ev);
     if (err) return err;
     st.converged = st.max_relative_residual <= rtol ? 1 : 0;

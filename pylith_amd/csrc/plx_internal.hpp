@@ -152,9 +152,13 @@ struct plx_dm_s {
     int min_waves = 2;
     int wg_per_cu = 2;               // resident workgroups per CU of the persistent cell kernels
     int num_cu = 0;                  // compute units of the device (queried once)
+    int device = 0;                  // HIP device current at plx_dm_create (the DM's buffers live there)
     // bumped by every call that changes device state or launch choices (buffers, kernels,
     // options, stream): captured graphs recorded at another epoch are stale
     uint64_t epoch = 0;
+    // bumped by every call that changes the operator itself (kernels, aux field, constants,
+    // constraints, fault): an assembled copy of the operator made at another value is stale
+    uint64_t op_epoch = 0;
     std::string last_error;
 };
 
@@ -171,6 +175,8 @@ int set_error(plx_dm dm, int code, const std::string &msg);
     } while (0)
 
 KeyPlan *find_key(plx_dm dm, int label_value, int part);
+// The key of a residual call after the argument checks of plx_compute_residual.
+int residual_key(plx_dm dm, int label_value, int part, const void *locX, void *locF, KeyPlan **out);
 
 // Records a named hipEvent pair around a launch when the handle's event logging is on.
 class ScopedEvent {
@@ -187,8 +193,13 @@ class ScopedEvent {
 void events_release(plx_dm dm);
 
 // Kernel launchers (plx_assembly.hip).
+// halo: the shared-node partial-sum buffer (nullptr: the key's own d_halo); log: record the
+// kernels' event timers when the handle's logging is on
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
-                 void *r, int mode, double scale = 1.0);
+                 void *r, int mode, double scale = 1.0, void *halo = nullptr, bool log = true);
+// Per-device set-up at plx_dm_create with the DM's device current: the element tables in
+// constant memory.
+int device_setup(plx_dm dm);
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
 int launch_add_load(plx_dm dm, KeyPlan *kp, int scalar_type, void *r);
 int ensure_mesh_indices(plx_dm dm);  // uploads d_closure and d_cone (each if absent)

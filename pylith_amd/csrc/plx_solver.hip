@@ -744,13 +744,23 @@ int plx_solver_destroy(plx_solver s) {
 int plx_solver_setup(plx_solver s) {
     if (!s) return 1;
     plx_dm dm = s->dm;
-    double *blocks = nullptr;
-    uint8_t *mask = nullptr;
-    PLX_HIP(dm, hipMalloc(&blocks, 9 * (size_t)s->nnodes * sizeof(double)));
-    PLX_HIP(dm, hipMemsetAsync(blocks, 0, 9 * (size_t)s->nnodes * sizeof(double), dm->stream));
+    // set-up buffers, freed on every exit path
+    struct Scratch {
+        double *blocks = nullptr;
+        uint8_t *mask = nullptr;
+        ~Scratch() {
+            if (blocks) (void)hipFree(blocks);
+            if (mask) (void)hipFree(mask);
+        }
+    } tmp;
+    PLX_HIP(dm, hipMalloc(&tmp.blocks, 9 * (size_t)s->nnodes * sizeof(double)));
+    PLX_HIP(dm, hipMemsetAsync(tmp.blocks, 0, 9 * (size_t)s->nnodes * sizeof(double), dm->stream));
     int err = 0;
     for (KeyPlan *kp : s->keys)
-        if ((err = launch_pbdiag(dm, kp, blocks))) return err;
+        if ((err = launch_pbdiag(dm, kp, tmp.blocks))) {
+            (void)hipStreamSynchronize(dm->stream);
+            return err;
+        }
     std::vector<uint8_t> hmask(s->n, 0);
     for (int32_t d : dm->con.dofs) hmask[d] = 1;
     if (s->fault) {
@@ -760,17 +770,16 @@ int plx_solver_setup(plx_solver s) {
             for (int c = 0; c < 3; ++c) hmask[3 * (size_t)p + c] = 1;
         const int nl = dm->fault->num_lag;
         hipLaunchKernelGGL(merge_blocks, dim3((nl + 255) / 256), dim3(256), 0, dm->stream, nl,
-                           dm->fault->d_neg_node, dm->fault->d_pos_node, blocks);
+                           dm->fault->d_neg_node, dm->fault->d_pos_node, tmp.blocks);
         PLX_HIP(dm, hipGetLastError());
     }
-    PLX_HIP(dm, hipMalloc(&mask, hmask.size()));
-    PLX_HIP(dm, hipMemcpyAsync(mask, hmask.data(), hmask.size(), hipMemcpyHostToDevice, dm->stream));
+    PLX_HIP(dm, hipMalloc(&tmp.mask, hmask.size()));
+    PLX_HIP(dm, hipMemcpyAsync(tmp.mask, hmask.data(), hmask.size(), hipMemcpyHostToDevice, dm->stream));
     hipLaunchKernelGGL(invert_blocks, dim3((s->nnodes + 255) / 256), dim3(256), 0, dm->stream, s->nnodes,
-                       blocks, mask, s->Binv64, s->Binv32);
+                       tmp.blocks, tmp.mask, s->Binv64, s->Binv32);
     PLX_HIP(dm, hipGetLastError());
+    // the host mask and the buffers must outlive the queued copy and kernels
     PLX_HIP(dm, hipStreamSynchronize(dm->stream));
-    (void)hipFree(blocks);
-    (void)hipFree(mask);
     clear_graphs(s);
     if (s->amg) {  // rebuild the hierarchy from the current operator
         plx_gf_destroy(s->amg);

@@ -140,3 +140,62 @@ def test_axial_strain_relaxation_solve(oracle):
         reac = r.cpu().numpy().reshape(-1, 3)[top, 2].sum()
         assert np.isclose(reac, -szz * 8000.0 * 8000.0, rtol=1e-9), step
         dm.update_state_vars(0, 0, x)
+
+
+def test_axial_strain_rate_solve(oracle):
+    """The reference's strain-rate problem (axialstrainrate_genmaxwell_soln.py): top face moved
+    at U0 + V0 t with V0 = 2 m/yr, 20 steps of 0.025 yr from the analytic state at t = 0.  Each
+    step solves on the device, updates the state there, and compares the solution, the sigma_zz
+    reaction and the device's viscous strains with the closed form (eVis of lines 126-128): the
+    increment dq (dev e - dev e_t) is nonzero at every step."""
+    import test_oracle_genmax as ref
+    from pylith_amd.dm import Solver
+    m = mg.hex_box(4, (-4000.0, -4000.0, -8000.0), (4000.0, 4000.0, 0.0), brick=4)
+    mu, K = mg.moduli(ref.RHO, ref.VS, ref.VP)
+    lam = K - 2.0 / 3.0 * mu
+    tau = np.array(ref.ETA) / (mu * np.array(ref.RATIO))
+    eps0, evis0, _, _ = ref.strain_rate_analytic(0.0, mu, lam, tau)
+    aux, sizes = mg.genmax_aux(m.num_cells, 8, ref.RHO, ref.VS, ref.VP, ref.ETA, ref.RATIO,
+                               viscous_strain=np.concatenate(evis0), total_strain=eps0)
+    dm = gm_dm(m, aux, sizes, ref.DT)
+    X = m.node_coords
+    lateral = [(np.isclose(X[:, 0], -4000) | np.isclose(X[:, 0], 4000), 0),
+               (np.isclose(X[:, 1], -4000) | np.isclose(X[:, 1], 4000), 1),
+               (np.isclose(X[:, 2], -8000), 2)]
+    top = np.isclose(X[:, 2], 0.0)
+    fixed = sorted({3 * n + c for sel, c in lateral for n in np.nonzero(sel)[0]})
+    top_dofs = [3 * n + 2 for n in np.nonzero(top)[0]]
+    dofs = np.array(sorted(set(fixed) | set(top_dofs)), np.int32)
+    is_top = np.isin(dofs, top_dofs)
+    dm.set_constraints(dofs)
+    solver = Solver(dm)
+    n = 3 * m.num_nodes
+    worst = {"u": 0.0, "szz": 0.0, "visc": 0.0}
+    for step in range(1, 21):
+        t = step * ref.DT
+        eps, evis, _, szz = ref.strain_rate_analytic(t, mu, lam, tau)
+        vals = np.where(is_top, ref.U0 + ref.V0 * t, 0.0)
+        zero = torch.zeros(n, dtype=torch.float64, device="cuda")
+        b = torch.zeros_like(zero)
+        dm.compute_residual(0, 0, zero, b, mode=L.INSERT_VALUES)  # the state term: F(0)
+        x = torch.zeros_like(zero)
+        dm.insert_boundary_values(dev(vals), x)
+        st = solver.solve(x, b=b, method=L.KSP_CG, rtol=1e-14, maxit=1000)
+        assert st["converged"], (step, st)
+        exact = np.zeros(n)
+        exact[2::3] = eps[2] * (X[:, 2] + 8000.0)
+        worst["u"] = max(worst["u"], np.abs(x.cpu().numpy() - exact).max() / np.abs(exact).max())
+        r = torch.zeros_like(zero)
+        dm.compute_residual(0, 0, x, r, mode=L.INSERT_VALUES)
+        torch.cuda.synchronize()
+        reac = r.cpu().numpy().reshape(-1, 3)[top, 2].sum()
+        worst["szz"] = max(worst["szz"], abs(reac / (-szz * 8000.0 * 8000.0) - 1.0))
+        dm.update_state_vars(0, 0, x)
+        got = dm.get_auxiliary(0, 0, aux)
+        scale = max(np.abs(v).max() for v in evis)
+        for i in range(3):
+            worst["visc"] = max(worst["visc"], np.abs(got[:, 9 + 6 * i:15 + 6 * i] - evis[i]).max() / scale)
+    print(f"[genmax strain rate] worst relative deviations {worst}", flush=True)
+    assert worst["u"] <= 1e-10, worst
+    assert worst["szz"] <= 1e-9, worst
+    assert worst["visc"] <= 1e-9, worst

@@ -118,3 +118,82 @@ def test_viscous_strain_update_closed_form(oracle):
         exp = x * vs0[6 * i:6 * i + 6] + dq * (dev(e) - dev(et0))
         assert np.allclose(new[:, 9 + 6 * i:15 + 6 * i], exp, rtol=1e-10, atol=1e-18)
     assert np.allclose(new[:, 27:33], e, rtol=1e-10, atol=1e-18)
+
+
+# axialstrainrate_genmaxwell_soln.py: the same column and material, U = U0 + V0 t on the top
+# face (V0 = 2 m/yr), 20 steps of dt = 0.025 yr from the state at t = 0
+V0 = 2.0 / mg.YEAR
+
+
+def strain_rate_analytic(t, mu, lam, tau):
+    """The reference's closed-form solution at time t (axialstrainrate_genmaxwell_soln.py:57-139):
+    total strain, the three viscous strains (eVis[:, :, i], lines 126-128) and sigma_xx, sigma_zz
+    (lines 131-143)."""
+    e0, edot0 = U0 / 8000.0, V0 / 8000.0
+    ezz = e0 + edot0 * t
+    eps = np.array([0.0, 0.0, ezz, 0.0, 0.0, 0.0])
+    one = np.array([1, 1, 1, 0, 0, 0.0])
+    edev0 = np.array([0, 0, e0, 0, 0, 0.0]) - one * e0 / 3.0
+    edevdot = np.array([0, 0, edot0, 0, 0, 0.0]) - one * edot0 / 3.0
+    evis = [edev0 * np.exp(-t / tau[i]) + edevdot * tau[i] * (1.0 - np.exp(-t / tau[i])) for i in range(3)]
+    edev = eps - one * ezz / 3.0
+    sdev = 2.0 * mu * ((1.0 - sum(RATIO)) * edev + sum(RATIO[i] * evis[i] for i in range(3)))
+    smean = ezz / 3.0 * (3.0 * lam + 2.0 * mu)
+    return eps, evis, sdev[0] + smean, sdev[2] + smean
+
+
+def run_strain_rate(oracle, steps=20):
+    """Step the oracle through the strain-rate problem at the exact displacement of each step;
+    returns the largest relative deviations from the analytic viscous strains, the top-face and
+    x-face reactions, and the interior residual."""
+    m = mg.hex_box(2, (-4000.0, -4000.0, -8000.0), (4000.0, 4000.0, 0.0), brick=2)
+    mu, K = mg.moduli(RHO, VS, VP)
+    lam = K - 2.0 / 3.0 * mu
+    tau = np.array(ETA) / (mu * np.array(RATIO))
+    eps0, evis0, _, _ = strain_rate_analytic(0.0, mu, lam, tau)
+    aux, sizes = mg.genmax_aux(m.num_cells, 8, RHO, VS, VP, ETA, RATIO,
+                               viscous_strain=np.concatenate(evis0), total_strain=eps0)
+    X = m.node_coords
+    top, xpos = np.isclose(X[:, 2], 0.0), np.isclose(X[:, 0], 4000.0)
+    interior = np.all(np.abs(X[:, :2]) < 3999.0, 1) & (X[:, 2] < -1) & (X[:, 2] > -7999)
+    area = 8000.0 * 8000.0
+    worst = {"visc": 0.0, "szz": 0.0, "sxx": 0.0, "interior": 0.0}
+    for n in range(1, steps + 1):
+        t = n * DT
+        eps, evis, sxx, szz = strain_rate_analytic(t, mu, lam, tau)
+        u = np.zeros(3 * m.num_nodes)
+        u[2::3] = eps[2] * (X[:, 2] + 8000.0)
+        P = oracle.Problem(m, aux, sizes, per_qp=True, constants=[DT])
+        r = P.residual(u, F1_GM).reshape(-1, 3)
+        worst["szz"] = max(worst["szz"], abs(r[top, 2].sum() / (-szz * area) - 1.0))
+        worst["sxx"] = max(worst["sxx"], abs(r[xpos, 0].sum() / (-sxx * area) - 1.0))
+        worst["interior"] = max(worst["interior"], np.abs(r[interior]).max() / (abs(szz) * area))
+        aux = gm_update(P, u)
+        scale = max(np.abs(v).max() for v in evis)
+        for i in range(3):
+            worst["visc"] = max(worst["visc"], np.abs(aux[:, 9 + 6 * i:15 + 6 * i] - evis[i]).max() / scale)
+        assert np.abs(aux[:, 27:33] - eps).max() <= 1e-12 * eps[2]
+    return worst
+
+
+def test_axial_strain_rate_matches_analytic(oracle):
+    """The viscous-strain increment dq (dev e - dev e_t) is nonzero at every step of the
+    strain-rate problem, and with strain linear in time inside a step the discrete update of
+    IsotropicLinearMaxwell.hh:220-223,256-278 reproduces the reference's closed form exactly."""
+    worst = run_strain_rate(oracle)
+    assert worst["visc"] <= 1e-11, worst
+    assert worst["szz"] <= 1e-11 and worst["sxx"] <= 1e-11, worst
+    assert worst["interior"] <= 1e-11, worst
+
+
+def test_strain_rate_pin_catches_planted_update_errors(oracle):
+    """A mistake planted in the oracle's viscous-strain update fails the strain-rate pin: the
+    coefficient without tau/dt, or dev(e) in place of dev(e) - dev(e_t)."""
+    try:
+        for kind in (1, 2):
+            oracle.plant_maxwell_error(kind)
+            worst = run_strain_rate(oracle, steps=4)
+            assert worst["visc"] > 1e-3 and worst["szz"] > 1e-6, (kind, worst)
+    finally:
+        oracle.plant_maxwell_error(0)
+    assert run_strain_rate(oracle, steps=2)["visc"] <= 1e-11
