@@ -154,6 +154,18 @@ int plx_dm_get_auxiliary(plx_dm dm, int label_value, int part, double *aux);
 int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, double t,
                          const void *locX, const void *locX_t, void *locF, int mode);
 
+/* Allocation-free variant of plx_compute_residual.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments, semantics and results
+ * as plx_compute_residual, but the shared-node partial sums of the apply live in a file-scope
+ * device array of the library (64 MiB per device, held until the process ends) instead of the
+ * key's own buffer, and the call allocates nothing and records no event timers.  Fails with an
+ * error when the key needs more than 64 MiB of partial sums (halo slots x 3 x scalar size;
+ * about a 140^3 Q1 mesh in FP64).  All calls on one device share the array: calls that overlap on
+ * one device (several streams or threads) race and must be serialised by the caller. */
+int plx_compute_residual_noalloc(plx_dm dm, int label_value, int part, int scalar_type, double t,
+                                 const void *locX, const void *locX_t, void *locF, int mode);
+
 /* DMPlexComputeJacobianActionByKey(dm, key, cellsIS, t, s_tshift, locX, locX_t, locY, locF)
  * (IntegratorDomain.cc:540-541): locF (+)= J(locX) locY without assembling J. */
 int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar_type,

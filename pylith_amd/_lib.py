@@ -49,6 +49,7 @@ SIGNATURES = {
     "plx_dm_set_auxiliary": (_c_int, [_vp, _c_int, _c_int, _c_int, _i32p, _c_int, _dblp]),
     "plx_dm_set_constants": (_c_int, [_vp, _c_int, _c_int, _c_int, _dblp]),
     "plx_compute_residual": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _c_int]),
+    "plx_compute_residual_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _c_int]),
     "plx_compute_jacobian_action": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp,
                                              _vp, _vp, _c_int]),
     "plx_compute_jacobian_pbdiag": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp]),

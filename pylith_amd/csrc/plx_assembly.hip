@@ -44,11 +44,33 @@ static int ensure_tables(plx_dm dm) {
     return 0;
 }
 
+// --------------------------------------------------------------------------------------------
+// Scratch of the allocation-free residual entry point (plx_compute_residual_noalloc): the
+// shared-node partial sums of one apply live in this file-scope device array instead of the
+// key's own halo buffer.  Each device holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns on that device (the array's
+// name is a host-side stand-in).  Every call on one device shares the array, so calls that
+// overlap on one device race.  The apply writes every halo slot before the reduction reads it,
+// so nothing needs clearing between calls.
+// --------------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static void *g_scratch_ptr[kMaxDevices] = {};
+
 int device_setup(plx_dm dm) {
     int dev = 0;
     PLX_HIP(dm, hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return set_error(dm, 1, "device index beyond the supported range");
     dm->device = dev;
+    {
+        std::lock_guard<std::mutex> lock(g_device_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *p = nullptr;
+            PLX_HIP(dm, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p) return set_error(dm, 76, "scratch array has no device address");
+            g_scratch_ptr[dev] = p;
+        }
+    }
     return ensure_tables(dm);
 }
 
@@ -765,3 +787,27 @@ int validate_geometry(plx_dm dm, int *num_bad) {
 
 }  // namespace plx
 
+using namespace plx;
+
+extern "C" {
+
+int plx_compute_residual_noalloc(plx_dm dm, int label_value, int part, int scalar_type, double t,
+                                 const void *locX, const void *locX_t, void *locF, int mode) {
+    (void)t;
+    (void)locX_t;
+    KeyPlan *kp = nullptr;
+    if (const int e = residual_key(dm, label_value, part, locX, locF, &kp)) return e;
+    if (scalar_type != PLX_FP64 && scalar_type != PLX_FP32) return set_error(dm, 1, "bad scalar type");
+    const size_t need = (size_t)kp->halo_slots * 3 * (scalar_type == PLX_FP64 ? sizeof(double) : sizeof(float));
+    if (need > kScratchBytes)
+        return set_error(dm, 1, "the key's shared-node partial sums (" + std::to_string(need) +
+                                    " bytes) exceed the allocation-free scratch (" +
+                                    std::to_string(kScratchBytes) + " bytes): use plx_compute_residual");
+    void *scr = g_scratch_ptr[dm->device];
+    if (!scr) return set_error(dm, 76, "scratch array not set up on the DM's device");
+    const int err = launch_apply(dm, kp, scalar_type, false, locX, locF, mode, 1.0, scr, false);
+    if (err || kp->num_load == 0) return err;
+    return launch_add_load(dm, kp, scalar_type, locF);
+}
+
+}  // extern "C"

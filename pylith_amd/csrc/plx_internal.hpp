@@ -198,7 +198,7 @@ void events_release(plx_dm dm);
 int launch_apply(plx_dm dm, KeyPlan *kp, int scalar_type, bool jacobian, const void *u,
                  void *r, int mode, double scale = 1.0, void *halo = nullptr, bool log = true);
 // Per-device set-up at plx_dm_create with the DM's device current: the element tables in
-// constant memory.
+// constant memory and the device address of the allocation-free entry point's scratch array.
 int device_setup(plx_dm dm);
 int launch_pbdiag(plx_dm dm, KeyPlan *kp, double *blocks);
 int launch_add_load(plx_dm dm, KeyPlan *kp, int scalar_type, void *r);

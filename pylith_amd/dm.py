@@ -134,6 +134,13 @@ class DM:
         self._check(self.lib.plx_compute_residual(self.h, label_value, part, _stype(locX), t,
                                                   _ptr(locX), _ptr(locX_t), _ptr(locF), mode))
 
+    def compute_residual_noalloc(self, label_value, part, locX, locF, mode=L.ADD_VALUES, t=0.0,
+                                 locX_t=None):
+        """plx_compute_residual_noalloc: the same residual with the library's static scratch
+        array for the shared-node partial sums (allocates nothing)."""
+        self._check(self.lib.plx_compute_residual_noalloc(self.h, label_value, part, _stype(locX), t,
+                                                          _ptr(locX), _ptr(locX_t), _ptr(locF), mode))
+
     def compute_jacobian_action(self, label_value, part, locY, locF, mode=L.ADD_VALUES, t=0.0,
                                 s_tshift=0.0, locX=None, locX_t=None):
         self._check(self.lib.plx_compute_jacobian_action(self.h, label_value, part, _stype(locY),
