@@ -93,6 +93,11 @@ int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size);
  *     structure of the gradient (Q1: the 8 Walsh-Hadamard modes with the Gauss orthogonality
  *     weights; P2: the 4 vertex gradients with the rule's P1 moment matrix); other cells fall
  *     back to quadrature points.  Identical results up to rounding;
+ *   PLX_ALGO_POINTFN: the generic quadrature-point path of include/pylith_amd_pointfn.hpp,
+ *     which calls the device kernels in the PetscPointFn / PetscPointJacFn parameter list (the
+ *     library's own twins of the reference kernels, written in that form; one thread per cell,
+ *     colour-ordered scatter, bitwise reproducible).  Elastic, reference-state and body-force /
+ *     gravity keys; generalized Maxwell keys keep the tile kernels;
  *   PLX_ALGO_AUTO (default): FACTORED where it applies.
  *   PLX_ACCUMULATE_COLORED: element vectors enter the tile's LDS node buffer colour by colour, a
  *     fixed order: results are bitwise reproducible;
@@ -101,7 +106,7 @@ int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size);
  *   PLX_ACCUMULATE_AUTO (default): LDS atomics.
  * min_waves_per_simd (0 = default 2): occupancy the cell kernels' register allocation must
  * allow (2: <= 256 VGPRs, 3: <= 168, 4: <= 128); the rest spills. */
-enum plx_algorithm { PLX_ALGO_AUTO = 0, PLX_ALGO_QUADRATURE_POINTS = 1, PLX_ALGO_FACTORED = 2 };
+enum plx_algorithm { PLX_ALGO_AUTO = 0, PLX_ALGO_QUADRATURE_POINTS = 1, PLX_ALGO_FACTORED = 2, PLX_ALGO_POINTFN = 3 };
 enum plx_accumulate { PLX_ACCUMULATE_AUTO = 0, PLX_ACCUMULATE_COLORED = 1, PLX_ACCUMULATE_LDS_ATOMIC = 2 };
 int plx_dm_set_kernel_options(plx_dm dm, int algorithm, int accumulate, int min_waves_per_simd);
 
@@ -176,6 +181,65 @@ int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar
  * PETSc's vpbjacobi preconditioner that PyLith selects (materials/Elasticity.cc:307-318). */
 int plx_compute_jacobian_pbdiag(plx_dm dm, int label_value, int part, double t,
                                 double s_tshift, const double *locX, double *blocks);
+
+/* ---------------------------------------------------------------------------------------------
+ * Assembled Jacobian: DMPlexComputeJacobianByKey(dm, key, cellsIS, t, s_tshift, locX, locX_t,
+ * J, P) as IntegratorDomain::computeLHSJacobian calls it (IntegratorDomain.cc:458-504, call at
+ * :489-490; Integrator.hh:189-191).
+ * The matrix is the displacement block in 3x3 blocks over the mesh nodes: block (i, j) exists
+ * when nodes i and j share a cell of the mesh (the blocked pattern DMCreateMatrix gives the
+ * displacement field, block size 3).  plx_jacobian_pattern returns it as CSR over the nodes:
+ * row_ptr [num_nodes + 1] and col [nnzb], columns ascending in each row (HOST arrays; pass NULL
+ * to query nnzb only).  plx_compute_jacobian adds (ADD_VALUES) or writes (INSERT_VALUES: all
+ * blocks zeroed first) the key's element matrices
+ *   K_ab[f][g] = sum_q w|J| (phi_a Jf0[f][g] phi_b + phi_a Jf1[f][g][d] dphi_b[d]
+ *                + dphi_a[d] Jf2[f][g][d] phi_b + dphi_a[d] Jf3[f][g][d][e] dphi_b[e])
+ * into values [nnzb][3][3] (DEVICE array, FP64, row-major blocks in pattern order), cell colour
+ * after cell colour (deterministic).  Rows and columns of constrained dofs are included; a
+ * caller inserting through PETSc's local-to-global map drops them (their global index is
+ * negative), as DMPlexComputeJacobian does.  Several keys (materials) add into one matrix. */
+int plx_jacobian_pattern(plx_dm dm, int64_t *nnzb, int32_t *row_ptr, int32_t *col);
+int plx_compute_jacobian(plx_dm dm, int label_value, int part, double t, double s_tshift,
+                         const double *locX, const double *locX_t, double *values, int mode);
+
+/* ---------------------------------------------------------------------------------------------
+ * Pointwise kernels supplied by the caller in the PetscPointFn / PetscPointJacFn parameter list
+ * (docs/developer/implementation/petsc-fem.md:447-490; e.g. IsotropicLinearElasticity.hh:
+ * 804-821).  The caller writes them as __device__ functions and instantiates the launchers of
+ * include/pylith_amd_pointfn.hpp in its own hipcc translation unit; the launchers are
+ * registered on a key here, replacing PetscWeakFormAddResidual / PetscWeakFormAddJacobian
+ * (IntegratorDomain.cc:145-146,183-184) for kernels without a built-in twin.  The DM hands a
+ * launcher everything the kernels see: */
+typedef struct {
+    int cell_type, nodes_per_cell, verts_per_cell, num_quad;
+    int num_cells;                   /* cells of the key */
+    const int32_t *cells;            /* DEVICE [num_cells]: the key's cells, colour after colour */
+    int num_colors;
+    const int32_t *color_ptr;        /* HOST [num_colors + 1]: cells of colour k share no node */
+    const int32_t *closure;          /* DEVICE [num_mesh_cells][nodes_per_cell] */
+    const int32_t *cone;             /* DEVICE [num_mesh_cells][verts_per_cell] */
+    const double *coords;            /* DEVICE [num_vertices][3] */
+    int num_aux;                     /* auxiliary subfields (numA) */
+    int aux_off[16];                 /* aOff: offsets of the subfields in a record */
+    int aux_stride;                  /* doubles per record */
+    int aux_per_qp;                  /* records per cell (0) or per cell and quadrature point (1) */
+    const double *aux;               /* DEVICE, the layout given to plx_dm_set_auxiliary */
+    int num_constants;
+    const double *constants;         /* DEVICE [num_constants] */
+    double t, s_tshift;
+    const double *u, *u_t;           /* DEVICE local vectors (u_t may be NULL: zero) */
+    const double *v;                 /* DEVICE direction of a Jacobian action */
+    double *f;                       /* DEVICE output local vector (residual / action) */
+    int mode;                        /* ADD_VALUES / INSERT_VALUES */
+    const int32_t *row_ptr, *col;    /* DEVICE Jacobian pattern (assembly launchers) */
+    double *values;                  /* DEVICE Jacobian values [nnzb][3][3] */
+} plx_pointfn_args;
+typedef int (*plx_pointfn_launcher)(const plx_pointfn_args *args, void *hip_stream);
+/* residual: f (+)= sum w|J| (f0 . phi + f1 : grad phi).  action: f (+)= J(u) v.  assemble:
+ * values (+)= element matrices.  NULL launchers leave that operation unregistered. */
+int plx_dm_add_residual_pointfn(plx_dm dm, int label_value, int part, plx_pointfn_launcher residual);
+int plx_dm_add_jacobian_pointfn(plx_dm dm, int label_value, int part, plx_pointfn_launcher action,
+                                plx_pointfn_launcher assemble);
 
 /* ---------------------------------------------------------------------------------------------
  * Field scatters (topology/Field.cc:394-422) and Dirichlet insertion

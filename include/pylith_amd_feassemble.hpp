@@ -9,6 +9,8 @@
 // exceptions/error.hh:33-44).  Fields are PETSc-layout local vectors in device memory.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <map>
 #include <stdexcept>
@@ -68,6 +70,36 @@ struct Field {
     double *local = nullptr;  // device, PETSc local layout
     int64_t size = 0;
 };
+
+// The assembled Jacobian of the displacement field (the Mat PyLith's problem creates with
+// DMCreateMatrix): 3x3 blocks over the mesh nodes, the pattern on the host, the values on the
+// device.  createMatrix gives it the DM's pattern and zeroed values.
+struct Matrix {
+    std::vector<int32_t> rowPtr, col;  // CSR over the nodes, columns ascending per row
+    double *values = nullptr;          // device [nnzb][3][3]
+    int64_t nnzb = 0;
+};
+
+inline Matrix createMatrix(plx_dm dm) {
+    Matrix m;
+    if (plx_jacobian_pattern(dm, &m.nnzb, nullptr, nullptr)) throw Error(std::string("createMatrix: ") + plx_dm_last_error(dm));
+    int64_t nrows = 0, nglob = 0;
+    plx_dm_get_sizes(dm, &nrows, &nglob);
+    (void)nglob;
+    m.col.resize((size_t)m.nnzb);
+    // row_ptr holds num_nodes + 1 entries: the displacement dofs are 3 per node
+    m.rowPtr.resize((size_t)(nrows / 3) + 1);
+    if (plx_jacobian_pattern(dm, &m.nnzb, m.rowPtr.data(), m.col.data()))
+        throw Error(std::string("createMatrix: ") + plx_dm_last_error(dm));
+    if (hipMalloc(&m.values, (size_t)m.nnzb * 9 * sizeof(double)) != hipSuccess) throw Error("createMatrix: hipMalloc");
+    if (hipMemset(m.values, 0, (size_t)m.nnzb * 9 * sizeof(double)) != hipSuccess) throw Error("createMatrix: hipMemset");
+    return m;
+}
+
+inline void destroyMatrix(Matrix *m) {
+    if (m && m->values) (void)hipFree(m->values);
+    if (m) m->values = nullptr;
+}
 
 // pylith::feassemble::IntegrationData (IntegrationData.hh:26-38): scalars and fields by key.
 class IntegrationData {
@@ -150,6 +182,26 @@ class IntegratorDomain {
     // IntegratorDomain::computeLHSResidual (IntegratorDomain.cc:421-453)
     void computeLHSResidual(Field *residual, const IntegrationData &data) {
         if (hasLHSResidual_) compute(residual, data, LHS);
+    }
+
+    // IntegratorDomain::computeLHSJacobian (IntegratorDomain.cc:458-504, Integrator.hh:189-191):
+    // adds the material's element matrices into the Jacobian and, when it is another matrix, into
+    // the preconditioner matrix (DMPlexComputeJacobianByKey, :489-490; the caller zeroes both,
+    // as MatZeroEntries before SNES's Jacobian evaluation).
+    void computeLHSJacobian(Matrix *jacobianMat, Matrix *precondMat, const IntegrationData &data) {
+        if (!hasLHSJacobian_) return;
+        if (!jacobianMat) throw Error("computeLHSJacobian: null Jacobian matrix");
+        const Field *solution = data.getField(IntegrationData::solution);
+        const Field *dot = data.hasField(IntegrationData::solution_dot) ? data.getField(IntegrationData::solution_dot) : nullptr;
+        const double t = data.getScalar(IntegrationData::time);
+        const double shift = data.hasScalar(IntegrationData::s_tshift) ? data.getScalar(IntegrationData::s_tshift) : 0.0;
+        Matrix *targets[2] = {jacobianMat, precondMat != jacobianMat ? precondMat : nullptr};
+        for (Matrix *mat : targets) {
+            if (!mat) continue;
+            check(plx_compute_jacobian(dm_, label_, LHS, t, shift, solution->local, dot ? dot->local : nullptr,
+                                       mat->values, PLX_ADD_VALUES),
+                  "computeLHSJacobian");
+        }
     }
 
     // DMPlexComputeJacobianActionByKey (IntegratorDomain.cc:540-541): result += J(solution) v.

@@ -294,8 +294,64 @@ static void f0v_gravbodyforce(int dim, int numS, int numA, const int sOff[], con
                   numConstants, constants, f0);
 }
 
+/* Test-only pointwise kernels (no counterpart in the reference), for the device path's
+ * PetscPointFn-form plugin launchers: they read every input a kernel can see.  Aux [density,
+ * shear_modulus, bulk_modulus], constants [c0, c1, c2]:
+ *   f0 = c1 s + density (c0 x + t);  f1 = -sigma(mu(x), K), mu(x) = mu (1 + c2 sin(x/1000 + z/2000));
+ *   Jf0 = c1 I;  Jf3 = -C(mu(x), K). */
+static double test_shear(const double *a, const int aOff[], int numA, const double x[], const double constants[]) {
+    return a[aOff[numA - 2]] * (1.0 + constants[2] * sin(x[0] / 1000.0 + x[2] / 2000.0));
+}
+
+static void f0_test(int dim, int numS, int numA, const int sOff[], const int sOff_x[], const double s[],
+                    const double s_t[], const double s_x[], const int aOff[], const int aOff_x[],
+                    const double a[], const double a_t[], const double a_x[], double t, const double x[],
+                    int numConstants, const double constants[], double f0[]) {
+    (void)numS; (void)numA; (void)sOff_x; (void)s_t; (void)s_x; (void)aOff_x; (void)a_t; (void)a_x;
+    (void)numConstants;
+    for (int i = 0; i < dim; ++i) f0[i] += constants[1] * s[sOff[0] + i] + a[aOff[0]] * (constants[0] * x[i] + t);
+}
+
+static void f1_test(int dim, int numS, int numA, const int sOff[], const int sOff_x[], const double s[],
+                    const double s_t[], const double s_x[], const int aOff[], const int aOff_x[],
+                    const double a[], const double a_t[], const double a_x[], double t, const double x[],
+                    int numConstants, const double constants[], double f1[]) {
+    (void)dim; (void)numS; (void)sOff; (void)s; (void)s_t; (void)aOff_x; (void)a_t; (void)a_x; (void)t;
+    (void)numConstants;
+    const double mu = test_shear(a, aOff, numA, x, constants), K = a[aOff[numA - 1]];
+    double e[6], sig[6], T[9];
+    strain_from_grad(&s_x[sOff_x[0]], e);
+    const double tr = e[0] + e[1] + e[2];
+    for (int i = 0; i < 3; ++i) sig[i] = K * tr + 2.0 * mu * (e[i] - tr / 3.0);
+    for (int i = 3; i < 6; ++i) sig[i] = 2.0 * mu * e[i];
+    sym_to_tensor(sig, T);
+    for (int i = 0; i < 9; ++i) f1[i] -= T[i];
+}
+
+static void jf0_test(int dim, int numS, int numA, const int sOff[], const int sOff_x[], const double s[],
+                     const double s_t[], const double s_x[], const int aOff[], const int aOff_x[],
+                     const double a[], const double a_t[], const double a_x[], double t, double s_tshift,
+                     const double x[], int numConstants, const double constants[], double J[]) {
+    (void)numS; (void)numA; (void)sOff; (void)sOff_x; (void)s; (void)s_t; (void)s_x; (void)aOff; (void)aOff_x;
+    (void)a; (void)a_t; (void)a_x; (void)t; (void)s_tshift; (void)x; (void)numConstants;
+    for (int i = 0; i < dim; ++i) J[i * dim + i] += constants[1];
+}
+
+static void jf3_test(int dim, int numS, int numA, const int sOff[], const int sOff_x[], const double s[],
+                     const double s_t[], const double s_x[], const int aOff[], const int aOff_x[],
+                     const double a[], const double a_t[], const double a_x[], double t, double s_tshift,
+                     const double x[], int numConstants, const double constants[], double J[]) {
+    (void)dim; (void)numS; (void)sOff; (void)sOff_x; (void)s; (void)s_t; (void)s_x; (void)aOff_x; (void)a_t;
+    (void)a_x; (void)t; (void)s_tshift; (void)numConstants;
+    const double mu = test_shear(a, aOff, numA, x, constants), K = a[aOff[numA - 1]];
+    const double lambda = K - 2.0 / 3.0 * mu;
+    jf3_isotropic(lambda + 2.0 * mu, lambda, mu, J);
+}
+
 oracle_point_fn oracle_point_kernel(int id) {
     switch (id) {
+    case ORACLE_TEST_F0: return f0_test;
+    case ORACLE_TEST_F1: return f1_test;
     case ORACLE_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D: return f1v_isolinear_elasticity_refstate_3d;
     case ORACLE_F0V_BODYFORCE: return f0v_bodyforce;
     case ORACLE_F0V_GRAV: return f0v_grav;
@@ -310,6 +366,8 @@ oracle_point_fn oracle_point_kernel(int id) {
 
 oracle_point_jac_fn oracle_point_jac_kernel(int id) {
     switch (id) {
+    case ORACLE_TEST_JF0: return jf0_test;
+    case ORACLE_TEST_JF3: return jf3_test;
     case ORACLE_JF3VU_ISOLINEAR_ELASTICITY_3D: return jf3vu_isolinear_elasticity_3d;
     case ORACLE_JF3VU_GENMAXWELL_3D: return jf3vu_genmaxwell_3d;
     default: return NULL;

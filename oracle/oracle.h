@@ -69,7 +69,12 @@ enum {
     ORACLE_F0V_BODYFORCE = 7,
     ORACLE_F0V_GRAV = 8,
     ORACLE_F0V_GRAVBODYFORCE = 9,
-    ORACLE_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D = 10
+    ORACLE_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D = 10,
+    /* test-only kernels for the device path's PetscPointFn-form plugin launchers (oracle.c) */
+    ORACLE_TEST_F0 = 90,
+    ORACLE_TEST_F1 = 91,
+    ORACLE_TEST_JF0 = 92,
+    ORACLE_TEST_JF3 = 93
 };
 
 /* Mesh + discretization + auxiliary field of one integrator key. */

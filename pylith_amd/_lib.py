@@ -26,7 +26,7 @@ F0V_GRAV = 8
 F0V_GRAVBODYFORCE = 9
 F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D = 10
 KSP_CG, KSP_FGMRES_IR, KSP_IR = 0, 1, 2
-ALGO_AUTO, ALGO_QUADRATURE_POINTS, ALGO_FACTORED = 0, 1, 2
+ALGO_AUTO, ALGO_QUADRATURE_POINTS, ALGO_FACTORED, ALGO_POINTFN = 0, 1, 2, 3
 ACCUMULATE_AUTO, ACCUMULATE_COLORED, ACCUMULATE_LDS_ATOMIC = 0, 1, 2
 
 _c_int, _c_dbl, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
@@ -80,6 +80,10 @@ SIGNATURES = {
     "plx_gf_probe_times": (_c_int, [_vp, _dblp]),
     "plx_gf_apply": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),
     "plx_gf_solve": (_c_int, [_vp, _c_int, _vp, _vp, _c_dbl, _c_int, _vp]),
+    "plx_jacobian_pattern": (_c_int, [_vp, _i64p, _i32p, _i32p]),
+    "plx_compute_jacobian": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp, _vp, _c_int]),
+    "plx_dm_add_residual_pointfn": (_c_int, [_vp, _c_int, _c_int, _vp]),
+    "plx_dm_add_jacobian_pointfn": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),
     "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
     "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
     "plx_dm_reset_event_stats": (_c_int, [_vp]),

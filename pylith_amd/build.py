@@ -33,6 +33,10 @@ AMG_PROBE_SRC = os.path.join(ROOT, "tests", "cpp", "amg_probe.cpp")
 AMG_PROBE = os.path.join(ROOT, "tests", "cpp", "libamg_probe.so")
 
 
+POINTFN_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "pointfn_test_kernels.hip")
+POINTFN_TEST = os.path.join(ROOT, "tests", "cpp", "libpointfn_test.so")
+
+
 def build(verbose=False):
     cmd = [HIPCC, *COMMON_FLAGS, *DEVICE_FLAGS, "-x", "hip", *sources(), "-o", LIB + ".tmp"]
     if verbose:
@@ -41,7 +45,21 @@ def build(verbose=False):
     os.replace(LIB + ".tmp", LIB)
     build_driver(verbose)
     build_amg_probe(verbose)
+    build_pointfn_test(verbose)
     return LIB
+
+
+def build_pointfn_test(verbose=False):
+    """Test-only pointwise kernels in the PetscPointFn form, in a library of their own that
+    includes only the public header (tests/cpp/pointfn_test_kernels.hip)."""
+    if not os.path.exists(POINTFN_TEST_SRC):
+        return None
+    cmd = [HIPCC, *COMMON_FLAGS, *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"), POINTFN_TEST_SRC,
+           "-o", POINTFN_TEST]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return POINTFN_TEST
 
 
 def build_amg_probe(verbose=False):

@@ -797,6 +797,8 @@ int plx_compute_residual_noalloc(plx_dm dm, int label_value, int part, int scala
     (void)locX_t;
     KeyPlan *kp = nullptr;
     if (const int e = residual_key(dm, label_value, part, locX, locF, &kp)) return e;
+    if (kp->pf_residual || !kp->f1)
+        return set_error(dm, 1, "PetscPointFn-form keys are not served allocation-free: use plx_compute_residual");
     if (scalar_type != PLX_FP64 && scalar_type != PLX_FP32) return set_error(dm, 1, "bad scalar type");
     const size_t need = (size_t)kp->halo_slots * 3 * (scalar_type == PLX_FP64 ? sizeof(double) : sizeof(float));
     if (need > kScratchBytes)

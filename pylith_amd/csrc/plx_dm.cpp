@@ -34,6 +34,7 @@ static int upload(plx_dm dm, T **dst, const T *src, size_t n) {
 }
 
 static void free_plan_arrays(KeyPlan *k) {
+    pointfn_release(nullptr, k);
     void *ptrs[] = {k->d_cells,       k->d_tile_cell_ptr, k->d_tile_node_ptr, k->d_tile_nown,
                     k->d_tile_nodes,  k->d_tile_vert_ptr, k->d_tile_verts,    k->d_tile_halo_off,
                     k->d_lclosure,    k->d_lcone,         k->d_shared_node,   k->d_color,
@@ -57,6 +58,9 @@ static void free_plan_arrays(KeyPlan *k) {
     fresh.aux_per_qp = k->aux_per_qp;
     fresh.aux_host = std::move(k->aux_host);
     fresh.constants = std::move(k->constants);
+    fresh.pf_residual = k->pf_residual;
+    fresh.pf_action = k->pf_action;
+    fresh.pf_assemble = k->pf_assemble;
     *k = std::move(fresh);
 }
 
@@ -306,6 +310,13 @@ static int repack_aux(plx_dm dm, KeyPlan *kp) {
     }
     const int per = kp->aux_per_qp ? dm->nq : 1;
     const int n = (int)kp->cells.size();
+    kp->aux_raw_valid = false;  // the PetscPointFn-form path re-uploads the original layout
+    if (kp->f1 == PLX_FN_NONE && kp->jf3 == PLX_FN_NONE) {
+        // kernels registered only as PetscPointFn-form launchers read the aux field as given
+        kp->genmax = false;
+        kp->has_aux = true;
+        return 0;
+    }
     const bool genmax = kp->f1 == PLX_F1V_GENMAXWELL_3D || kp->jf3 == PLX_JF3VU_GENMAXWELL_3D;
     if (genmax) {
         const int err = genmax_capture(dm, kp);
@@ -359,8 +370,12 @@ static int get_or_create_key(plx_dm dm, int label_value, int part, KeyPlan **out
 int residual_key(plx_dm dm, int label_value, int part, const void *locX, void *locF, KeyPlan **out) {
     if (!dm || !locX || !locF) return set_error(dm, 1, "null vector");
     KeyPlan *kp = find_key(dm, label_value, part);
-    if (!kp || !kp->f1) return set_error(dm, 1, "no residual kernels registered for key");
+    if (!kp || (!kp->f1 && !kp->pf_residual)) return set_error(dm, 1, "no residual kernels registered for key");
     if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
+    if (kp->pf_residual) {
+        *out = kp;
+        return 0;
+    }
     if (kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_3D && kp->f1 != PLX_F1V_ISOLINEAR_ELASTICITY_REFSTATE_3D &&
         kp->f1 != PLX_F1V_GENMAXWELL_3D)
         return set_error(dm, 1, "residual kernel not supported by this entry point");
@@ -464,6 +479,8 @@ int plx_dm_destroy(plx_dm dm) {
     if (dm->d_closure) (void)hipFree(dm->d_closure);
     if (dm->d_cone) (void)hipFree(dm->d_cone);
     if (dm->d_scalar) (void)hipFree(dm->d_scalar);
+    if (dm->d_jac_row_ptr) (void)hipFree(dm->d_jac_row_ptr);
+    if (dm->d_jac_col) (void)hipFree(dm->d_jac_col);
     if (dm->d_partials) (void)hipFree(dm->d_partials);
     if (dm->con.d_dofs) (void)hipFree(dm->con.d_dofs);
     if (dm->con.d_g2l) (void)hipFree(dm->con.d_g2l);
@@ -484,7 +501,7 @@ int plx_dm_set_kernel_options(plx_dm dm, int algorithm, int accumulate, int min_
     if (dm) dm->epoch++;
     if (!dm) return 1;
     if (algorithm != PLX_ALGO_AUTO && algorithm != PLX_ALGO_QUADRATURE_POINTS &&
-        algorithm != PLX_ALGO_FACTORED)
+        algorithm != PLX_ALGO_FACTORED && algorithm != PLX_ALGO_POINTFN)
         return set_error(dm, 1, "unknown algorithm");
     if (accumulate != PLX_ACCUMULATE_AUTO && accumulate != PLX_ACCUMULATE_COLORED &&
         accumulate != PLX_ACCUMULATE_LDS_ATOMIC)
@@ -556,6 +573,28 @@ int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, 
     return 0;
 }
 
+int plx_dm_add_residual_pointfn(plx_dm dm, int label_value, int part, plx_pointfn_launcher residual) {
+    if (dm) dm->epoch++, dm->op_epoch++;
+    if (!dm || !residual) return set_error(dm, 1, "null launcher");
+    KeyPlan *kp = nullptr;
+    const int err = get_or_create_key(dm, label_value, part, &kp);
+    if (err) return err;
+    kp->pf_residual = residual;
+    return 0;
+}
+
+int plx_dm_add_jacobian_pointfn(plx_dm dm, int label_value, int part, plx_pointfn_launcher action,
+                                plx_pointfn_launcher assemble) {
+    if (dm) dm->epoch++, dm->op_epoch++;
+    if (!dm || (!action && !assemble)) return set_error(dm, 1, "null launchers");
+    KeyPlan *kp = nullptr;
+    const int err = get_or_create_key(dm, label_value, part, &kp);
+    if (err) return err;
+    kp->pf_action = action;
+    kp->pf_assemble = assemble;
+    return 0;
+}
+
 int plx_dm_set_constants(plx_dm dm, int label_value, int part, int n, const double *constants) {
     if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm || n < 0 || (n > 0 && !constants)) return 1;
@@ -590,6 +629,11 @@ int plx_compute_residual(plx_dm dm, int label_value, int part, int scalar_type, 
     (void)locX_t;
     KeyPlan *kp = nullptr;
     if (const int e = residual_key(dm, label_value, part, locX, locF, &kp)) return e;
+    if (kp->pf_residual || (dm->algorithm == PLX_ALGO_POINTFN && pointfn_builtin_residual(kp->f0, kp->f1))) {
+        if (scalar_type != PLX_FP64) return set_error(dm, 1, "the PetscPointFn-form path is FP64");
+        ScopedEvent ev(dm, part == PLX_RHS ? "PL:IntegratorDomain:computeRHSResidual" : "PL:IntegratorDomain:computeLHSResidual");
+        return pointfn_residual(dm, kp, t, locX, locX_t, locF, mode);
+    }
     // the event of the reference's integrator around the same call (IntegratorDomain.cc:80-81)
     ScopedEvent ev(dm, part == PLX_RHS ? "PL:IntegratorDomain:computeRHSResidual" : "PL:IntegratorDomain:computeLHSResidual");
     const int err = launch_apply(dm, kp, scalar_type, false, locX, locF, mode);
@@ -606,8 +650,12 @@ int plx_compute_jacobian_action(plx_dm dm, int label_value, int part, int scalar
     (void)locX_t;
     if (!dm || !locY || !locF) return set_error(dm, 1, "null vector");
     KeyPlan *kp = find_key(dm, label_value, part);
-    if (!kp || !kp->jf3) return set_error(dm, 1, "no Jacobian kernels registered for key");
+    if (!kp || (!kp->jf3 && !kp->pf_action)) return set_error(dm, 1, "no Jacobian kernels registered for key");
     if (!kp->has_aux) return set_error(dm, 1, "auxiliary field not set");
+    if (kp->pf_action || (dm->algorithm == PLX_ALGO_POINTFN && kp->jf3 == PLX_JF3VU_ISOLINEAR_ELASTICITY_3D)) {
+        if (scalar_type != PLX_FP64) return set_error(dm, 1, "the PetscPointFn-form path is FP64");
+        return pointfn_action(dm, kp, t, s_tshift, locX, locX_t, locY, locF, mode);
+    }
     if ((kp->jf3 == PLX_JF3VU_GENMAXWELL_3D) != kp->genmax)
         return set_error(dm, 1, "Jacobian kernel and auxiliary layout disagree");
     return launch_apply(dm, kp, scalar_type, true, locY, locF, mode);

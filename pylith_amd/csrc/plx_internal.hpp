@@ -87,6 +87,16 @@ struct KeyPlan {
     int32_t *d_load_node = nullptr;
     double *d_load64 = nullptr;
     float *d_load32 = nullptr;
+    // PetscPointFn-form path: launchers registered by the caller (plx_dm_add_*_pointfn), the
+    // key's cells by global colour (cells of one colour share no node), the aux field in its
+    // original layout and the constants on the device
+    plx_pointfn_launcher pf_residual = nullptr, pf_action = nullptr, pf_assemble = nullptr;
+    std::vector<int32_t> pf_color_ptr;
+    int32_t *d_pf_cells = nullptr;
+    double *d_aux_raw = nullptr;
+    bool aux_raw_valid = false;
+    double *d_constants = nullptr;
+    int num_constants_dev = -1;
 };
 
 struct EventLog {
@@ -143,6 +153,9 @@ struct plx_dm_s {
     std::vector<plx::KeyPlan *> keys;
     plx::Constraints con;
     plx::FaultData *fault = nullptr;
+    // assembled-Jacobian pattern (3x3 blocks over the mesh nodes), built on first use
+    std::vector<int32_t> jac_row_ptr, jac_col;
+    int32_t *d_jac_row_ptr = nullptr, *d_jac_col = nullptr;
     double *d_scalar = nullptr;      // reduction scratch
     double *d_partials = nullptr;
     int partials_len = 0;
@@ -191,6 +204,14 @@ class ScopedEvent {
     bool active_ = false;
 };
 void events_release(plx_dm dm);
+
+// PetscPointFn-form path (plx_pointfn.hip): residual / Jacobian action of a key through the
+// generic quadrature-point launchers, FP64.
+int pointfn_residual(plx_dm dm, KeyPlan *kp, double t, const void *u, const void *u_t, void *f, int mode);
+int pointfn_action(plx_dm dm, KeyPlan *kp, double t, double s_tshift, const void *u, const void *u_t,
+                   const void *v, void *f, int mode);
+bool pointfn_builtin_residual(int f0, int f1);
+void pointfn_release(plx_dm dm, KeyPlan *kp);
 
 // Kernel launchers (plx_assembly.hip).
 // halo: the shared-node partial-sum buffer (nullptr: the key's own d_halo); log: record the

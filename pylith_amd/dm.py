@@ -151,6 +151,33 @@ class DM:
         self._check(self.lib.plx_compute_jacobian_pbdiag(self.h, label_value, part, t, s_tshift,
                                                          _ptr(locX), _ptr(blocks)))
 
+    # --- assembled Jacobian ------------------------------------------------------------------
+    def jacobian_pattern(self):
+        """(row_ptr [num_nodes + 1], col [nnzb]): 3x3 blocks over the mesh nodes (host arrays)."""
+        nnzb = ctypes.c_int64()
+        self._check(self.lib.plx_jacobian_pattern(self.h, ctypes.byref(nnzb), None, None))
+        row_ptr = np.zeros(self.mesh.num_nodes + 1, np.int32)
+        col = np.zeros(max(nnzb.value, 1), np.int32)
+        self._check(self.lib.plx_jacobian_pattern(self.h, ctypes.byref(nnzb), row_ptr.ctypes.data_as(L._i32p),
+                                                  col.ctypes.data_as(L._i32p)))
+        return row_ptr, col[:nnzb.value]
+
+    def compute_jacobian(self, label_value, part, values, mode=L.ADD_VALUES, t=0.0, s_tshift=0.0, locX=None,
+                         locX_t=None):
+        """values: float64 CUDA tensor [nnzb, 3, 3] (the pattern's blocks)."""
+        if values.dtype != torch.float64:
+            raise PlxError("the assembled Jacobian is float64")
+        self._check(self.lib.plx_compute_jacobian(self.h, label_value, part, t, s_tshift, _ptr(locX), _ptr(locX_t),
+                                                  _ptr(values), mode))
+
+    def add_residual_pointfn(self, label_value, part, launcher):
+        """Register a PetscPointFn-form residual launcher (an address from a hipcc-built library)."""
+        self._check(self.lib.plx_dm_add_residual_pointfn(self.h, label_value, part, ctypes.c_void_p(launcher)))
+
+    def add_jacobian_pointfn(self, label_value, part, action, assemble):
+        self._check(self.lib.plx_dm_add_jacobian_pointfn(self.h, label_value, part, ctypes.c_void_p(action),
+                                                         ctypes.c_void_p(assemble)))
+
     # --- fault (cohesive cells, prescribed slip) ----------------------------------------------
     def add_fault(self, fault):
         """Cohesive cells of `fault` (meshgen.Fault layout); the Lagrange-multiplier dofs follow

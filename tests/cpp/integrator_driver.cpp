@@ -5,7 +5,8 @@
 //   in:  int32 cell_type, num_cells, num_vertices, num_nodes, num_aux; int32 closure[num_cells*nb];
 //        double coords[num_vertices*3]; int32 aux_sizes[num_aux]; double aux[num_cells*sum];
 //        double u[3*num_nodes]
-//   out: double residual[3*num_nodes], double jacobian_action[3*num_nodes]
+//   out: double residual[3*num_nodes], double jacobian_action[3*num_nodes],
+//        int64 nnzb, int32 row_ptr[num_nodes+1], int32 col[nnzb], double jacobian[nnzb*9]
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -82,13 +83,24 @@ int main(int argc, char **argv) {
         integrator.computeRHSResidual(&residual, data);  // no RHS kernels: adds nothing
         integrator.computeLHSResidual(&residual, data);
         integrator.computeLHSJacobianAction(&jv, solution, data);
+        // the assembled Jacobian (DMCreateMatrix + IntegratorDomain::computeLHSJacobian), the
+        // preconditioner matrix being the Jacobian itself
+        Matrix jac = createMatrix(dm);
+        integrator.computeLHSJacobian(&jac, &jac, data);
         (void)hipDeviceSynchronize();
+        std::vector<double> jvals((size_t)jac.nnzb * 9);
+        (void)hipMemcpy(jvals.data(), jac.values, jvals.size() * sizeof(double), hipMemcpyDeviceToHost);
+        destroyMatrix(&jac);
         std::vector<double> r(residual.size), j(jv.size);
         (void)hipMemcpy(r.data(), residual.local, r.size() * sizeof(double), hipMemcpyDeviceToHost);
         (void)hipMemcpy(j.data(), jv.local, j.size() * sizeof(double), hipMemcpyDeviceToHost);
         std::ofstream o(argv[2], std::ios::binary);
         o.write(reinterpret_cast<const char *>(r.data()), (std::streamsize)(r.size() * sizeof(double)));
         o.write(reinterpret_cast<const char *>(j.data()), (std::streamsize)(j.size() * sizeof(double)));
+        o.write(reinterpret_cast<const char *>(&jac.nnzb), sizeof(int64_t));
+        o.write(reinterpret_cast<const char *>(jac.rowPtr.data()), (std::streamsize)(jac.rowPtr.size() * sizeof(int32_t)));
+        o.write(reinterpret_cast<const char *>(jac.col.data()), (std::streamsize)(jac.col.size() * sizeof(int32_t)));
+        o.write(reinterpret_cast<const char *>(jvals.data()), (std::streamsize)(jvals.size() * sizeof(double)));
         // a kernel without a device twin is refused with an exception (PylithCallPetsc throws)
         try {
             integrator.setKernelsResidual({ResidualKernels("displacement", LHS, PLX_F1V_ISOLINEAR_ELASTICITY_3D,

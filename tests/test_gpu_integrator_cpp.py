@@ -1,5 +1,6 @@
 """The C++ IntegratorDomain mirror (include/pylith_amd_feassemble.hpp), driven by a C++ program
-the way PyLith's problem layer drives its integrator, against the oracle (FP64, 1e-12)."""
+the way PyLith's problem layer drives its integrator (residual, Jacobian action, assembled
+Jacobian of computeLHSJacobian), against the oracle (FP64, 1e-12)."""
 import os
 import subprocess
 
@@ -31,10 +32,32 @@ def test_cpp_integrator_matches_oracle(oracle, tmp_path):
     res = subprocess.run([DRIVER, str(path_in), str(path_out)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     assert "refused" in res.stdout
-    out = np.fromfile(path_out, np.float64)
+    raw = open(path_out, "rb").read()
+    out = np.frombuffer(raw[:16 * u.size], np.float64)
     r, jv = out[:u.size], out[u.size:]
+    # the assembled Jacobian of IntegratorDomain::computeLHSJacobian
+    off = 16 * u.size
+    nnzb = int(np.frombuffer(raw[off:off + 8], np.int64)[0])
+    off += 8
+    row_ptr = np.frombuffer(raw[off:off + 4 * (m.num_nodes + 1)], np.int32)
+    off += 4 * (m.num_nodes + 1)
+    col = np.frombuffer(raw[off:off + 4 * nnzb], np.int32)
+    off += 4 * nnzb
+    vals = np.frombuffer(raw[off:off + 72 * nnzb], np.float64).reshape(nnzb, 3, 3)
+    n = u.size
+    D = np.zeros((n, n))
+    for i in range(m.num_nodes):
+        for e in range(row_ptr[i], row_ptr[i + 1]):
+            D[3 * i:3 * i + 3, 3 * col[e]:3 * col[e] + 3] += vals[e]
     P = oracle.Problem(m, aux, sizes)
     ref = P.residual(u, F1)
     assert np.abs(r - ref).max() <= 1e-12 * np.abs(ref).max()
     jref = P.jacobian_action(u, JF3)
     assert np.abs(jv - jref).max() <= 1e-12 * np.abs(jref).max()
+    Jd = np.zeros((n, n))
+    e = np.zeros(n)
+    for i in range(n):
+        e[i] = 1.0
+        Jd[:, i] = P.jacobian_action(e, JF3)
+        e[i] = 0.0
+    assert np.abs(D - Jd).max() <= 1e-12 * np.abs(Jd).max()
