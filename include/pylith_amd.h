@@ -369,6 +369,30 @@ int plx_gf_apply(plx_gf gf, int k, int scalar_type, const void *X, void *Y);
 int plx_gf_solve(plx_gf gf, int num_impulses, const double *slip, double *u, double rtol, int maxit,
                  plx_gf_stats *stats);
 
+/* Continuous batching on one handle (the serving loop): k slots (k in {8, 16, 32, 64}) solved as
+ * independent columns of one block.  A slot is loaded with an impulse while the others keep
+ * iterating; each step replays one chunk of the FP32 block PCG for all columns, and the slots
+ * whose inner solve converged take an FP64 refinement step: they retire CONVERGED when
+ * ||b - A x|| <= rtol ||b|| (FP64), FAILED after `maxit` refinement steps (or a non-finite
+ * residual), or restart their inner PCG.  A slot's iterates do not depend on the other columns.
+ *   plx_gf_slots_open: (re)initialise k empty slots;
+ *   plx_gf_slots_load: load n empty slots; slip [n][num_lag][3] (device, fault frame);
+ *   plx_gf_slots_step: one step; state[k] (host, may be NULL) receives the slot states;
+ *   plx_gf_slot_take:  the displacement of a finished slot into u (device local vector, may be
+ *                      NULL) and its statistics; the slot becomes EMPTY.
+ * All synchronise the handle's stream. */
+enum { PLX_SLOT_EMPTY = 0, PLX_SLOT_RUNNING = 1, PLX_SLOT_CONVERGED = 2, PLX_SLOT_FAILED = 3 };
+typedef struct {
+    int converged;
+    int outer_iterations;   /* FP64 refinement steps */
+    int inner_iterations;   /* FP32 PCG iterations */
+    double relative_residual;
+} plx_gf_slot_stats;
+int plx_gf_slots_open(plx_gf gf, int k, double rtol, int maxit);
+int plx_gf_slots_load(plx_gf gf, int n, const int *slot_ids, const double *slip);
+int plx_gf_slots_step(plx_gf gf, int *state);
+int plx_gf_slot_take(plx_gf gf, int slot, double *u, plx_gf_slot_stats *stats);
+
 /* ---------------------------------------------------------------------------------------------
  * Event timers (the device counterpart of pylith::petsc::EventLogger, petsc/EventLogger.icc:23-47):
  * when on, every kernel launch of the handle is bracketed by hipEvents on its stream.

@@ -84,6 +84,10 @@ SIGNATURES = {
     "plx_compute_jacobian": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _vp, _vp, _vp, _c_int]),
     "plx_dm_add_residual_pointfn": (_c_int, [_vp, _c_int, _c_int, _vp]),
     "plx_dm_add_jacobian_pointfn": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),
+    "plx_gf_slots_open": (_c_int, [_vp, _c_int, _c_dbl, _c_int]),
+    "plx_gf_slots_load": (_c_int, [_vp, _c_int, _i32p, _vp]),
+    "plx_gf_slots_step": (_c_int, [_vp, _i32p]),
+    "plx_gf_slot_take": (_c_int, [_vp, _c_int, _vp, _vp]),
     "plx_dm_set_event_logging": (_c_int, [_vp, _c_int]),
     "plx_dm_get_event_stats": (_c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_c_int), _dblp]),
     "plx_dm_reset_event_stats": (_c_int, [_vp]),
@@ -103,6 +107,13 @@ class GfStats(ctypes.Structure):
     _fields_ = [("block", _c_int), ("num_blocks", _c_int), ("outer_iterations", _c_int),
                 ("inner_iterations", _c_int), ("converged", _c_int), ("max_relative_residual", _c_dbl)]
 
+
+class SlotStats(ctypes.Structure):
+    _fields_ = [("converged", _c_int), ("outer_iterations", _c_int), ("inner_iterations", _c_int),
+                ("relative_residual", _c_dbl)]
+
+
+SLOT_EMPTY, SLOT_RUNNING, SLOT_CONVERGED, SLOT_FAILED = 0, 1, 2, 3
 
 _lib = None
 

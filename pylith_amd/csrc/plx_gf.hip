@@ -727,6 +727,18 @@ struct plx_gf_s {
     float *coarse_inv = nullptr, *cb = nullptr, *cx = nullptr;
     int amg_kmax = 0;
     uint64_t op_epoch = 0;  // the DM's operator epoch the assembled copies were made at
+    // continuous batching (plx_gf_slots_*): per-column solves that start and retire on their own
+    struct Slot {
+        int state = 0;        // PLX_SLOT_EMPTY / RUNNING / CONVERGED / FAILED
+        int outer = 0, inner = 0;
+        int ready_since = -1; // step at which its inner solve converged (-1: not yet)
+        double b0 = 0.0, scale = 0.0, relres = 0.0;
+    };
+    std::vector<Slot> slots;
+    int slots_k = 0, slots_maxit = 0, slots_step = 0;
+    double slots_rtol = 0.0;
+    double *d_rr0 = nullptr;       // [64] r.r of each column at its inner PCG start
+    double *d_slot_slip = nullptr; // [k][num_lag][3] each slot's slip
 };
 
 namespace plx {
@@ -1275,7 +1287,7 @@ int plx_gf_destroy(plx_gf g) {
         if (g->graph[i]) (void)hipGraphExecDestroy(g->graph[i]);
     void *ptrs[] = {g->d_row_ptr, g->d_col, g->d_val64, g->d_val32, g->d_Binv64, g->d_Binv32, g->B64, g->X64,
                     g->R64,       g->X32,   g->R32,     g->P32,     g->Q32,      g->Z32,      g->v1,  g->v2,
-                    g->jump,      g->scal,  g->prz,     g->prr,     g->nrm};
+                    g->jump,      g->scal,  g->prz,     g->prr,     g->nrm,      g->d_rr0,    g->d_slot_slip};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     gf_amg_free(g);
@@ -1453,6 +1465,385 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
     if (err) return err;
     st.converged = st.max_relative_residual <= rtol ? 1 : 0;
     if (stats) *stats = st;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Continuous batching: k columns solved as independent slots.  A slot is loaded with a new
+// right-hand side while the other columns keep iterating; every step replays one captured chunk
+// of the FP32 block PCG for all columns, then the slots whose inner solve has converged take
+// their FP64 refinement step (X += scale X32, R = B - A X in FP64) and either retire (relative
+// FP64 residual <= rtol), fail (outer iterations exhausted) or restart their inner PCG.  Columns
+// share no arithmetic: each has its own PCG scalars, so a slot's iterates do not depend on the
+// other columns.
+// ---------------------------------------------------------------------------------------------
+namespace plx {
+
+// Column selections and per-column scales travel as kernel arguments (k <= 64): bit c of `mask`
+// selects column c.
+struct ColumnScale {
+    double v[64];
+};
+
+__device__ __forceinline__ bool col_on(uint64_t mask, int c) { return (mask >> c) & 1ull; }
+
+__global__ void __launch_bounds__(256) gf_to32_mask(int64_t ndof, int k, uint64_t mask,
+                                                    const double *__restrict__ R64, ColumnScale scale,
+                                                    float *R32, float *X32) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % k);
+        if (!col_on(mask, c)) continue;
+        const double sc = scale.v[c];
+        R32[t] = sc > 0.0 ? (float)(R64[t] / sc) : 0.0f;
+        X32[t] = 0.0f;
+    }
+}
+
+// X64 += scale[c] X32 on the selected columns
+__global__ void __launch_bounds__(256) gf_correct_mask(int64_t ndof, int k, uint64_t mask, ColumnScale scale,
+                                                       const float *__restrict__ X32, double *X64) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % k);
+        if (col_on(mask, c)) X64[t] += scale.v[c] * (double)X32[t];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gf_zero_cols(int64_t ndof, int k, uint64_t mask, T *A) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        if (col_on(mask, (int)(t % k))) A[t] = T(0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gf_copy_cols(int64_t ndof, int k, uint64_t mask, const T *__restrict__ A,
+                                                    T *B) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        if (col_on(mask, (int)(t % k))) B[t] = A[t];
+}
+
+// z = Binv r on every column (the point-block Jacobi preconditioner)
+template <int K>
+__global__ void __launch_bounds__(256) gfk_zjacobi(int nrows, const float *__restrict__ Binv,
+                                                   const float *__restrict__ R, float *Z) {
+    const int64_t n = (int64_t)nrows * K;
+    const int rc = (int)(threadIdx.x % K);
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / K;
+        const size_t o = (3 * (size_t)i) * K + rc;
+        const float *B = Binv + 9 * i;
+        const float r0 = R[o], r1 = R[o + K], r2 = R[o + 2 * K];
+        for (int c = 0; c < 3; ++c) Z[o + c * K] = B[3 * c] * r0 + B[3 * c + 1] * r1 + B[3 * c + 2] * r2;
+    }
+}
+
+// PCG start of the selected columns: beta = 0, r.z and r.r from the partials; r.r also kept as the
+// column's starting residual (rr0) for its convergence test
+__global__ void __launch_bounds__(256) gf_finish_start_mask(int k, const double *__restrict__ prz,
+                                                            const double *__restrict__ prr, double *scal,
+                                                            uint64_t mask, double *rr0) {
+    const int r = blockIdx.x;
+    const double rz = col_sum(prz, k, r);
+    __syncthreads();
+    const double rr = col_sum(prr, k, r);
+    if (threadIdx.x == 0 && col_on(mask, r)) {
+        scal[G_BETA * 64 + r] = 0.0;
+        scal[G_RZ * 64 + r] = rz;
+        scal[G_RR * 64 + r] = rr;
+        rr0[r] = rr;
+    }
+}
+
+// Launches of the handle run on its own stream; the DM's launches inside are redirected there,
+// ordered after the work already queued on the DM's stream, and the DM's stream waits for them.
+class GfStreamScope {
+  public:
+    explicit GfStreamScope(plx_gf g) : g_(g), saved_(g->dm->stream) {
+        if (hipEventCreateWithFlags(&ev_, hipEventDisableTiming) != hipSuccess) return;
+        if (hipEventRecord(ev_, saved_) != hipSuccess || hipStreamWaitEvent(g->stream, ev_, 0) != hipSuccess) return;
+        g->dm->stream = g->stream;
+        ok_ = true;
+    }
+    ~GfStreamScope() {
+        g_->dm->stream = saved_;
+        if (ok_ && hipEventRecord(ev_, g_->stream) == hipSuccess) (void)hipStreamWaitEvent(saved_, ev_, 0);
+        if (ev_) (void)hipEventDestroy(ev_);
+    }
+    bool ok() const { return ok_; }
+
+  private:
+    plx_gf g_;
+    hipStream_t saved_;
+    hipEvent_t ev_ = nullptr;
+    bool ok_ = false;
+};
+
+// Column c of B64 = the impulse right-hand side -U^T J u_jump of one slip (as plx_gf_solve).
+static int gf_rhs_column(plx_gf g, const double *slip, int c, int k) {
+    plx_dm dm = g->dm;
+    FaultData &F = *dm->fault;
+    const int64_t nloc = dm->local_size, ndof = g->ndof;
+    const int nc = (int)dm->con.dofs.size();
+    int err = launch_planar_jump(dm, F.num_lag, slip, g->jump);
+    if (err) return err;
+    hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, g->v1, 0.0);
+    hipLaunchKernelGGL(gf_scatter_jump, dim3((F.num_lag + 255) / 256), dim3(256), 0, g->stream, F.num_lag,
+                       F.d_pos_node, g->jump, g->v1);
+    if (g->keys.size() == 1) {
+        if ((err = launch_apply(dm, g->kp, PLX_FP64, true, g->v1, g->v2, PLX_INSERT_VALUES, 1.0))) return err;
+    } else {
+        hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, g->v2, 0.0);
+        for (KeyPlan *kp : g->keys)
+            if ((err = launch_apply(dm, kp, PLX_FP64, true, g->v1, g->v2, PLX_ADD_VALUES, 1.0))) return err;
+    }
+    if ((err = launch_fold(dm, PLX_FP64, g->v2, kFold))) return err;
+    if (nc > 0) hipLaunchKernelGGL(gf_mask, dim3((nc + 255) / 256), dim3(256), 0, g->stream, nc, dm->con.d_dofs, g->v2);
+    hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->v2, g->B64);
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
+static uint64_t mask_bits(const std::vector<int> &mask) {
+    uint64_t m = 0;
+    for (size_t c = 0; c < mask.size() && c < 64; ++c)
+        if (mask[c]) m |= 1ull << c;
+    return m;
+}
+
+static ColumnScale slot_scales(plx_gf g) {
+    ColumnScale sc;
+    for (int c = 0; c < 64; ++c) sc.v[c] = c < g->slots_k ? g->slots[c].scale : 0.0;
+    return sc;
+}
+
+template <int K>
+static void gf_start_mask_k(plx_gf g, uint64_t mask) {
+    if (g->pc == PLX_GF_PC_AMG)
+        amg_vcycle<K>(g, 0, g->R32, g->Z32);
+    else
+        hipLaunchKernelGGL(gfk_zjacobi<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_Binv32, g->R32, g->Z32);
+    hipLaunchKernelGGL(gf_copy_cols<float>, gf_grid(g->ndof * K), dim3(256), 0, g->stream, g->ndof, K, mask,
+                       g->Z32, g->P32);
+    hipLaunchKernelGGL(gfk_dot2<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, g->Z32, g->R32, g->prz, g->prr);
+    hipLaunchKernelGGL(gf_finish_start_mask, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->prr, g->scal, mask,
+                       g->d_rr0);
+}
+
+// (Re)start the inner PCG of the selected slots from R32 = R64 / scale, X32 = 0 (no host wait:
+// the starting residuals stay on the device for the next step's convergence test).
+static int gf_restart(plx_gf g, uint64_t mask) {
+    plx_dm dm = g->dm;
+    const int k = g->slots_k;
+    hipLaunchKernelGGL(gf_to32_mask, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k, mask, g->R64,
+                       slot_scales(g), g->R32, g->X32);
+    switch (k) {
+    case 8: gf_start_mask_k<8>(g, mask); break;
+    case 16: gf_start_mask_k<16>(g, mask); break;
+    case 32: gf_start_mask_k<32>(g, mask); break;
+    default: gf_start_mask_k<64>(g, mask); break;
+    }
+    PLX_HIP(dm, hipGetLastError());
+    for (int c = 0; c < k; ++c)
+        if ((mask >> c) & 1ull) {
+            g->slots[c].inner = 0;
+            g->slots[c].ready_since = -1;
+        }
+    return 0;
+}
+
+// Idle columns: zero vectors (the PCG then computes alpha = beta = 0 for them).
+static int gf_idle(plx_gf g, uint64_t mask, bool clear_solution) {
+    plx_dm dm = g->dm;
+    const int k = g->slots_k;
+    for (float *v : {g->R32, g->X32, g->P32, g->Z32})
+        hipLaunchKernelGGL(gf_zero_cols<float>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k, mask, v);
+    if (clear_solution)
+        for (double *v : {g->B64, g->X64, g->R64})
+            hipLaunchKernelGGL(gf_zero_cols<double>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k, mask, v);
+    PLX_HIP(dm, hipGetLastError());
+    return 0;
+}
+
+}  // namespace plx
+
+extern "C" {
+
+int plx_gf_slots_open(plx_gf g, int k, double rtol, int maxit) {
+    if (!g) return 1;
+    plx_dm dm = g->dm;
+    if (k != 8 && k != 16 && k != 32 && k != 64) return set_error(dm, 1, "slot count must be 8, 16, 32 or 64");
+    if (!(rtol > 0.0) || maxit <= 0) return set_error(dm, 1, "bad slot tolerance");
+    if (!dm->fault) return set_error(dm, 1, "Green's-function slots need a fault");
+    if (g->op_epoch != dm->op_epoch)
+        return set_error(dm, 1, "the DM's operator changed after plx_gf_create: destroy and recreate the handle");
+    int err = gf_alloc(g, k);
+    if (err) return err;
+    if (!g->d_rr0) PLX_HIP(dm, hipMalloc(&g->d_rr0, 64 * sizeof(double)));
+    if (g->d_slot_slip) (void)hipFree(g->d_slot_slip);
+    g->d_slot_slip = nullptr;
+    PLX_HIP(dm, hipMalloc(&g->d_slot_slip, (size_t)k * 3 * dm->fault->num_lag * sizeof(double)));
+    g->slots.assign(k, plx_gf_s::Slot());
+    g->slots_k = k;
+    g->slots_rtol = rtol;
+    g->slots_maxit = maxit;
+    g->slots_step = 0;
+    GfStreamScope scope(g);
+    if (!scope.ok()) return set_error(dm, 76, "stream ordering failed");
+    hipLaunchKernelGGL(gf_fill, dim3(1), dim3(64), 0, g->stream, 64, g->d_rr0, 0.0);
+    const uint64_t all = k == 64 ? ~0ull : ((1ull << k) - 1);
+    if ((err = gf_idle(g, all, true))) return err;
+    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int plx_gf_slots_load(plx_gf g, int n, const int *slot_ids, const double *slip) {
+    if (!g || n < 0 || (n > 0 && (!slot_ids || !slip))) return 1;
+    plx_dm dm = g->dm;
+    if (!g->slots_k) return set_error(dm, 1, "plx_gf_slots_open first");
+    if (g->op_epoch != dm->op_epoch)
+        return set_error(dm, 1, "the DM's operator changed after plx_gf_create: destroy and recreate the handle");
+    const int k = g->slots_k;
+    const size_t nl3 = 3 * (size_t)dm->fault->num_lag;
+    uint64_t mask = 0;
+    for (int i = 0; i < n; ++i) {
+        const int c = slot_ids[i];
+        if (c < 0 || c >= k) return set_error(dm, 1, "slot index out of range");
+        if (g->slots[c].state != PLX_SLOT_EMPTY) return set_error(dm, 1, "slot is not empty");
+        if ((mask >> c) & 1ull) return set_error(dm, 1, "slot listed twice");
+        mask |= 1ull << c;
+    }
+    if (n == 0) return 0;
+    GfStreamScope scope(g);
+    if (!scope.ok()) return set_error(dm, 76, "stream ordering failed");
+    int err = 0;
+    for (int i = 0; i < n && !err; ++i) {
+        const int c = slot_ids[i];
+        PLX_HIP(dm, hipMemcpyAsync(g->d_slot_slip + c * nl3, slip + i * nl3, nl3 * sizeof(double),
+                                   hipMemcpyDeviceToDevice, g->stream));
+        err = gf_rhs_column(g, g->d_slot_slip + c * nl3, c, k);
+    }
+    if (err) return err;
+    // X = 0, R = B on the loaded columns
+    hipLaunchKernelGGL(gf_zero_cols<double>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k, mask, g->X64);
+    hipLaunchKernelGGL(gf_copy_cols<double>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k, mask, g->B64,
+                       g->R64);
+    std::vector<double> b0;
+    if ((err = gf_norms(g, k, g->B64, b0))) return err;
+    for (int c = 0; c < k; ++c)
+        if ((mask >> c) & 1ull) {
+            plx_gf_s::Slot &S = g->slots[c];
+            S = plx_gf_s::Slot();
+            S.state = PLX_SLOT_RUNNING;
+            S.b0 = b0[c];
+            S.scale = b0[c];
+            S.relres = 1.0;
+        }
+    return gf_restart(g, mask);
+}
+
+int plx_gf_slots_step(plx_gf g, int *state) {
+    if (!g) return 1;
+    plx_dm dm = g->dm;
+    const int k = g->slots_k;
+    if (!k) return set_error(dm, 1, "plx_gf_slots_open first");
+    if (g->op_epoch != dm->op_epoch)
+        return set_error(dm, 1, "the DM's operator changed after plx_gf_create: destroy and recreate the handle");
+    int running = 0;
+    for (const auto &S : g->slots) running += S.state == PLX_SLOT_RUNNING;
+    int err = 0;
+    if (running) {
+        GfStreamScope scope(g);
+        if (!scope.ok()) return set_error(dm, 76, "stream ordering failed");
+        if ((err = gf_run_chunk(g, k))) return err;
+        const int its = g->pc == PLX_GF_PC_AMG ? kAmgChunk : 10;
+        const int step = ++g->slots_step;
+        double rr[64], rr0[64];
+        PLX_HIP(dm, hipMemcpyAsync(rr, g->scal + G_RR * 64, k * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        PLX_HIP(dm, hipMemcpyAsync(rr0, g->d_rr0, k * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        PLX_HIP(dm, hipStreamSynchronize(g->stream));
+        int ready = 0, oldest = 0;
+        for (int c = 0; c < k; ++c) {
+            plx_gf_s::Slot &S = g->slots[c];
+            if (S.state != PLX_SLOT_RUNNING) continue;
+            S.inner += its;
+            const bool done = rr0[c] <= 0.0 || rr[c] <= g->inner_rtol * g->inner_rtol * rr0[c] || S.inner >= g->inner_maxit;
+            if (done && S.ready_since < 0) S.ready_since = step;
+            if (S.ready_since >= 0) {
+                ++ready;
+                oldest = std::max(oldest, step - S.ready_since);
+            }
+        }
+        // refinement rounds: when a quarter of the running slots (at least one) are ready, or a
+        // ready slot has waited 3 steps; a waiting slot's inner PCG keeps refining its correction
+        if (ready > 0 && (4 * ready >= running || oldest >= 3)) {
+            uint64_t sel = 0;
+            for (int c = 0; c < k; ++c)
+                if (g->slots[c].state == PLX_SLOT_RUNNING && g->slots[c].ready_since >= 0) sel |= 1ull << c;
+            // X64 += scale X32 on the selected columns, then the FP64 residual of every column
+            hipLaunchKernelGGL(gf_correct_mask, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k, sel,
+                               slot_scales(g), g->X32, g->X64);
+            gf_spmm_launch<double>(g, k, g->d_val64, g->X64, g->R64);
+            hipLaunchKernelGGL(gf_resid, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof * k, g->B64, g->R64);
+            PLX_HIP(dm, hipGetLastError());
+            std::vector<double> rn;
+            if ((err = gf_norms(g, k, g->R64, rn))) return err;
+            uint64_t restart = 0, finished = 0;
+            for (int c = 0; c < k; ++c) {
+                if (!((sel >> c) & 1ull)) continue;
+                plx_gf_s::Slot &S = g->slots[c];
+                S.outer++;
+                S.relres = S.b0 > 0.0 ? rn[c] / S.b0 : 0.0;
+                if (S.relres <= g->slots_rtol) {
+                    S.state = PLX_SLOT_CONVERGED;
+                    finished |= 1ull << c;
+                } else if (S.outer >= g->slots_maxit || !std::isfinite(S.relres)) {
+                    S.state = PLX_SLOT_FAILED;
+                    finished |= 1ull << c;
+                } else {
+                    S.scale = rn[c];
+                    restart |= 1ull << c;
+                }
+            }
+            if (finished && (err = gf_idle(g, finished, false))) return err;
+            if (restart && (err = gf_restart(g, restart))) return err;
+        }
+    }
+    if (state)
+        for (int c = 0; c < k; ++c) state[c] = g->slots[c].state;
+    return 0;
+}
+
+int plx_gf_slot_take(plx_gf g, int slot, double *u, plx_gf_slot_stats *stats) {
+    if (!g) return 1;
+    plx_dm dm = g->dm;
+    if (!g->slots_k || slot < 0 || slot >= g->slots_k) return set_error(dm, 1, "slot index out of range");
+    plx_gf_s::Slot &S = g->slots[slot];
+    if (S.state != PLX_SLOT_CONVERGED && S.state != PLX_SLOT_FAILED) return set_error(dm, 1, "slot has no result");
+    if (stats) {
+        stats->converged = S.state == PLX_SLOT_CONVERGED;
+        stats->outer_iterations = S.outer;
+        stats->inner_iterations = S.inner;
+        stats->relative_residual = S.relres;
+    }
+    const int k = g->slots_k;
+    const size_t nl3 = 3 * (size_t)dm->fault->num_lag;
+    GfStreamScope scope(g);
+    if (!scope.ok()) return set_error(dm, 76, "stream ordering failed");
+    int err = 0;
+    if (u) {
+        // u = U w + u_jump (Lagrange multipliers left at zero)
+        hipLaunchKernelGGL(gf_fill, gf_grid(dm->local_size), dim3(256), 0, g->stream, dm->local_size, u, 0.0);
+        hipLaunchKernelGGL(gf_unpack, gf_grid(g->ndof), dim3(256), 0, g->stream, g->ndof, k, slot, g->X64, u);
+        if ((err = launch_planar_jump(dm, dm->fault->num_lag, g->d_slot_slip + slot * nl3, g->jump))) return err;
+        if ((err = launch_sync_pos(dm, g->jump, u))) return err;
+    }
+    if ((err = gf_idle(g, 1ull << slot, true))) return err;
+    S = plx_gf_s::Slot();
     return 0;
 }
 

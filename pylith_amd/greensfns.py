@@ -8,6 +8,7 @@ preconditioned by point-block Jacobi or by a smoothed-aggregation multigrid V-cy
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -57,6 +58,30 @@ class GreensFunctions:
         self._check(self.lib.plx_gf_solve(self.h, slip.shape[0], _ptr(slip), _ptr(u), rtol, maxit,
                                           ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in L.GfStats._fields_}
+
+    # --- continuous batching: k independent slots of one block ---------------------------------
+    def slots_open(self, k, rtol, maxit=30):
+        self._check(self.lib.plx_gf_slots_open(self.h, k, rtol, maxit))
+        self.k = k
+
+    def slots_load(self, slot_ids, slip):
+        """slip: [n, num_lag, 3] float64 CUDA tensor for the empty slots `slot_ids`."""
+        ids = np.ascontiguousarray(slot_ids, np.int32)
+        if slip.dtype != torch.float64 or slip.shape[0] != ids.size:
+            raise PlxError("slip must be float64 with one row per slot")
+        self._check(self.lib.plx_gf_slots_load(self.h, ids.size, ids.ctypes.data_as(L._i32p), _ptr(slip)))
+
+    def slots_step(self):
+        """One step of every running slot; returns the slot states (L.SLOT_*)."""
+        st = np.zeros(self.k, np.int32)
+        self._check(self.lib.plx_gf_slots_step(self.h, st.ctypes.data_as(L._i32p)))
+        return st
+
+    def slot_take(self, slot, u=None):
+        """Result of a finished slot into u ([local size] float64 CUDA tensor or None)."""
+        stats = L.SlotStats()
+        self._check(self.lib.plx_gf_slot_take(self.h, slot, _ptr(u), ctypes.byref(stats)))
+        return {f: getattr(stats, f) for f, _ in L.SlotStats._fields_}
 
     def close(self):
         if getattr(self, "h", None):

@@ -24,16 +24,16 @@ class FaultedProblem:
         return self.dm.sizes()[0]
 
 
-def faulted_box(n, length=6000.0, dip_deg=None, seed=None, vs_sigma=None, vp_vs=1.732, brick=8):
+def faulted_box(n, length=6000.0, dip_deg=None, seed=None, vs_sigma=None, vp_vs=1.732, brick=8, vs=3000.0):
     """n^3 Q1 cells in a cube of side `length` (z from -length to 0), fault plane x = length/2
     (i = n/2); `dip_deg` shears x by cot(dip) z so the fault dips at that angle."""
     from .dm import DM
     shear = None if dip_deg is None else 1.0 / math.tan(math.radians(dip_deg))
     m, f = mg.hex_box_with_fault(n, (0.0, 0.0, -length), (length, length, 0.0), n // 2, brick=brick, shear=shear)
-    if vs_sigma is None:
-        aux, sizes = mg.elastic_aux(m.num_cells)
+    if vs_sigma is None:  # uniform: Vp / Vs of the 6 km benchmark cube (5292 / 3000 m/s)
+        aux, sizes = mg.elastic_aux(m.num_cells, vs=vs, vp=vs * 5292.0 / 3000.0)
     else:
-        aux, sizes = mg.elastic_aux(m.num_cells, seed=seed, vs_sigma=vs_sigma, vp_vs=vp_vs)
+        aux, sizes = mg.elastic_aux(m.num_cells, vs=vs, seed=seed, vs_sigma=vs_sigma, vp_vs=vp_vs)
     x0 = m.node_coords[:, 0] - (shear or 0.0) * m.node_coords[:, 2]
     fixed = np.nonzero(np.isclose(x0, 0.0, atol=1e-9 * length) | np.isclose(x0, length, atol=1e-9 * length))[0]
     dofs = np.sort((3 * fixed[:, None] + np.arange(3)).ravel()).astype(np.int32)
