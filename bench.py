@@ -428,12 +428,15 @@ def bench_c4(torch, mg, L, n_impulses=512):
     return out
 
 
-def bench_c5(torch, n_requests=2000, max_block=32, n_capacity=400):
+def bench_c5(torch, n_requests=2000, max_block=64, n_capacity=400, inner_rtol=1e-5, load=0.6):
     """C5: Poisson stream of fault-slip solves on 16^3/32^3/64^3/96^3 meshes (40/30/20/10%),
     64 patches with N(0, 0.5 m) slip, lognormal Vs per cell (per mesh size), 200 GB budget,
     continuous batching (requests join the running block of their size as slots free up), each
     request solved to an FP64 relative residual of 1e-9 (errors vs FP64 <= 1e-8).  Capacity:
-    `n_capacity` requests queued at once; the stream: `n_requests` Poisson arrivals at 70% of it."""
+    `n_capacity` requests queued at once; the stream: `n_requests` Poisson arrivals at `load` of
+    it (60%: the stream's served rate then keeps up with the offered rate, so p50/p99 are those of
+    a steady state and not of a growing queue).  64 slots per mesh size and an inner FP32 tolerance
+    of 1e-5 (one refinement step fewer than 1e-4) were the fastest of the measured settings."""
     import numpy as np
     from pylith_amd import _lib as L
     from pylith_amd import serving
@@ -441,7 +444,7 @@ def bench_c5(torch, n_requests=2000, max_block=32, n_capacity=400):
     from pylith_amd.dm import Solver
     sizes, probs = (16, 32, 64, 96), (0.4, 0.3, 0.2, 0.1)
     rtol = 1e-9
-    eng = serving.ServingEngine(sizes, block=max_block, rtol=rtol)
+    eng = serving.ServingEngine(sizes, block=max_block, rtol=rtol, inner_rtol=inner_rtol)
     t0 = time.perf_counter()
     eng.warm()
     t_warm = time.perf_counter() - t0
@@ -449,7 +452,7 @@ def bench_c5(torch, n_requests=2000, max_block=32, n_capacity=400):
     cap_reqs = serving.make_requests(n_capacity, sizes, probs, rate=None, seed=11)
     done = eng.run(cap_reqs)
     cap = serving.latency_stats(done)["solves_per_s"]
-    rate = 0.7 * cap
+    rate = load * cap
     reqs = serving.make_requests(n_requests, sizes, probs, rate=rate, seed=12)
     done = eng.run(reqs)
     stats = serving.latency_stats(done)
@@ -491,11 +494,34 @@ def bench_c5(torch, n_requests=2000, max_block=32, n_capacity=400):
             "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"], "solves_per_s": stats["solves_per_s"],
             "errors": stats["errors"], "joined_a_running_block": joined,
             "per_size": per, "warm_seconds": t_warm, "resident_bytes_measured": resident,
-            "rtol": rtol, "max_block": max_block,
+            "rtol": rtol, "max_block": max_block, "inner_rtol": inner_rtol, "load_fraction_of_capacity": load,
             "fp64_single_seconds": t64, "fp64_solves_per_s_mix": fp64_rate,
             "fp64_multigrid_single_seconds": t64mg, "fp64_multigrid_solves_per_s_mix": fp64_mg_rate,
             "capacity_speedup_vs_fp64": cap / fp64_rate,
             "capacity_speedup_vs_fp64_multigrid": cap / fp64_mg_rate}
+
+
+def bench_c5_sweep(torch):
+    """Serving capacity (requests queued at once) of the C5 mix for several slot-block widths
+    and inner FP32 tolerances, and of each mesh size served alone."""
+    from pylith_amd import serving
+    sizes, probs = (16, 32, 64, 96), (0.4, 0.3, 0.2, 0.1)
+    out = {}
+    for block, inner in ((32, 1e-4), (32, 1e-5), (64, 1e-4), (16, 1e-4)):
+        eng = serving.ServingEngine(sizes, block=block, rtol=1e-9, inner_rtol=inner)
+        eng.warm()
+        done = eng.run(serving.make_requests(400, sizes, probs, rate=None, seed=11))
+        st = serving.latency_stats(done)
+        st["outer_mean"] = float(sum(r.stats["outer_iterations"] for r in done) / len(done))
+        st["inner_mean"] = float(sum(r.stats["inner_iterations"] for r in done) / len(done))
+        out[f"block{block}_inner{inner:g}"] = st
+        if block == 32 and inner == 1e-4:
+            for sz in sizes:
+                done = eng.run(serving.make_requests(100, (sz,), (1.0,), rate=None, seed=21))
+                out[f"alone_{sz}"] = serving.latency_stats(done)
+        eng.close()
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_p2(torch, mg, n_edge, steps, warmup):
@@ -555,7 +581,8 @@ def main():
     ap.add_argument("--q1-large", type=int, default=192, help="edge of the large Q1 cube")
     ap.add_argument("--no-systems", action="store_true",
                     help="skip the solver / viscoelastic / Green's-function / serving measurements")
-    ap.add_argument("--only", choices=["c1solve", "c2solve", "c3", "c3amg", "c4", "c5", "c5b64", "q1", "p2large"],
+    ap.add_argument("--only", choices=["c1solve", "c2solve", "c3", "c3amg", "c4", "c5", "c5b64", "c5b32", "c5sweep", "q1",
+                                       "p2large"],
                     default=None,
                     help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
@@ -578,7 +605,8 @@ def main():
               "c2solve": lambda: bench_c2_solve(torch, mg, L), "c3": lambda: bench_c3(torch, mg, L),
               "c3amg": lambda: bench_c3(torch, mg, L, "amg"),
               "c4": lambda: bench_c4(torch, mg, L), "c5": lambda: bench_c5(torch),
-              "c5b64": lambda: bench_c5(torch, max_block=64),
+              "c5b64": lambda: bench_c5(torch, max_block=64), "c5sweep": lambda: bench_c5_sweep(torch),
+              "c5b32": lambda: bench_c5(torch, max_block=32),
               "q1": lambda: bench_q1(torch, mg, args.q1_large, args.steps, args.warmup),
               "p2large": lambda: bench_p2(torch, mg, 96, args.steps, args.warmup)}[args.only]
         print(f"[bench] {args.only}", file=sys.stderr, flush=True)

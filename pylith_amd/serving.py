@@ -53,7 +53,7 @@ class MeshState:
     """Resident device state of one mesh size: DM, fault, operator, preconditioner, k slots."""
 
     def __init__(self, size, seed, block, rtol, maxit=30, n_patch=(8, 8), precond="amg", vs=3000.0,
-                 vs_sigma=0.2):
+                 vs_sigma=0.2, inner_rtol=1e-4):
         from .greensfns import GreensFunctions
         if not (np.isfinite(vs) and vs > 0.0):  # materials/Query.cc:223-231: Vs must be positive
             raise ValueError(f"mesh size {size}: shear-wave speed {vs} is not positive")
@@ -66,6 +66,7 @@ class MeshState:
         self.prob.dm.set_stream(self.stream)
         self.gf = GreensFunctions(self.prob.dm)
         self.gf.set_preconditioner(precond)
+        self.gf.set_options(block=block, inner_rtol=inner_rtol)
         self.gf.slots_open(block, rtol, maxit)
         self.block = block
         self.rtol = rtol
@@ -121,8 +122,9 @@ class MeshState:
 
 class ServingEngine:
     def __init__(self, sizes, budget_bytes=200 * GB, block=16, rtol=1e-9, maxit=30, seed=20261202,
-                 keep_results=False, precond="amg", material=None, state_factory=None):
+                 keep_results=False, precond="amg", material=None, state_factory=None, inner_rtol=1e-4):
         self.state_factory = state_factory or MeshState  # the resident state class of a mesh size
+        self.inner_rtol = inner_rtol
         self.sizes = list(sizes)
         self.precond = precond
         self.budget = budget_bytes
@@ -150,7 +152,8 @@ class ServingEngine:
         if size not in self.states:
             try:
                 st = self.state_factory(size, self.seed + size, self.block, self.rtol, self.maxit,
-                                        precond=self.precond, **self.material.get(size, {}))
+                                        precond=self.precond, inner_rtol=self.inner_rtol,
+                                        **self.material.get(size, {}))
             except Exception as e:
                 self.state_errors[size] = e
                 raise

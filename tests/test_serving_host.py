@@ -17,7 +17,7 @@ class FakeState:
     n_patches = 4
     steps = 3
 
-    def __init__(self, size, seed, block, rtol, maxit=30, precond="amg", vs=3000.0, vs_sigma=0.2):
+    def __init__(self, size, seed, block, rtol, maxit=30, precond="amg", vs=3000.0, vs_sigma=0.2, inner_rtol=1e-4):
         if not vs > 0.0:
             raise ValueError(f"mesh size {size}: shear-wave speed {vs} is not positive")
         self.size, self.block = size, block
