@@ -524,9 +524,12 @@ def bench_c5_sweep(torch):
     return out
 
 
-def bench_p2(torch, mg, n_edge, steps, warmup):
-    """P2 FP64 residual apply on an n^3 x 6 box (same synthetic inputs as the headline)."""
+def bench_p2(torch, mg, n_edge, steps, warmup, renumber_seed=None):
+    """P2 FP64 residual apply on an n^3 x 6 box (same synthetic inputs as the headline); with
+    `renumber_seed` the cells, nodes and vertices are renumbered at random first."""
     mesh = mg.tet_p2_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    if renumber_seed is not None:
+        mesh = mg.renumbered(mesh, renumber_seed)
     aux, sizes = mg.elastic_aux(mesh.num_cells, seed=SEED, vs_sigma=0.2, vp_vs=1.732)
     dm = setup(mesh, aux, sizes)
     x = torch.from_numpy(mg.sinusoid_field(mesh.node_coords, SEED)).cuda()
@@ -541,10 +544,13 @@ def bench_p2(torch, mg, n_edge, steps, warmup):
             "hbm_frac_kernel": B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def bench_q1(torch, mg, n_edge, steps, warmup):
+def bench_q1(torch, mg, n_edge, steps, warmup, renumber_seed=None):
     """Q1 hex FP64 residual apply on an n^3 cube (6 km, u ~ U[-1e-3, 1e-3] m with the shear
-    Dirichlet values inserted): wall time per apply, tile-kernel time and HBM fractions."""
+    Dirichlet values inserted): wall time per apply, tile-kernel time and HBM fractions; with
+    `renumber_seed` the cells and nodes are renumbered at random first."""
     hm = mg.hex_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    if renumber_seed is not None:
+        hm = mg.renumbered(hm, renumber_seed)
     haux, hs = mg.elastic_aux(hm.num_cells)
     hu = mg.random_field(hm.num_nodes, SEED)
     dofs_bc, vals = mg.shear_bcs(hm.node_coords, 0.0, 6000.0)
@@ -756,6 +762,15 @@ def main():
             steps = args.steps if n_edge <= 64 else max(10, args.steps // 2)
             extra[f"q1_fp64_residual_{n_edge}cube"] = bench_q1(torch, mg, n_edge, steps, args.warmup)
             torch.cuda.empty_cache()
+        # meshes numbered without locality (random renumbering of cells and nodes): the planner
+        # tiles them in Morton order of the cell centroids
+        print("[bench] renumbered meshes", file=sys.stderr, flush=True)
+        extra["q1_fp64_residual_192cube_renumbered"] = bench_q1(torch, mg, args.q1_large, max(10, args.steps // 2),
+                                                                args.warmup, renumber_seed=5)
+        torch.cuda.empty_cache()
+        extra["p2_fp64_residual_64cube_renumbered"] = bench_p2(torch, mg, 64, max(10, args.steps // 2), args.warmup,
+                                                               renumber_seed=6)
+        torch.cuda.empty_cache()
         result["extra"] = extra
     if not args.no_systems:
         from pylith_amd import _lib as L

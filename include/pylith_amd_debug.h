@@ -19,6 +19,12 @@
 extern "C" {
 #endif
 
+/* Tile-plan statistics of a key: tiles, halo slots (shared-node partial sums per apply) and
+ * whether the planner took the cells in Morton order of their centroids (input order tiled with
+ * more than 1.5 halo slots per cell). */
+int plx_debug_plan_info(plx_dm dm, int label_value, int part, int *num_tiles, int64_t *halo_slots,
+                        int *reordered);
+
 int plx_debug_emulate_apply(int cell_type, int num_cells, int num_vertices, int num_nodes,
                             const int32_t *closure, const int32_t *cone, const double *coords,
                             const int32_t *labels, int label_value, const double *lam_mu,

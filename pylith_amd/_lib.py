@@ -139,3 +139,5 @@ def load():
 SIGNATURES["plx_debug_emulate_apply"] = (_c_int, [_c_int, _c_int, _c_int, _c_int, _i32p, _i32p,
                                                   _dblp, _i32p, _c_int, _dblp, _c_int, _vp, _vp,
                                                   _c_int, _c_int])
+SIGNATURES["plx_debug_plan_info"] = (_c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_c_int), _i64p,
+                                              ctypes.POINTER(_c_int)])

@@ -73,7 +73,7 @@ static void free_key(KeyPlan *k) {
 static int build_plan(plx_dm dm, KeyPlan *kp) {
     MeshView mv{dm->nb, dm->nv, dm->num_cells, dm->num_nodes, dm->num_vertices,
                 dm->closure.data(), dm->cone.data(), dm->labels.data(),
-                dm->cell_type == PLX_HEX_Q1};
+                dm->cell_type == PLX_HEX_Q1, dm->coords.data()};
     HostPlan hp;
     std::string msg;
     // P2 tiles keep colour order (the tets of a cube share nodes: colour order keeps a wave's
@@ -103,6 +103,7 @@ static int build_plan(plx_dm dm, KeyPlan *kp) {
     kp->max_colors = hp.max_colors;
     kp->halo_slots = hp.halo_slots;
     kp->num_shared = (int)hp.shared_node.size();
+    kp->reordered = hp.reordered;
     int err = 0;
     if ((err = upload(dm, &kp->d_cells, hp.cells.data(), hp.cells.size()))) return err;
     if ((err = upload(dm, &kp->d_tile_cell_ptr, hp.tile_cell_ptr.data(), hp.tile_cell_ptr.size()))) return err;
@@ -699,6 +700,16 @@ int plx_vec_norm2(plx_dm dm, int scalar_type, int64_t n, const void *x, double *
     const int err = launch_dot(dm, scalar_type, n, x, x, &d);
     if (err) return err;
     *result = std::sqrt(d);
+    return 0;
+}
+
+int plx_debug_plan_info(plx_dm dm, int label_value, int part, int *num_tiles, int64_t *halo_slots,
+                        int *reordered) {
+    KeyPlan *kp = dm ? find_key(dm, label_value, part) : nullptr;
+    if (!kp) return set_error(dm, 1, "unknown key");
+    if (num_tiles) *num_tiles = kp->num_tiles;
+    if (halo_slots) *halo_slots = kp->halo_slots;
+    if (reordered) *reordered = kp->reordered ? 1 : 0;
     return 0;
 }
 

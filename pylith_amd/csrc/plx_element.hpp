@@ -107,6 +107,51 @@ __host__ __device__ inline void iso_flux(const Real G[9], const Real Ji[9], Real
                            s[3 * c + 2] * Ji[3 * k + 2];
 }
 
+// Sum of the products a_k b_k over the k set in the 3-bit mask M (terms of structurally zero
+// operands are not formed: FP64 0 * x cannot be folded away by the compiler).
+template <int M, typename Real>
+__host__ __device__ inline Real msum3(Real a0, Real b0, Real a1, Real b1, Real a2, Real b2) {
+    if constexpr (M == 1) return a0 * b0;
+    else if constexpr (M == 2) return a1 * b1;
+    else if constexpr (M == 4) return a2 * b2;
+    else if constexpr (M == 3) return a0 * b0 + a1 * b1;
+    else if constexpr (M == 5) return a0 * b0 + a2 * b2;
+    else if constexpr (M == 6) return a1 * b1 + a2 * b2;
+    else return a0 * b0 + a1 * b1 + a2 * b2;
+}
+
+// iso_flux for a reference gradient whose columns outside IN are zero, forming only the flux
+// columns in OUT (bit k: reference direction k).
+template <typename Real, int IN, int OUT>
+__host__ __device__ inline void iso_flux_cols(const Real G[9], const Real Ji[9], Real lam, Real mu, Real scale,
+                                              Real P[9]) {
+    Real g[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            g[3 * c + d] = msum3<IN, Real>(G[3 * c + 0], Ji[0 * 3 + d], G[3 * c + 1], Ji[1 * 3 + d], G[3 * c + 2],
+                                           Ji[2 * 3 + d]);
+    const Real tr = g[0] + g[4] + g[8];
+    const Real ls = -scale * lam * tr;
+    const Real m2 = -scale * Real(2) * mu;
+    const Real m1 = -scale * mu;
+    Real s[9];
+    s[0] = ls + m2 * g[0];
+    s[4] = ls + m2 * g[4];
+    s[8] = ls + m2 * g[8];
+    s[1] = s[3] = m1 * (g[1] + g[3]);
+    s[5] = s[7] = m1 * (g[5] + g[7]);
+    s[2] = s[6] = m1 * (g[2] + g[6]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if ((OUT >> k) & 1)
+                P[3 * c + k] = s[3 * c + 0] * Ji[3 * k + 0] + s[3 * c + 1] * Ji[3 * k + 1] +
+                               s[3 * c + 2] * Ji[3 * k + 2];
+}
+
 // iso_flux plus a given stress ps (vector order xx, yy, zz, xy, yz, xz) added to sigma.
 template <typename Real>
 __host__ __device__ inline void iso_flux_pre(const Real G[9], const Real Ji[9], Real lam, Real mu,
@@ -450,11 +495,8 @@ __host__ __device__ inline void q1_element_affine(const double J[9], const UAcc 
     // Gradient modes (times 8), then fluxes scaled by det * nu_m / 8:
     //   const: [T1, T2, T4];  xi: [0, T3, T5];  eta: [T3, 0, T6];  zeta: [T5, T6, 0];
     //   eta zeta: [T7, 0, 0]; xi zeta: [0, T7, 0]; xi eta: [0, 0, T7]  (T3 = xi eta, T5 = xi zeta, T6 = eta zeta)
+    // Only the nonzero gradient columns enter and only the flux columns a mode feeds are formed.
     Real R[3][8];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int b = 0; b < 8; ++b) R[c][b] = Real(0);
     const Real s0 = (Real)(0.125 * det), s1 = (Real)(0.125 * det / 3.0), s2 = (Real)(0.125 * det / 9.0);
     Real G[9], P[9];
     // constant mode -> R_xi (b=1), R_eta (b=2), R_zeta (b=4): column k of P
@@ -464,65 +506,66 @@ __host__ __device__ inline void q1_element_affine(const double J[9], const UAcc 
         G[3 * c + 1] = T[c][2];
         G[3 * c + 2] = T[c][4];
     }
-    iso_flux<Real>(G, Ji, lam, mu, s0, P);
+    iso_flux_cols<Real, 7, 7>(G, Ji, lam, mu, s0, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        R[c][1] += P[3 * c + 0];
-        R[c][2] += P[3 * c + 1];
-        R[c][4] += P[3 * c + 2];
+        R[c][0] = Real(0);
+        R[c][1] = P[3 * c + 0];
+        R[c][2] = P[3 * c + 1];
+        R[c][4] = P[3 * c + 2];
     }
-    // xi mode: P_xi, columns eta -> R(xi eta) b=3, zeta -> R(xi zeta) b=5
+    // xi mode: columns eta, zeta of G; P_eta -> R(xi eta) b=3, P_zeta -> R(xi zeta) b=5
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        G[3 * c + 0] = Real(0);
         G[3 * c + 1] = T[c][3];
         G[3 * c + 2] = T[c][5];
     }
-    iso_flux<Real>(G, Ji, lam, mu, s1, P);
+    iso_flux_cols<Real, 6, 6>(G, Ji, lam, mu, s1, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        R[c][3] += P[3 * c + 1];
-        R[c][5] += P[3 * c + 2];
+        R[c][3] = P[3 * c + 1];
+        R[c][5] = P[3 * c + 2];
     }
-    // eta mode: columns xi -> b=3, zeta -> R(eta zeta) b=6
+    // eta mode: columns xi, zeta; P_xi -> b=3, P_zeta -> R(eta zeta) b=6
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         G[3 * c + 0] = T[c][3];
-        G[3 * c + 1] = Real(0);
         G[3 * c + 2] = T[c][6];
     }
-    iso_flux<Real>(G, Ji, lam, mu, s1, P);
+    iso_flux_cols<Real, 5, 5>(G, Ji, lam, mu, s1, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         R[c][3] += P[3 * c + 0];
-        R[c][6] += P[3 * c + 2];
+        R[c][6] = P[3 * c + 2];
     }
-    // zeta mode: columns xi -> b=5, eta -> b=6
+    // zeta mode: columns xi, eta; P_xi -> b=5, P_eta -> b=6
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         G[3 * c + 0] = T[c][5];
         G[3 * c + 1] = T[c][6];
-        G[3 * c + 2] = Real(0);
     }
-    iso_flux<Real>(G, Ji, lam, mu, s1, P);
+    iso_flux_cols<Real, 3, 3>(G, Ji, lam, mu, s1, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         R[c][5] += P[3 * c + 0];
         R[c][6] += P[3 * c + 1];
     }
-    // bilinear modes -> R(xi eta zeta) b=7: eta zeta (column xi), xi zeta (column eta), xi eta (column zeta)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            G[3 * c + 0] = k == 0 ? T[c][7] : Real(0);
-            G[3 * c + 1] = k == 1 ? T[c][7] : Real(0);
-            G[3 * c + 2] = k == 2 ? T[c][7] : Real(0);
-        }
-        iso_flux<Real>(G, Ji, lam, mu, s2, P);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) R[c][7] += P[3 * c + k];
-    }
+    // the three bilinear modes feed R(xi eta zeta) b=7 with one gradient column each, a = T[.][7]
+    // times row k of Jinv (b_k): sigma b_k = (lambda + mu)(a . b_k) b_k + mu |b_k|^2 a, so their
+    // sum is (lambda + mu) N a + mu tr(N) a with N = sum_k b_k b_k^T (the physical metric)
+    Real N[6];  // xx, yy, zz, xy, yz, xz
+    N[0] = Ji[0] * Ji[0] + Ji[3] * Ji[3] + Ji[6] * Ji[6];
+    N[1] = Ji[1] * Ji[1] + Ji[4] * Ji[4] + Ji[7] * Ji[7];
+    N[2] = Ji[2] * Ji[2] + Ji[5] * Ji[5] + Ji[8] * Ji[8];
+    N[3] = Ji[0] * Ji[1] + Ji[3] * Ji[4] + Ji[6] * Ji[7];
+    N[4] = Ji[1] * Ji[2] + Ji[4] * Ji[5] + Ji[7] * Ji[8];
+    N[5] = Ji[0] * Ji[2] + Ji[3] * Ji[5] + Ji[6] * Ji[8];
+    const Real trN = N[0] + N[1] + N[2];
+    const Real lm = -s2 * (lam + mu), mt = -s2 * mu * trN;
+    const Real a0 = T[0][7], a1 = T[1][7], a2 = T[2][7];
+    R[0][7] = lm * (N[0] * a0 + N[3] * a1 + N[5] * a2) + mt * a0;
+    R[1][7] = lm * (N[3] * a0 + N[1] * a1 + N[4] * a2) + mt * a1;
+    R[2][7] = lm * (N[5] * a0 + N[4] * a1 + N[2] * a2) + mt * a2;
 #pragma unroll
     for (int c = 0; c < 3; ++c) iwht8<Real>(R[c], re[c]);
 }

@@ -173,7 +173,7 @@ extern "C" int plx_debug_emulate_apply(int cell_type, int num_cells, int num_ver
         for (size_t i = 0; i < (size_t)num_cells * 8; ++i)
             if (cone[i] != closure[i]) return 1;
     plx::MeshView mv{nb, cell_type == PLX_HEX_Q1 ? 8 : 4, num_cells, num_nodes, num_vertices,
-                     closure, cone, labels, cell_type == PLX_HEX_Q1};
+                     closure, cone, labels, cell_type == PLX_HEX_Q1, coords};
     plx::HostPlan hp;
     std::string msg;
     if (plx::build_host_plan(mv, label_value, plx::tile_limits(cell_type), &hp, &msg)) return 1;

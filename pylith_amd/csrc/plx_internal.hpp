@@ -51,6 +51,7 @@ struct KeyPlan {
     int max_nodes = 0, max_verts = 0;
     int64_t halo_slots = 0;
     int num_shared = 0;
+    bool reordered = false;          // the planner took the cells in Morton order
     // device arrays
     int32_t *d_cells = nullptr;
     int32_t *d_tile_cell_ptr = nullptr, *d_tile_node_ptr = nullptr, *d_tile_nown = nullptr;

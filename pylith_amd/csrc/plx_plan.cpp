@@ -6,23 +6,89 @@
 // get different colours; cells sorted by colour), the node list with owned nodes first, 16-bit
 // local closures, and halo slots for nodes other tiles also touch.
 #include <algorithm>
+#include <cmath>
 
 #include "plx_plan.hpp"
 
 namespace plx {
 
+// Morton (Z-order) rank of each cell's centroid on a 2^21 grid over the cells' bounding box (a
+// finer grid than the cells, so cells sharing a grid point are rare).
+static std::vector<int32_t> morton_order(const MeshView &m, const std::vector<int32_t> &cells) {
+    const size_t n = cells.size();
+    std::vector<double> cen(3 * n);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t *cn = m.cone + (size_t)cells[i] * m.nv;
+        for (int d = 0; d < 3; ++d) {
+            double acc = 0.0;
+            for (int v = 0; v < m.nv; ++v) acc += m.coords[3 * (size_t)cn[v] + d];
+            cen[3 * i + d] = acc / m.nv;
+            lo[d] = std::min(lo[d], cen[3 * i + d]);
+            hi[d] = std::max(hi[d], cen[3 * i + d]);
+        }
+    }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    auto spread = [](uint64_t x) {  // 21 bits -> every third bit
+        x &= 0x1fffff;
+        x = (x | x << 32) & 0x1f00000000ffffull;
+        x = (x | x << 16) & 0x1f0000ff0000ffull;
+        x = (x | x << 8) & 0x100f00f00f00f00full;
+        x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double t = (cen[3 * i + d] - lo[d]) / span;
+            code |= spread((uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0))) << d;
+        }
+        key[i] = {code, cells[i]};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, int32_t> &a, const std::pair<uint64_t, int32_t> &b) {
+                         return a.first < b.first;
+                     });
+    std::vector<int32_t> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = key[i].second;
+    return out;
+}
+
+static int plan_cells(const MeshView &m, const TileLimits &lim, HostPlan *hp, std::string *err,
+                      bool sort_by_color);
+
 int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
                     std::string *err, bool sort_by_color) {
-    const int nb = m.nb, nv = m.nv;
     std::vector<int32_t> &cells = hp->cells;
     cells.clear();
     for (int c = 0; c < m.num_cells; ++c)
         if ((m.labels ? m.labels[c] : 0) == label_value) cells.push_back(c);
-    const int ncells = (int)cells.size();
-    if (ncells == 0) {
+    if (cells.empty()) {
         *err = "no cells carry the key's label value";
         return 1;
     }
+    const std::vector<int32_t> input = cells;
+    int e = plan_cells(m, lim, hp, err, sort_by_color);
+    hp->reordered = false;
+    if (e || !m.coords || (double)hp->halo_slots <= 1.5 * (double)input.size()) return e;
+    HostPlan alt;
+    alt.cells = morton_order(m, input);
+    std::string err2;
+    if (plan_cells(m, lim, &alt, &err2, sort_by_color) == 0 && alt.halo_slots < hp->halo_slots) {
+        *hp = std::move(alt);
+        hp->reordered = true;
+    }
+    return 0;
+}
+
+// The plan of hp->cells taken in their given order.
+static int plan_cells(const MeshView &m, const TileLimits &lim, HostPlan *hp, std::string *err,
+                      bool sort_by_color) {
+    const int nb = m.nb, nv = m.nv;
+    std::vector<int32_t> &cells = hp->cells;
+    const int ncells = (int)cells.size();
 
     // Greedy tiling in cell order, bounded by LDS capacity.
     std::vector<int32_t> node_mark(m.num_nodes, -1), vert_mark(m.num_vertices, -1);

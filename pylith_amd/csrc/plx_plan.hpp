@@ -27,6 +27,7 @@ struct MeshView {
     const int32_t *cone;
     const int32_t *labels;  // may be null (all 0)
     bool cone_is_closure;   // isoparametric Q1: vertices are the nodes, in closure order
+    const double *coords = nullptr;  // [num_vertices][3] (spatial reordering of badly ordered keys)
 };
 
 struct HostPlan {
@@ -41,11 +42,15 @@ struct HostPlan {
     // With cone_is_closure the vertex list of a tile IS its node list (same order), so the
     // kernels index staged coordinates with the local node index.
     std::vector<int32_t> shared_node, shared_ptr, shared_slot;
+    bool reordered = false;  // cells taken in Morton order of their centroids instead of input order
 };
 
 // sort_by_color: order each tile's cells by colour (colour-phase accumulation activates
 // contiguous threads); otherwise cells keep their mesh order inside the tile (neighbouring
 // threads then touch neighbouring nodes: structured LDS access for atomic accumulation).
+// When the key's cells in input order tile badly (more than 1.5 halo slots per cell, e.g. a mesh
+// numbered row by row), the cells are also tiled in Morton order of their centroids and the plan
+// with fewer halo slots is kept.
 int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
                     std::string *err, bool sort_by_color = true);
 

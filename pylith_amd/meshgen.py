@@ -93,6 +93,31 @@ def hex_box(n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), brick=8, ordered=True, sh
                 np.zeros(closure.shape[0], np.int32), colors)
 
 
+def renumbered(mesh, seed):
+    """The same mesh with cells, nodes and vertices renumbered by seeded random permutations (a
+    mesh whose numbering carries no locality, as a general mesh file or partitioner may give)."""
+    rng = np.random.default_rng(seed)
+    cperm = rng.permutation(mesh.num_cells)           # new cell i = old cell cperm[i]
+    nnew = np.empty(mesh.num_nodes, np.int64)
+    nnew[rng.permutation(mesh.num_nodes)] = np.arange(mesh.num_nodes)   # old node -> new id
+    isoparametric = mesh.cell_type == HEX_Q1
+    if isoparametric:
+        vnew = nnew
+    else:
+        vnew = np.empty(mesh.num_vertices, np.int64)
+        vnew[rng.permutation(mesh.num_vertices)] = np.arange(mesh.num_vertices)
+    closure = nnew[mesh.closure[cperm]].astype(np.int32)
+    cone = vnew[mesh.cone[cperm]].astype(np.int32)
+    node_coords = np.empty_like(mesh.node_coords)
+    node_coords[nnew] = mesh.node_coords
+    coords = node_coords if isoparametric else np.empty_like(mesh.coords)
+    if not isoparametric:
+        coords[vnew] = mesh.coords
+    labels = None if mesh.labels is None else mesh.labels[cperm]
+    colors = None if mesh.colors is None else mesh.colors[cperm]
+    return Mesh(mesh.cell_type, closure, cone, coords, node_coords, mesh.shape, labels, colors)
+
+
 # Kuhn split: the 6 vertex paths (0,0,0) -> ... -> (1,1,1) through the cube corners.
 _PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 _TET_EDGES = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]

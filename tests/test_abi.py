@@ -28,5 +28,5 @@ def test_library_exports_every_declared_entry_point():
     assert len(names) > 40
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    unbound = [n for n in sorted(names) if n not in _lib.SIGNATURES and n != "plx_debug_emulate_apply"]
+    unbound = [n for n in sorted(names) if n not in _lib.SIGNATURES]
     assert not unbound, unbound

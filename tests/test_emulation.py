@@ -52,6 +52,9 @@ def cases():
         "hex_ref_fixture": (ref, 24),
         "tet8": (mg.tet_p2_box(8, (0, 0, 0), (6000.0, 6000.0, 6000.0)), 0),
         "tet_unordered": (mg.tet_p2_box((3, 4, 2), (0, 0, 0), (3.0, 4.0, 2.0), ordered=False), 0),
+        # numbering without locality: the planner tiles these in Morton order
+        "hex_renumbered": (mg.renumbered(mg.hex_box(12, (0, 0, 0), (1200.0, 1200.0, 1200.0)), 7), 0),
+        "tet_renumbered": (mg.renumbered(mg.tet_p2_box(6, (0, 0, 0), (600.0, 600.0, 600.0)), 8), 0),
     }
 
 
