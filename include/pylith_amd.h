@@ -80,6 +80,21 @@ int plx_dm_create(plx_dm *dm, int cell_type, int num_cells, int num_vertices, in
                   const int32_t *closure, const int32_t *cone, const double *coords,
                   const int32_t *labels);
 int plx_dm_destroy(plx_dm dm);
+/* plx_dm_create with the caller's vertex order of the hexahedra (Q1): our vertex i of every cell
+ * is the caller's vertex vertex_perm[i] of that cell (NULL: the order above).  A DMPlex cone made
+ * by DMPlexInvertCell from the mesh file's order (meshio/MeshBuilder.cc:151; PETSc swaps
+ * hexahedron vertices 1 and 3) is mapped back by {0, 3, 2, 1, 4, 5, 6, 7}; cells are then checked
+ * for positive Jacobians as in plx_dm_create, so a wrong order is refused. */
+int plx_dm_create_ordered(plx_dm *dm, int cell_type, int num_cells, int num_vertices, int num_nodes,
+                          const int32_t *closure, const int32_t *cone, const double *coords,
+                          const int32_t *labels, const int32_t *vertex_perm);
+/* The caller's quadrature-point order of per-point auxiliary fields (call before registering
+ * kernels): our point q is the caller's point quad_perm[q]; per-point aux given to
+ * plx_dm_set_auxiliary and returned by plx_dm_get_auxiliary is in the caller's order (NULL: ours).
+ * Ours: Q1 q = i + 2 j + 4 k at reference coordinates (-1/sqrt3, +1/sqrt3)[i, j, k] (xi fastest),
+ * P2 the 14-point rule's order; a Q1 rule ordered xi slowest (q' = 4 i + 2 j + k) maps by
+ * {0, 4, 2, 6, 1, 5, 3, 7}. */
+int plx_dm_set_point_order(plx_dm dm, int num_quad, const int32_t *quad_perm);
 int plx_dm_set_stream(plx_dm dm, void *hip_stream);
 const char *plx_dm_last_error(plx_dm dm);
 /* Local and global vector lengths (dofs). */

@@ -39,6 +39,9 @@ SIGNATURES = {
     "plx_dm_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _i32p, _i32p,
                                _dblp, _i32p]),
     "plx_dm_destroy": (_c_int, [_vp]),
+    "plx_dm_create_ordered": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _i32p, _i32p,
+                                       _dblp, _i32p, _i32p]),
+    "plx_dm_set_point_order": (_c_int, [_vp, _c_int, _i32p]),
     "plx_dm_set_stream": (_c_int, [_vp, _vp]),
     "plx_dm_last_error": (ctypes.c_char_p, [_vp]),
     "plx_dm_get_sizes": (_c_int, [_vp, _i64p, _i64p]),

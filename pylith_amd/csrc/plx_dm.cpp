@@ -574,6 +574,42 @@ int plx_dm_add_jacobian(plx_dm dm, int label_value, int part, int jf0, int jf1, 
     return 0;
 }
 
+int plx_dm_create_ordered(plx_dm *out, int cell_type, int num_cells, int num_vertices, int num_nodes,
+                          const int32_t *closure, const int32_t *cone, const double *coords, const int32_t *labels,
+                          const int32_t *vertex_perm) {
+    if (!vertex_perm) return plx_dm_create(out, cell_type, num_cells, num_vertices, num_nodes, closure, cone, coords, labels);
+    if (!out || cell_type != PLX_HEX_Q1 || num_cells <= 0 || !closure) return 1;
+    std::vector<char> seen(8, 0);
+    for (int i = 0; i < 8; ++i) {
+        if (vertex_perm[i] < 0 || vertex_perm[i] > 7 || seen[vertex_perm[i]]) return 1;
+        seen[vertex_perm[i]] = 1;
+    }
+    // our vertex i of every cell is the caller's vertex vertex_perm[i] of that cell
+    std::vector<int32_t> cl((size_t)num_cells * 8);
+    for (int c = 0; c < num_cells; ++c)
+        for (int i = 0; i < 8; ++i) cl[(size_t)c * 8 + i] = closure[(size_t)c * 8 + vertex_perm[i]];
+    return plx_dm_create(out, cell_type, num_cells, num_vertices, num_nodes, cl.data(), cl.data(), coords, labels);
+}
+
+int plx_dm_set_point_order(plx_dm dm, int num_quad, const int32_t *quad_perm) {
+    if (!dm) return 1;
+    if (!dm->keys.empty()) return set_error(dm, 1, "set the point order before registering kernels");
+    if (quad_perm) {
+        std::vector<char> seen(dm->nq, 0);
+        if (num_quad != dm->nq) return set_error(dm, 1, "bad quadrature-point permutation");
+        for (int i = 0; i < num_quad; ++i) {
+            if (quad_perm[i] < 0 || quad_perm[i] >= num_quad || seen[quad_perm[i]])
+                return set_error(dm, 1, "bad quadrature-point permutation");
+            seen[quad_perm[i]] = 1;
+        }
+        dm->quad_perm.assign(quad_perm, quad_perm + num_quad);
+    } else {
+        dm->quad_perm.clear();
+    }
+    dm->epoch++, dm->op_epoch++;
+    return 0;
+}
+
 int plx_dm_add_residual_pointfn(plx_dm dm, int label_value, int part, plx_pointfn_launcher residual) {
     if (dm) dm->epoch++, dm->op_epoch++;
     if (!dm || !residual) return set_error(dm, 1, "null launcher");
@@ -621,6 +657,12 @@ int plx_dm_set_auxiliary(plx_dm dm, int label_value, int part, int num_aux, cons
     kp->aux_sizes.assign(aux_sizes, aux_sizes + num_aux);
     kp->aux_per_qp = aux_per_qp;
     kp->aux_host.assign(aux, aux + (size_t)dm->num_cells * per * naux_tot);
+    if (aux_per_qp && !dm->quad_perm.empty())  // caller's point order -> ours
+        for (int c = 0; c < dm->num_cells; ++c)
+            for (int q = 0; q < per; ++q)
+                std::copy(aux + ((size_t)c * per + dm->quad_perm[q]) * naux_tot,
+                          aux + ((size_t)c * per + dm->quad_perm[q] + 1) * naux_tot,
+                          kp->aux_host.begin() + ((size_t)c * per + q) * naux_tot);
     return repack_aux(dm, kp);
 }
 

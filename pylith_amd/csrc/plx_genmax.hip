@@ -297,21 +297,30 @@ int plx_dm_get_auxiliary(plx_dm dm, int label_value, int part, double *aux) {
     if (!dm || !aux) return set_error(dm, 1, "null output");
     KeyPlan *kp = find_key(dm, label_value, part);
     if (!kp || kp->num_aux == 0) return set_error(dm, 1, "no auxiliary field for key");
-    std::copy(kp->aux_host.begin(), kp->aux_host.end(), aux);
-    if (!kp->genmax) return 0;
     const int num_aux = kp->num_aux, nq = dm->nq, n = (int)kp->cells.size();
     int tot = 0;
     for (int i = 0; i < num_aux; ++i) tot += kp->aux_sizes[i];
+    // per-point records back in the caller's point order
+    const bool permute = kp->aux_per_qp && !dm->quad_perm.empty();
+    auto out_point = [&](int c, int q) { return aux + ((size_t)c * nq + (permute ? dm->quad_perm[q] : q)) * tot; };
+    if (permute) {
+        for (int c = 0; c < dm->num_cells; ++c)
+            for (int q = 0; q < nq; ++q)
+                std::copy(kp->aux_host.begin() + ((size_t)c * nq + q) * tot,
+                          kp->aux_host.begin() + ((size_t)c * nq + q + 1) * tot, out_point(c, q));
+    } else {
+        std::copy(kp->aux_host.begin(), kp->aux_host.end(), aux);
+    }
+    if (!kp->genmax) return 0;
     const int o_et = tot - 6, o_vs = tot - 24;
     std::vector<double> state(24 * (size_t)nq * n);
     PLX_HIP(dm, hipStreamSynchronize(dm->stream));
     PLX_HIP(dm, hipMemcpy(state.data(), kp->d_state, state.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int p = 0; p < n; ++p) {
-        double *a = aux + (size_t)kp->cells[p] * nq * tot;
-        for (int q = 0; q < nq; ++q)
-            for (int k = 0; k < 24; ++k)
-                a[(size_t)q * tot + (k < 18 ? o_vs + k : o_et + k - 18)] = state[(size_t)(8 * k + q) * n + p];
-    }
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q < nq; ++q) {
+            double *a = out_point(kp->cells[p], q);
+            for (int k = 0; k < 24; ++k) a[k < 18 ? o_vs + k : o_et + k - 18] = state[(size_t)(8 * k + q) * n + p];
+        }
     return 0;
 }
 

@@ -147,6 +147,9 @@ struct plx_dm_s {
     int num_cells = 0, num_vertices = 0, num_nodes = 0;
     int64_t local_size = 0;
     std::vector<int32_t> closure, cone, labels;
+    // caller's quadrature-point order: our point q is the caller's point quad_perm[q] (empty:
+    // the same order); applied to per-point aux on the way in and out
+    std::vector<int32_t> quad_perm;
     std::vector<double> coords;
     double *d_coords = nullptr;
     int32_t *d_closure = nullptr, *d_cone = nullptr;  // uploaded on first use (set-up kernels)

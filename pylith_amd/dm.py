@@ -45,7 +45,8 @@ def _stype(t):
 class DM:
     """Mesh + section + weak form on the device (one stream, one host thread)."""
 
-    def __init__(self, mesh, labels=None, stream=None):
+    def __init__(self, mesh, labels=None, stream=None, vertex_perm=None):
+        """vertex_perm: the caller's vertex order of the Q1 cells (plx_dm_create_ordered)."""
         self.lib = L.load()
         self.mesh = mesh
         h = ctypes.c_void_p()
@@ -57,8 +58,13 @@ class DM:
             lb, lbp = _i32(lab)
         else:
             lbp = None
-        err = self.lib.plx_dm_create(ctypes.byref(h), mesh.cell_type, mesh.num_cells,
-                                     mesh.num_vertices, mesh.num_nodes, clp, cop, xyp, lbp)
+        if vertex_perm is None:
+            err = self.lib.plx_dm_create(ctypes.byref(h), mesh.cell_type, mesh.num_cells,
+                                         mesh.num_vertices, mesh.num_nodes, clp, cop, xyp, lbp)
+        else:
+            vp, vpp = _i32(vertex_perm)
+            err = self.lib.plx_dm_create_ordered(ctypes.byref(h), mesh.cell_type, mesh.num_cells,
+                                                 mesh.num_vertices, mesh.num_nodes, clp, cop, xyp, lbp, vpp)
         if err:
             raise PlxError(f"plx_dm_create failed with code {err}")
         self.h = h
@@ -92,6 +98,12 @@ class DM:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.lib.plx_dm_get_sizes(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_point_order(self, quad_perm=None):
+        """The caller's quadrature-point order of per-point aux (plx_dm_set_point_order)."""
+        qp = None if quad_perm is None else _i32(quad_perm)
+        self._check(self.lib.plx_dm_set_point_order(self.h, 0 if qp is None else qp[0].size,
+                                                    None if qp is None else qp[1]))
 
     def set_kernel_options(self, algorithm=L.ALGO_AUTO, accumulate=L.ACCUMULATE_AUTO, min_waves=0):
         self._check(self.lib.plx_dm_set_kernel_options(self.h, algorithm, accumulate, min_waves))

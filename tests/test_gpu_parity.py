@@ -335,3 +335,50 @@ def test_badly_numbered_meshes_are_tiled_in_morton_order(oracle, kind):
     torch.cuda.synchronize()
     assert relerr(r.cpu().numpy(), ref) <= TOL64
     dm.close()
+
+
+def test_dmplex_vertex_order_and_tensor_point_order(oracle):
+    """PyLith builds its DM from cells inverted by DMPlexInvertCell (meshio/MeshBuilder.cc:151:
+    hexahedron vertices 1 and 3 swapped) and may hand per-point aux in a tensor rule's order with
+    xi slowest.  The reference fixture mesh in that vertex order is refused as inverted cells
+    without the order and accepted with it; per-point generalized Maxwell state that varies inside
+    each cell, given in the xi-slowest order, gives the oracle's residual, and the updated state
+    comes back in the caller's order."""
+    import dataclasses
+    from pylith_amd.dm import DM, PlxError
+    m, groups = meshio_ascii.read(os.path.join(GOLDEN, "hex.mesh"))
+    inv = np.ascontiguousarray(m.closure[:, [0, 3, 2, 1, 4, 5, 6, 7]])  # DMPlexInvertCell
+    m_dm = dataclasses.replace(m, closure=inv, cone=inv.copy())
+    with pytest.raises(PlxError):
+        DM(m_dm)  # reflected cells: non-positive Jacobians
+    vperm = [0, 3, 2, 1, 4, 5, 6, 7]
+    qperm = np.array([0, 4, 2, 6, 1, 5, 3, 7])  # our q = i + 2j + 4k is the caller's 4i + 2j + k
+    rng = np.random.default_rng(31)
+    aux, sizes = mg.genmax_aux(m.num_cells, 8, seed=4, log_eta_range=(1e17, 1e19), ratios=(0.3, 0.2, 0.1))
+    aux[:, 9:33] = rng.standard_normal((aux.shape[0], 24)) * 1e-4  # state varies point by point
+    caller = aux.reshape(m.num_cells, 8, -1)[:, np.argsort(qperm)].reshape(aux.shape)
+    assert np.array_equal(caller.reshape(m.num_cells, 8, -1)[:, qperm], aux.reshape(m.num_cells, 8, -1))
+    dt = 0.1 * mg.YEAR
+    dm = DM(m_dm, vertex_perm=vperm)
+    dm.set_point_order(qperm)
+    dm.add_residual(24, 0, 0, 3)
+    dm.add_jacobian(24, 0, 0, 0, 0, 4)
+    dm.set_auxiliary(24, 0, np.ascontiguousarray(caller), sizes, per_qp=True)
+    dm.set_constants(24, 0, [dt])
+    u = mg.random_field(m.num_nodes, 12)
+    cells = np.nonzero(m.labels == 24)[0]
+    P = oracle.Problem(m, aux, sizes, per_qp=True, constants=[dt])
+    ref = P.residual(u, 3, cells=cells)
+    r = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+    dm.compute_residual(24, 0, dev(u), r, mode=1)
+    torch.cuda.synchronize()
+    assert relerr(r.cpu().numpy(), ref) <= TOL64
+    # the state update returns per-point records in the caller's order
+    dm.update_state_vars(24, 0, dev(u))
+    got = dm.get_auxiliary(24, 0, caller)
+    new = P.update_state(u, 5, 5)
+    new[:, 27:33] = P.update_state(u, 6, 6)[:, 27:33]
+    new_caller = new.reshape(m.num_cells, 8, -1)[:, np.argsort(qperm)].reshape(new.shape)
+    sel = np.repeat(np.isin(np.arange(m.num_cells), cells), 8)
+    assert relerr(got[sel][:, 9:33], new_caller[sel][:, 9:33]) <= TOL64
+    dm.close()
