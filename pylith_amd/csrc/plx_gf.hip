@@ -653,17 +653,35 @@ __global__ void __launch_bounds__(256) gf_resid(int64_t n, const double *__restr
         R[t] = B[t] - R[t];
 }
 
-// column c of a block <-> a local vector (first ndof entries)
-__global__ void __launch_bounds__(256) gf_pack(int64_t ndof, int k, int c, const double *__restrict__ v,
-                                               double *blk) {
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ndof; t += (int64_t)gridDim.x * 256)
-        blk[t * k + c] = v[t];
+// column c of a block (handle row order) <-> a local vector (DM order, first ndof entries);
+// perm[row] is the DM node of a handle row
+__global__ void __launch_bounds__(256) gf_pack(int64_t ndof, int k, int c, const int32_t *__restrict__ perm,
+                                               const double *__restrict__ v, double *blk) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ndof; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / 3;
+        blk[t * k + c] = v[3 * (int64_t)perm[i] + (t - 3 * i)];
+    }
 }
 
-__global__ void __launch_bounds__(256) gf_unpack(int64_t ndof, int k, int c, const double *__restrict__ blk,
-                                                 double *v) {
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ndof; t += (int64_t)gridDim.x * 256)
-        v[t] = blk[t * k + c];
+__global__ void __launch_bounds__(256) gf_unpack(int64_t ndof, int k, int c, const int32_t *__restrict__ perm,
+                                                 const double *__restrict__ blk, double *v) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ndof; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / 3;
+        v[3 * (int64_t)perm[i] + (t - 3 * i)] = blk[t * k + c];
+    }
+}
+
+// [dof][k] blocks between DM order and handle row order (to_rows: DM -> handle)
+template <typename T>
+__global__ void __launch_bounds__(256) gf_permute_block(int64_t ndof, int k, const int32_t *__restrict__ perm,
+                                                        bool to_rows, const T *__restrict__ A, T *B) {
+    const int64_t n = ndof * k;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int64_t d = t / k, i = d / 3;
+        const int64_t o = (3 * (int64_t)perm[i] + (d - 3 * i)) * k + (t - d * k);
+        if (to_rows) B[t] = A[o];
+        else B[o] = A[t];
+    }
 }
 
 // v[pos node of i] = jump[i] (v zero elsewhere beforehand)
@@ -695,6 +713,10 @@ struct plx_gf_s {
     int nrows = 0;
     int64_t ndof = 0, nnzb = 0;
     int32_t *d_row_ptr = nullptr, *d_col = nullptr;
+    // rows are the merged nodes in Morton order of their coordinates (neighbouring rows then share
+    // cache lines of the neighbour gathers): perm[row] is the DM node of a row
+    std::vector<int32_t> perm;
+    int32_t *d_perm = nullptr;
     double *d_val64 = nullptr;
     float *d_val32 = nullptr;
     double *d_Binv64 = nullptr;
@@ -725,6 +747,7 @@ struct plx_gf_s {
     std::vector<Level> lv;
     int coarse_m = 0;
     float *coarse_inv = nullptr, *cb = nullptr, *cx = nullptr;
+    float *vr1 = nullptr, *vz1 = nullptr;  // single-column V-cycle operands in row order
     int amg_kmax = 0;
     uint64_t op_epoch = 0;  // the DM's operator epoch the assembled copies were made at
     // continuous batching (plx_gf_slots_*): per-column solves that start and retire on their own
@@ -771,17 +794,23 @@ static void gf_spmm_launch(plx_gf g, int k, const Real *val, const Real *X, Real
                        g->d_row_ptr, g->d_col, val, X, Y);
 }
 
+// q = A p and the per-column partials of p.q for the k-column block
+template <int K>
+static void gf_spmm_dot_launch(plx_gf g) {
+    if (K == 64)
+        hipLaunchKernelGGL(gfk_spmm_dot64, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr, g->d_col,
+                           g->d_val32, g->P32, g->Q32, g->prz);
+    else
+        hipLaunchKernelGGL(gfk_spmm_dot<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr, g->d_col,
+                           g->d_val32, g->P32, g->Q32, g->prz);
+}
+
 // Ten iterations of the FP32 block PCG (p, z, scalars carried on the device): per iteration
 // q = A p with p.q fused, the x/r/z update with r.z and r.r fused, and p = z + beta p.
 template <int K>
 static void gf_pcg_chunk_k(plx_gf g) {
     for (int it = 0; it < 10; ++it) {
-        if (K == 64)
-            hipLaunchKernelGGL(gfk_spmm_dot64, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
-                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
-        else
-            hipLaunchKernelGGL(gfk_spmm_dot<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
-                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
+        gf_spmm_dot_launch<K>(g);
         hipLaunchKernelGGL(gf_finish_alpha, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->scal);
         hipLaunchKernelGGL((gfk_update<K, false>), dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_Binv32,
                            g->scal, g->X32, g->R32, g->P32, g->Q32, g->Z32, g->prz, g->prr);
@@ -908,12 +937,7 @@ constexpr int kAmgChunk = 2;
 template <int K>
 static void gf_pcg_chunk_amg(plx_gf g) {
     for (int it = 0; it < kAmgChunk; ++it) {
-        if (K == 64)
-            hipLaunchKernelGGL(gfk_spmm_dot64, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
-                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
-        else
-            hipLaunchKernelGGL(gfk_spmm_dot<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->nrows, g->d_row_ptr,
-                               g->d_col, g->d_val32, g->P32, g->Q32, g->prz);
+        gf_spmm_dot_launch<K>(g);
         hipLaunchKernelGGL(gf_finish_alpha, dim3(K), dim3(256), 0, g->stream, K, g->prz, g->scal);
         hipLaunchKernelGGL(gfk_update_xr<K>, dim3(kGfRed), dim3(256), 0, g->stream, g->ndof, g->scal, g->X32, g->R32,
                            g->P32, g->Q32, g->prr);
@@ -1035,9 +1059,9 @@ static void gf_amg_free(plx_gf g) {
                         (void *)L.bv, (void *)L.xv})
             if (p) (void)hipFree(p);
     g->lv.clear();
-    for (void *p : {(void *)g->coarse_inv, (void *)g->cb, (void *)g->cx})
+    for (void *p : {(void *)g->coarse_inv, (void *)g->cb, (void *)g->cx, (void *)g->vr1, (void *)g->vz1})
         if (p) (void)hipFree(p);
-    g->coarse_inv = g->cb = g->cx = nullptr;
+    g->coarse_inv = g->cb = g->cx = g->vr1 = g->vz1 = nullptr;
     g->coarse_m = 0;
 }
 
@@ -1066,9 +1090,13 @@ static int gf_amg_setup(plx_gf g) {
                 }
             fixed[3 * (size_t)i + c] = ident;
         }
+    // node coordinates in row order (the aggregation's rigid-body modes)
+    std::vector<double> xyz(3 * (size_t)nn);
+    for (int i = 0; i < nn; ++i)
+        for (int d = 0; d < 3; ++d) xyz[3 * (size_t)i + d] = dm->coords[3 * (size_t)g->perm[i] + d];
     AmgHierarchy h;
     std::string msg;
-    if (amg_setup(nn, ptr.data(), col.data(), val.data(), dm->coords.data(), fixed.data(), 768, 8, &h, &msg))
+    if (amg_setup(nn, ptr.data(), col.data(), val.data(), xyz.data(), fixed.data(), 768, 8, &h, &msg))
         return set_error(dm, 1, msg.c_str());
     gf_amg_free(g);
     auto upf = [&](float **d, const std::vector<double> &v) -> int {
@@ -1108,6 +1136,8 @@ static int gf_amg_setup(plx_gf g) {
     g->coarse_m = h.coarse_n;
     if ((err = upf(&g->coarse_inv, h.coarse_inv)) || (err = ws(&g->cb, h.coarse_n)) || (err = ws(&g->cx, h.coarse_n)))
         return err;
+    PLX_HIP(dm, hipMalloc((void **)&g->vr1, (size_t)g->ndof * sizeof(float)));
+    PLX_HIP(dm, hipMalloc((void **)&g->vz1, (size_t)g->ndof * sizeof(float)));
     return 0;
 }
 
@@ -1116,13 +1146,17 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
              const std::vector<int32_t> &mcl, const std::vector<int32_t> &diag_pos,
              const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos);
 
-// One V-cycle on a single column, z = M r (3 x num_nodes FP32 vectors, merged layout), queued on
-// `stream`; used by the matrix-free solver as its multigrid preconditioner.
+// One V-cycle on a single column, z = M r (3 x num_nodes FP32 vectors in DM node order), queued
+// on `stream`; used by the matrix-free solver as its multigrid preconditioner.
 int gf_vcycle1(plx_gf g, hipStream_t stream, const float *r, float *z) {
     if (g->lv.empty()) return set_error(g->dm, 1, "multigrid hierarchy not built");
     hipStream_t saved = g->stream;
     g->stream = stream;
-    amg_vcycle<1>(g, 0, r, z);
+    hipLaunchKernelGGL(gf_permute_block<float>, gf_grid(g->ndof), dim3(256), 0, stream, g->ndof, 1, g->d_perm, true,
+                       r, g->vr1);
+    amg_vcycle<1>(g, 0, g->vr1, g->vz1);
+    hipLaunchKernelGGL(gf_permute_block<float>, gf_grid(g->ndof), dim3(256), 0, stream, g->ndof, 1, g->d_perm, false,
+                       (const float *)g->vz1, z);
     g->stream = saved;
     PLX_HIP(g->dm, hipGetLastError());
     return 0;
@@ -1151,28 +1185,41 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     }
     if (allcells.size() != (size_t)dm->num_cells) return set_error(dm, 1, "the keys must cover every cell");
     const int nn = dm->num_nodes, nc = (int)allcells.size();
+    // handle rows: the DM's nodes in Morton order of their coordinates (Q1: the nodes are the
+    // vertices), perm[row] = node, iperm[node] = row
+    std::vector<int32_t> perm, iperm(nn);
+    if (dm->coords.size() == 3 * (size_t)nn) {
+        perm = morton_permutation(dm->coords.data(), nn);
+    } else {
+        perm.resize(nn);
+        for (int i = 0; i < nn; ++i) perm[i] = i;
+    }
+    for (int i = 0; i < nn; ++i) iperm[perm[i]] = i;
+    // merged[row]: the row a node's equations go to (a positive split node's go to its negative
+    // twin's row; its own row keeps only the identity)
     std::vector<int32_t> merged(nn);
     std::vector<uint8_t> is_pos(nn, 0);
     for (int i = 0; i < nn; ++i) merged[i] = i;
     if (dm->fault) {  // without a fault the merged operator is the masked stiffness itself
         const FaultData &F = *dm->fault;
         for (int l = 0; l < F.num_lag; ++l) {
-            merged[F.pos_node[l]] = F.neg_node[l];
-            is_pos[F.pos_node[l]] = 1;
+            merged[iperm[F.pos_node[l]]] = iperm[F.neg_node[l]];
+            is_pos[iperm[F.pos_node[l]]] = 1;
         }
     }
     std::vector<uint8_t> conmask(3 * (size_t)nn, 0);
     for (int32_t d : dm->con.dofs) {
         if (d >= 3 * nn) return set_error(dm, 1, "constrained Lagrange-multiplier dofs are not supported");
-        if (is_pos[d / 3]) return set_error(dm, 1, "constrain the negative-side twin of a split fault node");
-        conmask[d] = 1;
+        const int row = iperm[d / 3];
+        if (is_pos[row]) return set_error(dm, 1, "constrain the negative-side twin of a split fault node");
+        conmask[3 * (size_t)row + d % 3] = 1;
     }
-    // merged closures of the key cells (key order) and node -> (cell, vertex) incidence
+    // merged closures of the key cells (key order, in rows) and row -> (cell, vertex) incidence
     std::vector<int32_t> mcl(8 * (size_t)nc);
     std::vector<int32_t> inc_ptr(nn + 1, 0), inc;
     for (int p = 0; p < nc; ++p)
         for (int a = 0; a < 8; ++a) {
-            mcl[8 * (size_t)p + a] = merged[dm->closure[8 * (size_t)allcells[p] + a]];
+            mcl[8 * (size_t)p + a] = merged[iperm[dm->closure[8 * (size_t)allcells[p] + a]]];
             inc_ptr[mcl[8 * (size_t)p + a] + 1]++;
         }
     for (int i = 0; i < nn; ++i) inc_ptr[i + 1] += inc_ptr[i];
@@ -1209,6 +1256,7 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     g->ndof = 3 * (int64_t)nn;
     g->nnzb = (int64_t)col.size();
     g->op_epoch = dm->op_epoch;
+    g->perm = std::move(perm);
     const int err = gf_setup(g, nc, row_ptr, col, inc_ptr, inc, mcl, diag_pos, conmask, is_pos);
     if (err) {
         plx_gf_destroy(g);
@@ -1238,7 +1286,8 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
     uint8_t *d_con = nullptr, *d_pos = nullptr;
     double *d_Ke = nullptr;
     int err = 0;
-    if ((err = up(&g->d_row_ptr, row_ptr)) || (err = up(&g->d_col, col)) || (err = up(&d_inc_ptr, inc_ptr)) ||
+    if ((err = up(&g->d_row_ptr, row_ptr)) || (err = up(&g->d_col, col)) || (err = up(&g->d_perm, g->perm)) ||
+        (err = up(&d_inc_ptr, inc_ptr)) ||
         (err = up(&d_inc, inc)) || (err = up(&d_mcl, mcl)) || (err = up(&d_diag, diag_pos)) ||
         (err = up(&d_con, conmask)) || (err = up(&d_pos, is_pos)))
         return err;
@@ -1285,7 +1334,7 @@ int plx_gf_destroy(plx_gf g) {
     if (!g) return 0;
     for (int i = 0; i <= 64; ++i)
         if (g->graph[i]) (void)hipGraphExecDestroy(g->graph[i]);
-    void *ptrs[] = {g->d_row_ptr, g->d_col, g->d_val64, g->d_val32, g->d_Binv64, g->d_Binv32, g->B64, g->X64,
+    void *ptrs[] = {g->d_perm, g->d_row_ptr, g->d_col, g->d_val64, g->d_val32, g->d_Binv64, g->d_Binv32, g->B64, g->X64,
                     g->R64,       g->X32,   g->R32,     g->P32,     g->Q32,      g->Z32,      g->v1,  g->v2,
                     g->jump,      g->scal,  g->prz,     g->prr,     g->nrm,      g->d_rr0,    g->d_slot_slip};
     for (void *p : ptrs)
@@ -1336,12 +1385,32 @@ int plx_gf_apply(plx_gf g, int k, int scalar_type, const void *X, void *Y) {
     plx_dm dm = g->dm;
     if (g->op_epoch != dm->op_epoch)
         return set_error(dm, 1, "the DM's operator changed after plx_gf_create: destroy and recreate the handle");
-    if (scalar_type == PLX_FP64)
-        gf_spmm_launch<double>(g, k, g->d_val64, (const double *)X, (double *)Y);
-    else
-        gf_spmm_launch<float>(g, k, g->d_val32, (const float *)X, (float *)Y);
-    PLX_HIP(dm, hipGetLastError());
-    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    // X and Y are in DM order: through row-ordered copies (a verification path, not a hot one)
+    const size_t es = scalar_type == PLX_FP64 ? sizeof(double) : sizeof(float);
+    void *Xr = nullptr, *Yr = nullptr;
+    PLX_HIP(dm, hipMalloc(&Xr, (size_t)g->ndof * k * es));
+    if (hipMalloc(&Yr, (size_t)g->ndof * k * es) != hipSuccess) {
+        (void)hipFree(Xr);
+        return set_error(dm, 1, "out of device memory");
+    }
+    if (scalar_type == PLX_FP64) {
+        hipLaunchKernelGGL(gf_permute_block<double>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k,
+                           g->d_perm, true, (const double *)X, (double *)Xr);
+        gf_spmm_launch<double>(g, k, g->d_val64, (const double *)Xr, (double *)Yr);
+        hipLaunchKernelGGL(gf_permute_block<double>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k,
+                           g->d_perm, false, (const double *)Yr, (double *)Y);
+    } else {
+        hipLaunchKernelGGL(gf_permute_block<float>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k,
+                           g->d_perm, true, (const float *)X, (float *)Xr);
+        gf_spmm_launch<float>(g, k, g->d_val32, (const float *)Xr, (float *)Yr);
+        hipLaunchKernelGGL(gf_permute_block<float>, gf_grid(g->ndof * k), dim3(256), 0, g->stream, g->ndof, k,
+                           g->d_perm, false, (const float *)Yr, (float *)Y);
+    }
+    const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(g->stream);
+    (void)hipFree(Xr);
+    (void)hipFree(Yr);
+    PLX_HIP(dm, e1);
+    PLX_HIP(dm, e2);
     return 0;
 }
 
@@ -1440,7 +1509,7 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
             if ((err = launch_fold(dm, PLX_FP64, g->v2, kFold))) break;
             if (nc > 0)
                 hipLaunchKernelGGL(gf_mask, dim3((nc + 255) / 256), dim3(256), 0, g->stream, nc, dm->con.d_dofs, g->v2);
-            hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->v2, g->B64);
+            hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->d_perm, g->v2, g->B64);
         }
         if (err) break;
         if (hipGetLastError() != hipSuccess) {
@@ -1452,7 +1521,7 @@ int plx_gf_solve(plx_gf g, int num_impulses, const double *slip, double *u, doub
         for (int c = 0; c < kb && !err; ++c) {
             double *uj = u + (size_t)(j0 + c) * nloc;
             hipLaunchKernelGGL(gf_fill, gf_grid(nloc), dim3(256), 0, g->stream, nloc, uj, 0.0);
-            hipLaunchKernelGGL(gf_unpack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->X64, uj);
+            hipLaunchKernelGGL(gf_unpack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->d_perm, g->X64, uj);
             if ((err = launch_planar_jump(dm, F.num_lag, slip + (size_t)(j0 + c) * nl3, g->jump))) break;
             err = launch_sync_pos(dm, g->jump, uj);
         }
@@ -1603,7 +1672,7 @@ static int gf_rhs_column(plx_gf g, const double *slip, int c, int k) {
     }
     if ((err = launch_fold(dm, PLX_FP64, g->v2, kFold))) return err;
     if (nc > 0) hipLaunchKernelGGL(gf_mask, dim3((nc + 255) / 256), dim3(256), 0, g->stream, nc, dm->con.d_dofs, g->v2);
-    hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->v2, g->B64);
+    hipLaunchKernelGGL(gf_pack, gf_grid(ndof), dim3(256), 0, g->stream, ndof, k, c, g->d_perm, g->v2, g->B64);
     PLX_HIP(dm, hipGetLastError());
     return 0;
 }
@@ -1838,7 +1907,7 @@ int plx_gf_slot_take(plx_gf g, int slot, double *u, plx_gf_slot_stats *stats) {
     if (u) {
         // u = U w + u_jump (Lagrange multipliers left at zero)
         hipLaunchKernelGGL(gf_fill, gf_grid(dm->local_size), dim3(256), 0, g->stream, dm->local_size, u, 0.0);
-        hipLaunchKernelGGL(gf_unpack, gf_grid(g->ndof), dim3(256), 0, g->stream, g->ndof, k, slot, g->X64, u);
+        hipLaunchKernelGGL(gf_unpack, gf_grid(g->ndof), dim3(256), 0, g->stream, g->ndof, k, slot, g->d_perm, g->X64, u);
         if ((err = launch_planar_jump(dm, dm->fault->num_lag, g->d_slot_slip + slot * nl3, g->jump))) return err;
         if ((err = launch_sync_pos(dm, g->jump, u))) return err;
     }

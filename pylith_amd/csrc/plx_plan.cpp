@@ -12,22 +12,15 @@
 
 namespace plx {
 
-// Morton (Z-order) rank of each cell's centroid on a 2^21 grid over the cells' bounding box (a
-// finer grid than the cells, so cells sharing a grid point are rare).
-static std::vector<int32_t> morton_order(const MeshView &m, const std::vector<int32_t> &cells) {
-    const size_t n = cells.size();
-    std::vector<double> cen(3 * n);
+// Morton (Z-order) permutation of n points on a 2^21 grid over their bounding box (a finer grid
+// than the mesh spacing, so points sharing a grid point are rare; ties keep input order).
+std::vector<int32_t> morton_permutation(const double *xyz, size_t n) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (size_t i = 0; i < n; ++i) {
-        const int32_t *cn = m.cone + (size_t)cells[i] * m.nv;
+    for (size_t i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) {
-            double acc = 0.0;
-            for (int v = 0; v < m.nv; ++v) acc += m.coords[3 * (size_t)cn[v] + d];
-            cen[3 * i + d] = acc / m.nv;
-            lo[d] = std::min(lo[d], cen[3 * i + d]);
-            hi[d] = std::max(hi[d], cen[3 * i + d]);
+            lo[d] = std::min(lo[d], xyz[3 * i + d]);
+            hi[d] = std::max(hi[d], xyz[3 * i + d]);
         }
-    }
     const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
     auto spread = [](uint64_t x) {  // 21 bits -> every third bit
         x &= 0x1fffff;
@@ -42,10 +35,10 @@ static std::vector<int32_t> morton_order(const MeshView &m, const std::vector<in
     for (size_t i = 0; i < n; ++i) {
         uint64_t code = 0;
         for (int d = 0; d < 3; ++d) {
-            const double t = (cen[3 * i + d] - lo[d]) / span;
+            const double t = (xyz[3 * i + d] - lo[d]) / span;
             code |= spread((uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0))) << d;
         }
-        key[i] = {code, cells[i]};
+        key[i] = {code, (int32_t)i};
     }
     std::stable_sort(key.begin(), key.end(),
                      [](const std::pair<uint64_t, int32_t> &a, const std::pair<uint64_t, int32_t> &b) {
@@ -53,6 +46,23 @@ static std::vector<int32_t> morton_order(const MeshView &m, const std::vector<in
                      });
     std::vector<int32_t> out(n);
     for (size_t i = 0; i < n; ++i) out[i] = key[i].second;
+    return out;
+}
+
+// The cells in Morton order of their centroids.
+static std::vector<int32_t> morton_order(const MeshView &m, const std::vector<int32_t> &cells) {
+    const size_t n = cells.size();
+    std::vector<double> cen(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t *cn = m.cone + (size_t)cells[i] * m.nv;
+        for (int d = 0; d < 3; ++d) {
+            double acc = 0.0;
+            for (int v = 0; v < m.nv; ++v) acc += m.coords[3 * (size_t)cn[v] + d];
+            cen[3 * i + d] = acc / m.nv;
+        }
+    }
+    std::vector<int32_t> rank = morton_permutation(cen.data(), n), out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = cells[rank[i]];
     return out;
 }
 

@@ -51,6 +51,9 @@ struct HostPlan {
 // When the key's cells in input order tile badly (more than 1.5 halo slots per cell, e.g. a mesh
 // numbered row by row), the cells are also tiled in Morton order of their centroids and the plan
 // with fewer halo slots is kept.
+// Permutation (new position -> point) listing n points [n][3] in Morton (Z-curve) order.
+std::vector<int32_t> morton_permutation(const double *xyz, size_t n);
+
 int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
                     std::string *err, bool sort_by_color = true);
 
