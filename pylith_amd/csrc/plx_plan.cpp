@@ -3,9 +3,10 @@
 // The key's cells (PyLith's DSLabelAccess pointsIS: cells with the key's label value,
 // feassemble/DSLabelAccess.icc:53-84) are cut, in cell order, into tiles bounded by the
 // workgroup size and the LDS capacity.  Per tile: greedy cell colouring (cells sharing a node
-// get different colours; cells sorted by colour), the node list with owned nodes first, 16-bit
-// local closures, and halo slots for nodes other tiles also touch.
+// get different colours; cells sorted by colour), the node list with owned nodes first (numbered
+// for LDS bank balance), 16-bit local closures, and halo slots for nodes other tiles also touch.
 #include <algorithm>
+#include <array>
 #include <cmath>
 
 #include "plx_plan.hpp"
@@ -68,6 +69,98 @@ static std::vector<int32_t> morton_order(const MeshView &m, const std::vector<in
 
 static int plan_cells(const MeshView &m, const TileLimits &lim, HostPlan *hp, std::string *err,
                       bool sort_by_color);
+
+// LDS bank balance of a tile's local numbering.  The cell kernels address the staged arrays as
+// [3 k + c] doubles, so an 8-byte access to local index k falls on bank pair (3 k + c) mod 16 of
+// the 32 four-byte banks, which k mod 16 fixes.  A half-wave (32 consecutive cells of the tile)
+// accesses slot b of its 32 cells in one instruction and takes as many cycles as its most loaded
+// pair.  `ids` are the tile's items (nodes or vertices), `ranges` their position ranges (owned,
+// then shared: an item keeps its range), and slot[b][i] the tile-local item of slot b of cell i.
+// Items get residues mod 16 that spread every such access, greedily in order of decreasing use
+// (each takes the residue its range still has room for that adds least to the summed squared
+// loads of its accesses).  On return `ids` is reordered so that position p holds an item of
+// residue p mod 16.
+constexpr int kFixShared = 1;
+static void balance_banks(std::vector<int32_t> &ids, const std::vector<int> &ranges,
+                          const std::vector<std::vector<int32_t>> &slot, int ncell, int nfixed = 0) {
+    const int n = (int)ids.size();
+    if (n <= 16) return;
+    // accesses: (half-wave, slot) -> its distinct items
+    std::vector<int32_t> acc_ptr(1, 0), acc_items, item_acc_ptr(n + 1, 0), item_acc;
+    std::vector<int32_t> seen(n, -1);
+    int nacc = 0;
+    for (int g = 0; g < ncell; g += 32)
+        for (size_t b = 0; b < slot.size(); ++b, ++nacc) {
+            for (int i = g; i < std::min(ncell, g + 32); ++i) {
+                const int32_t k = slot[b][i];
+                if (seen[k] != nacc) {
+                    seen[k] = nacc;
+                    acc_items.push_back(k);
+                    item_acc_ptr[k + 1]++;
+                }
+            }
+            acc_ptr.push_back((int32_t)acc_items.size());
+        }
+    for (int k = 0; k < n; ++k) item_acc_ptr[k + 1] += item_acc_ptr[k];
+    item_acc.resize(acc_items.size());
+    {
+        std::vector<int32_t> fill(item_acc_ptr.begin(), item_acc_ptr.end() - 1);
+        for (int a = 0; a < nacc; ++a)
+            for (int j = acc_ptr[a]; j < acc_ptr[a + 1]; ++j) item_acc[fill[acc_items[j]]++] = a;
+    }
+    // residue capacities of each position range, the item's range
+    const int nr = (int)ranges.size() - 1;
+    std::vector<std::array<int, 16>> cap(nr);
+    std::vector<int> range_of(n);
+    for (int q = 0; q < nr; ++q) {
+        cap[q].fill(0);
+        for (int p = ranges[q]; p < ranges[q + 1]; ++p) {
+            cap[q][p & 15]++;
+            range_of[p] = q;
+        }
+    }
+    std::vector<std::array<int16_t, 16>> load(nacc);
+    for (auto &l : load) l.fill(0);
+    std::vector<int> res(n, -1), order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    // the last nfixed ranges keep their positions (residue p mod 16)
+    for (int k = 0; k < n; ++k)
+        if (range_of[k] >= nr - nfixed) {
+            res[k] = k & 15;
+            cap[range_of[k]][k & 15]--;
+            for (int j = item_acc_ptr[k]; j < item_acc_ptr[k + 1]; ++j) load[item_acc[j]][k & 15]++;
+        }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return item_acc_ptr[a + 1] - item_acc_ptr[a] > item_acc_ptr[b + 1] - item_acc_ptr[b];
+    });
+    for (int k : order) {
+        if (res[k] >= 0) continue;
+        const int q = range_of[k];
+        int best = -1, best_cost = 0;
+        for (int r = 0; r < 16; ++r) {
+            if (cap[q][r] == 0) continue;
+            int cost = 0;
+            for (int j = item_acc_ptr[k]; j < item_acc_ptr[k + 1]; ++j) cost += 2 * load[item_acc[j]][r] + 1;
+            if (best < 0 || cost < best_cost) {
+                best = r;
+                best_cost = cost;
+            }
+        }
+        res[k] = best;
+        cap[q][best]--;
+        for (int j = item_acc_ptr[k]; j < item_acc_ptr[k + 1]; ++j) load[item_acc[j]][best]++;
+    }
+    // positions: within each range, residue r takes positions p = r (mod 16)
+    std::vector<int32_t> out(n);
+    for (int q = 0; q < nr; ++q) {
+        std::array<std::vector<int>, 16> byres;
+        for (int k = 0; k < n; ++k)
+            if (range_of[k] == q) byres[res[k]].push_back(k);
+        std::array<size_t, 16> next{};
+        for (int p = ranges[q]; p < ranges[q + 1]; ++p) out[p] = ids[byres[p & 15][next[p & 15]++]];
+    }
+    ids.swap(out);
+}
 
 int build_host_plan(const MeshView &m, int label_value, const TileLimits &lim, HostPlan *hp,
                     std::string *err, bool sort_by_color) {
@@ -229,13 +322,30 @@ static int plan_cells(const MeshView &m, const TileLimits &lim, HostPlan *hp, st
         std::sort(own.begin(), own.end());
         std::sort(sh.begin(), sh.end());
         std::sort(vl.begin(), vl.end());
-        int k = 0;
-        for (int32_t n : own) {
-            local_of[n] = k++;
-            hp->tile_nodes.push_back(n);
+        const int c0 = tile_cell_ptr[t], tc = tile_cell_ptr[t + 1] - c0;
+        // local numbering: owned nodes, then shared ones.  Colour-sorted tiles put scattered cells in
+        // each half-wave: their owned nodes are numbered for LDS bank balance around the shared
+        // ones, which keep node order (the halo reduction reads their partials in node order).
+        // Tiles in mesh order keep node order: neighbouring lanes already touch spread-out banks.
+        std::vector<int32_t> ids(own);
+        ids.insert(ids.end(), sh.begin(), sh.end());
+        if (sort_by_color) {
+            for (size_t q = 0; q < ids.size(); ++q) local_of[ids[q]] = (int32_t)q;
+            std::vector<std::vector<int32_t>> slot(nb, std::vector<int32_t>(tc));
+            for (int i = 0; i < tc; ++i)
+                for (int b = 0; b < nb; ++b) slot[b][i] = local_of[m.closure[(size_t)cells[c0 + i] * nb + b]];
+            balance_banks(ids, {0, (int)own.size(), (int)ids.size()}, slot, tc, kFixShared);
         }
-        for (int32_t n : sh) {
-            shared_pairs.emplace_back(n, (int32_t)(halo + (k - (int)own.size())));
+        if (sort_by_color && !m.cone_is_closure && vl.size() > 16) {
+            for (size_t q = 0; q < vl.size(); ++q) vlocal_of[vl[q]] = (int32_t)q;
+            std::vector<std::vector<int32_t>> slot(nv, std::vector<int32_t>(tc));
+            for (int i = 0; i < tc; ++i)
+                for (int b = 0; b < nv; ++b) slot[b][i] = vlocal_of[m.cone[(size_t)cells[c0 + i] * nv + b]];
+            balance_banks(vl, {0, (int)vl.size()}, slot, tc);
+        }
+        int k = 0;
+        for (int32_t n : ids) {
+            if (k >= (int)own.size()) shared_pairs.emplace_back(n, (int32_t)(halo + (k - (int)own.size())));
             local_of[n] = k++;
             hp->tile_nodes.push_back(n);
         }
