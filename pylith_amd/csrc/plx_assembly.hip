@@ -31,6 +31,7 @@ static std::mutex g_device_mu;
 struct DevTab {
     __device__ static double w(int q) { return c_p2.w[q]; }
     __device__ static double dphi(int q, int b, int k) { return c_p2.dphi[q][b][k]; }
+    __device__ static double lam(int q, int v) { return c_p2.lam[q][v]; }
 };
 
 // Called with the DM's device current (plx_dm_create).

@@ -330,8 +330,65 @@ __host__ __device__ inline void q1_element(const double X[3][8], const Real U[3]
     }
 }
 
-// Element vector of one P2 tet (affine map, 14-point rule): re[c][b].  Tab::w(q) and
-// Tab::dphi(q, b, k) give the reference tables (constant memory on the device).
+// Reference gradients of a P2 field at the four vertices, one solution component at a time:
+// S = (u1-u0, u2-u0, u3-u0), G_v = 4 d_v - S with d_0 = (u01-u0, u02-u0, u03-u0),
+// d_1 = (u1-u01, u12-u01, u13-u01), d_2 = (u12-u02, u2-u02, u23-u02), d_3 = (u13-u03, u23-u03, u3-u03).
+// G[v][3 c + k]: component c, reference direction k.
+template <typename Real, typename UAcc>
+__host__ __device__ inline void p2_vertex_grads(const UAcc &U, Real G[4][9]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Real u[10];
+#pragma unroll
+        for (int b = 0; b < 10; ++b) u[b] = U(c, b);
+        const Real S0 = u[1] - u[0], S1 = u[2] - u[0], S2 = u[3] - u[0];
+        G[0][3 * c + 0] = Real(4) * (u[4] - u[0]) - S0;
+        G[0][3 * c + 1] = Real(4) * (u[5] - u[0]) - S1;
+        G[0][3 * c + 2] = Real(4) * (u[6] - u[0]) - S2;
+        G[1][3 * c + 0] = Real(4) * (u[1] - u[4]) - S0;
+        G[1][3 * c + 1] = Real(4) * (u[7] - u[4]) - S1;
+        G[1][3 * c + 2] = Real(4) * (u[8] - u[4]) - S2;
+        G[2][3 * c + 0] = Real(4) * (u[7] - u[5]) - S0;
+        G[2][3 * c + 1] = Real(4) * (u[2] - u[5]) - S1;
+        G[2][3 * c + 2] = Real(4) * (u[9] - u[5]) - S2;
+        G[3][3 * c + 0] = Real(4) * (u[8] - u[6]) - S0;
+        G[3][3 * c + 1] = Real(4) * (u[9] - u[6]) - S1;
+        G[3][3 * c + 2] = Real(4) * (u[3] - u[6]) - S2;
+    }
+}
+
+// r_i = sum_v M_v : dphi_i(vertex v) for the P2 basis (reference gradients linear in the
+// barycentric coordinates): vertex a: (4 M_a - sum M) . e_a; edge (a, b): 4 (M_a . e_b + M_b . e_a),
+// with e_0 = (-1, -1, -1), e_k = unit k (k = 1, 2, 3).  SM = sum_v M_v.
+template <typename Real>
+__host__ __device__ inline void p2_vertex_contract(const Real M[4][9], const Real SM[9], Real re[3][10]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Real Me0[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Me0[v] = -(M[v][3 * c] + M[v][3 * c + 1] + M[v][3 * c + 2]);
+        const Real SMe0 = -(SM[3 * c] + SM[3 * c + 1] + SM[3 * c + 2]);
+        re[c][0] = Real(4) * Me0[0] - SMe0;
+        re[c][1] = Real(4) * M[1][3 * c + 0] - SM[3 * c + 0];
+        re[c][2] = Real(4) * M[2][3 * c + 1] - SM[3 * c + 1];
+        re[c][3] = Real(4) * M[3][3 * c + 2] - SM[3 * c + 2];
+        re[c][4] = Real(4) * (M[0][3 * c + 0] + Me0[1]);           // (0,1)
+        re[c][5] = Real(4) * (M[0][3 * c + 1] + Me0[2]);           // (0,2)
+        re[c][6] = Real(4) * (M[0][3 * c + 2] + Me0[3]);           // (0,3)
+        re[c][7] = Real(4) * (M[1][3 * c + 1] + M[2][3 * c + 0]);  // (1,2)
+        re[c][8] = Real(4) * (M[1][3 * c + 2] + M[3][3 * c + 0]);  // (1,3)
+        re[c][9] = Real(4) * (M[2][3 * c + 2] + M[3][3 * c + 1]);  // (2,3)
+    }
+}
+
+// Element vector of one P2 tet (affine map, 14-point rule): re[c][b], with the stress evaluated at
+// every quadrature point (any small-strain constitutive law sigma(eps) fits in place of the
+// isotropic one).  On an affine tet the P2 gradients are linear: the strain at a point is
+// interpolated from the strains at the four vertices, eps(q) = sum_v l_v(q) eps_v, and the test
+// sum collects vertex moments of the stress, r_i = sum_v M_v : dphi_i(v) with
+// M_v = -det (sum_q w_q l_v(q) sigma(q)) Jinv^T -- the same quadrature sum as through the basis
+// tables (identical up to rounding), with 6 + 6 products per point and vertex instead of 90 + 90.
+// Tab::w(q) and Tab::lam(q, v) give the rule's weights and barycentric coordinates.
 template <typename Real, typename Tab>
 __host__ __device__ inline void p2_element(const double X[4][3], const Real U[3][10], Real lam,
                                            Real mu, Real re[3][10]) {
@@ -344,31 +401,77 @@ __host__ __device__ inline void p2_element(const double X[4][3], const Real U[3]
     Real Ji[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Ji[i] = (Real)Jid[i];
+    // strains at the vertices (xx, yy, zz, xy, yz, xz) from the physical vertex gradients
+    Real ev[4][6];
+    {
+        struct Acc {
+            const Real (*U)[10];
+            __host__ __device__ Real operator()(int c, int b) const { return U[c][b]; }
+        };
+        Real Gv[4][9];
+        p2_vertex_grads<Real>(Acc{U}, Gv);
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int v = 0; v < 4; ++v) {
+            Real g[9];
 #pragma unroll
-        for (int b = 0; b < 10; ++b) re[c][b] = Real(0);
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    g[3 * c + d] = Gv[v][3 * c + 0] * Ji[0 * 3 + d] + Gv[v][3 * c + 1] * Ji[1 * 3 + d] +
+                                   Gv[v][3 * c + 2] * Ji[2 * 3 + d];
+            ev[v][0] = g[0];
+            ev[v][1] = g[4];
+            ev[v][2] = g[8];
+            ev[v][3] = Real(0.5) * (g[1] + g[3]);
+            ev[v][4] = Real(0.5) * (g[5] + g[7]);
+            ev[v][5] = Real(0.5) * (g[2] + g[6]);
+        }
+    }
+    Real Ms[4][6];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Ms[v][i] = Real(0);
 #pragma unroll 2
     for (int q = 0; q < kP2Quad; ++q) {
-        Real G[9], P[9];
+        const Real l0 = (Real)Tab::lam(q, 0), l1 = (Real)Tab::lam(q, 1), l2 = (Real)Tab::lam(q, 2),
+                   l3 = (Real)Tab::lam(q, 3), w = (Real)Tab::w(q);
+        Real e[6];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 6; ++i) e[i] = l0 * ev[0][i] + l1 * ev[1][i] + l2 * ev[2][i] + l3 * ev[3][i];
+        // pointwise stress (isotropic linear elasticity), weighted
+        const Real ls = w * lam * (e[0] + e[1] + e[2]), m2 = w * Real(2) * mu;
+        Real sg[6];
+        sg[0] = ls + m2 * e[0];
+        sg[1] = ls + m2 * e[1];
+        sg[2] = ls + m2 * e[2];
+        sg[3] = m2 * e[3];
+        sg[4] = m2 * e[4];
+        sg[5] = m2 * e[5];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                Real acc = Real(0);
-#pragma unroll
-                for (int b = 0; b < 10; ++b) acc += U[c][b] * (Real)Tab::dphi(q, b, k);
-                G[3 * c + k] = acc;
-            }
-        iso_flux<Real>(G, Ji, lam, mu, (Real)(Tab::w(q) * det), P);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int b = 0; b < 10; ++b)
-                re[c][b] += P[3 * c + 0] * (Real)Tab::dphi(q, b, 0) +
-                            P[3 * c + 1] * (Real)Tab::dphi(q, b, 1) +
-                            P[3 * c + 2] * (Real)Tab::dphi(q, b, 2);
+        for (int i = 0; i < 6; ++i) {
+            Ms[0][i] += l0 * sg[i];
+            Ms[1][i] += l1 * sg[i];
+            Ms[2][i] += l2 * sg[i];
+            Ms[3][i] += l3 * sg[i];
+        }
     }
+    // M_v = -det S_v Jinv^T (S_v the symmetric stress moment), then the vertex contraction
+    Real M[4][9], SM[9];
+    const Real nd = -(Real)det;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const Real S[9] = {Ms[v][0], Ms[v][3], Ms[v][5], Ms[v][3], Ms[v][1], Ms[v][4], Ms[v][5], Ms[v][4], Ms[v][2]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                M[v][3 * c + k] = nd * (S[3 * c + 0] * Ji[3 * k + 0] + S[3 * c + 1] * Ji[3 * k + 1] +
+                                        S[3 * c + 2] * Ji[3 * k + 2]);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) SM[i] = M[0][i] + M[1][i] + M[2][i] + M[3][i];
+    p2_vertex_contract<Real>(M, SM, re);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -588,29 +691,8 @@ __host__ __device__ inline void p2_element_linear(const double X[4][3], const UA
     Real Ji[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Ji[i] = (Real)Jid[i];
-    // vertex reference gradients, one solution component at a time: S = (u1-u0, u2-u0, u3-u0),
-    // G_v = 4 d_v - S with d_0 = (u01-u0, u02-u0, u03-u0), d_1 = (u1-u01, u12-u01, u13-u01),
-    // d_2 = (u12-u02, u2-u02, u23-u02), d_3 = (u13-u03, u23-u03, u3-u03)
     Real P[4][9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Real u[10];
-#pragma unroll
-        for (int b = 0; b < 10; ++b) u[b] = U(c, b);
-        const Real S0 = u[1] - u[0], S1 = u[2] - u[0], S2 = u[3] - u[0];
-        P[0][3 * c + 0] = Real(4) * (u[4] - u[0]) - S0;
-        P[0][3 * c + 1] = Real(4) * (u[5] - u[0]) - S1;
-        P[0][3 * c + 2] = Real(4) * (u[6] - u[0]) - S2;
-        P[1][3 * c + 0] = Real(4) * (u[1] - u[4]) - S0;
-        P[1][3 * c + 1] = Real(4) * (u[7] - u[4]) - S1;
-        P[1][3 * c + 2] = Real(4) * (u[8] - u[4]) - S2;
-        P[2][3 * c + 0] = Real(4) * (u[7] - u[5]) - S0;
-        P[2][3 * c + 1] = Real(4) * (u[2] - u[5]) - S1;
-        P[2][3 * c + 2] = Real(4) * (u[9] - u[5]) - S2;
-        P[3][3 * c + 0] = Real(4) * (u[8] - u[6]) - S0;
-        P[3][3 * c + 1] = Real(4) * (u[9] - u[6]) - S1;
-        P[3][3 * c + 2] = Real(4) * (u[3] - u[6]) - S2;
-    }
+    p2_vertex_grads<Real>(U, P);
     // vertex fluxes, in place
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -629,26 +711,10 @@ __host__ __device__ inline void p2_element_linear(const double X[4][3], const UA
 #pragma unroll
         for (int i = 0; i < 9; ++i) P[v][i] = a * P[v][i] + b * SP[i];
     const Real sa = a + Real(4) * b;
-    // contraction with e_k: k >= 1 -> column k-1; k = 0 -> minus the row sum
+    Real SM[9];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Real Me0[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Me0[v] = -(P[v][3 * c] + P[v][3 * c + 1] + P[v][3 * c + 2]);
-        const Real SMe0 = -sa * (SP[3 * c] + SP[3 * c + 1] + SP[3 * c + 2]);
-        // vertices: r_a = (4 M_a - sum M) . e_a
-        re[c][0] = Real(4) * Me0[0] - SMe0;
-        re[c][1] = Real(4) * P[1][3 * c + 0] - sa * SP[3 * c + 0];
-        re[c][2] = Real(4) * P[2][3 * c + 1] - sa * SP[3 * c + 1];
-        re[c][3] = Real(4) * P[3][3 * c + 2] - sa * SP[3 * c + 2];
-        // edges: r_ab = 4 (M_a . e_b + M_b . e_a)
-        re[c][4] = Real(4) * (P[0][3 * c + 0] + Me0[1]);          // (0,1)
-        re[c][5] = Real(4) * (P[0][3 * c + 1] + Me0[2]);          // (0,2)
-        re[c][6] = Real(4) * (P[0][3 * c + 2] + Me0[3]);          // (0,3)
-        re[c][7] = Real(4) * (P[1][3 * c + 1] + P[2][3 * c + 0]); // (1,2)
-        re[c][8] = Real(4) * (P[1][3 * c + 2] + P[3][3 * c + 0]); // (1,3)
-        re[c][9] = Real(4) * (P[2][3 * c + 2] + P[3][3 * c + 1]); // (2,3)
-    }
+    for (int i = 0; i < 9; ++i) SM[i] = sa * SP[i];
+    p2_vertex_contract<Real>(P, SM, re);
 }
 
 }  // namespace plx

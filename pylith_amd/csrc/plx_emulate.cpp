@@ -23,6 +23,7 @@ struct HostTab {
     }
     static double w(int q) { return t().w[q]; }
     static double dphi(int q, int b, int k) { return t().dphi[q][b][k]; }
+    static double lam(int q, int v) { return t().lam[q][v]; }
 };
 
 #define CHECK(cond)                                                                              \
