@@ -544,18 +544,24 @@ def bench_p2(torch, mg, n_edge, steps, warmup, renumber_seed=None):
             "hbm_frac_kernel": B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def bench_q1(torch, mg, n_edge, steps, warmup, renumber_seed=None):
+def bench_q1(torch, mg, n_edge, steps, warmup, renumber_seed=None, jitter=None, algorithm=None):
     """Q1 hex FP64 residual apply on an n^3 cube (6 km, u ~ U[-1e-3, 1e-3] m with the shear
     Dirichlet values inserted): wall time per apply, tile-kernel time and HBM fractions; with
-    `renumber_seed` the cells and nodes are renumbered at random first."""
+    `renumber_seed` the cells and nodes are renumbered at random first, with `jitter` the interior
+    nodes move by up to that fraction of the spacing (non-affine hexes), and `algorithm` forces
+    an element path (the quadrature-point one: the pointwise stress at every Gauss point)."""
     hm = mg.hex_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
     if renumber_seed is not None:
         hm = mg.renumbered(hm, renumber_seed)
+    if jitter is not None:
+        hm = mg.jittered(hm, jitter * 6000.0 / n_edge, 7)
     haux, hs = mg.elastic_aux(hm.num_cells)
     hu = mg.random_field(hm.num_nodes, SEED)
     dofs_bc, vals = mg.shear_bcs(hm.node_coords, 0.0, 6000.0)
     hu[dofs_bc] = vals
     hdm = setup(hm, haux, hs)
+    if algorithm is not None:
+        hdm.set_kernel_options(algorithm)
     hx = torch.from_numpy(hu).cuda()
     hr = torch.zeros_like(hx)
     f = lambda: hdm.compute_residual(0, 0, hx, hr, mode=1)  # noqa: E731
@@ -770,6 +776,16 @@ def main():
         torch.cuda.empty_cache()
         extra["p2_fp64_residual_64cube_renumbered"] = bench_p2(torch, mg, 64, max(10, args.steps // 2), args.warmup,
                                                                renumber_seed=6)
+        torch.cuda.empty_cache()
+        # the quadrature-point Q1 paths: the pointwise stress at every Gauss point on affine cells,
+        # and on non-affine cells (interior nodes moved by up to 0.2 h: the Jacobian at each point)
+        from pylith_amd import _lib as L
+        print("[bench] Q1 quadrature-point paths", file=sys.stderr, flush=True)
+        extra["q1_fp64_residual_192cube_quadrature_points"] = bench_q1(
+            torch, mg, args.q1_large, max(10, args.steps // 2), args.warmup, algorithm=L.ALGO_QUADRATURE_POINTS)
+        torch.cuda.empty_cache()
+        extra["q1_fp64_residual_192cube_nonaffine"] = bench_q1(torch, mg, args.q1_large, max(10, args.steps // 2),
+                                                               args.warmup, jitter=0.2)
         torch.cuda.empty_cache()
         result["extra"] = extra
     if not args.no_systems:

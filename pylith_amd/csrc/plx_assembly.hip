@@ -232,9 +232,15 @@ __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, in
     }
 }
 
+// Element paths of the tile kernel: the factorized form (kernels linear in the gradient with
+// per-cell coefficients, affine cells), the quadrature-point form on affine cells, and the
+// quadrature-point form on any Q1 cell (P2 tets are affine: their quadrature-point form is
+// kPathQpAffine).
+enum { kPathFactored = 0, kPathQpAffine = 1, kPathQpGeneral = 2 };
+
 // Persistent, software-pipelined tile kernel: each workgroup walks its slots; while it works
 // on one tile, the gathers of its next tile are in flight.
-template <typename Real, int CT, bool LINEAR, int ACC, int MINW>
+template <typename Real, int CT, int PATH, int ACC, int MINW>
 __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(ApplyArgs A) {
     using S = Shape<CT, Real>;
     constexpr int NT = S::NT, NB = S::NB, NV = S::NV;
@@ -319,26 +325,17 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
                 const Real lam = (Real)A.scale * cur.lam[i], mu = (Real)A.scale * cur.mu[i];
                 if constexpr (CT == PLX_HEX_Q1) {
                     Real rl[3][8];
-                    if constexpr (LINEAR) {
+                    const LdsAccQ1<double> xacc{sX, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7]}};
+                    const LdsAccQ1<Real> acc{sU, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7]}};
+                    if constexpr (PATH == kPathFactored) {
                         // affine tiles only (the plan routes other tiles to the quadrature-point kernel)
                         double J[9];
-                        q1_affine_jacobian(LdsAccQ1<double>{sX, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7]}}, J);
-                        LdsAccQ1<Real> acc{sU, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7]}};
+                        q1_affine_jacobian(xacc, J);
                         q1_element_affine<Real>(J, acc, lam, mu, rl);
                     } else {
-                        double X[3][8];
-                        Real U[3][8];
-#pragma unroll
-                        for (int l = 0; l < 8; ++l) {
-                            const int v = lc[q1_lex(l)];
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) {
-                                X[d][l] = sX[3 * v + d];
-                                U[d][l] = sU[3 * v + d];
-                            }
-                        }
                         const double *pre = A.prestress ? A.prestress + cur.c0 + tid + i * NT : nullptr;
-                        q1_element<Real>(X, U, lam, mu, rl, pre, (size_t)A.p.stride, (Real)A.scale);
+                        q1_element_qp<Real, PATH == kPathQpAffine>(xacc, acc, lam, mu, rl, pre, (size_t)A.p.stride,
+                                                                   (Real)A.scale);
                     }
 #pragma unroll
                     for (int l = 0; l < 8; ++l)
@@ -352,7 +349,7 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
 #pragma unroll
                         for (int d = 0; d < 3; ++d) X[v][d] = sX[3 * lv + d];
                     }
-                    if constexpr (LINEAR) {
+                    if constexpr (PATH == kPathFactored) {
                         LdsAcc<Real, 10> acc{sU, {lc[0], lc[1], lc[2], lc[3], lc[4], lc[5], lc[6], lc[7], lc[8], lc[9]}};
                         p2_element_linear<Real>(X, acc, lam, mu, c_p2.wdiag, c_p2.woff, re);
                     } else {
@@ -528,46 +525,52 @@ static int apply_typed(plx_dm dm, KeyPlan *kp, const void *u, void *r, int mode,
     const int ncu = std::max(dm->num_cu, 8);
     {
     ScopedEvent ev(dm, log ? "plx:apply_tiles" : nullptr);
-#define PLX_LAUNCH_W(CT, LIN, ACC, W, N)                                                             \
+#define PLX_LAUNCH_W(CT, PATH, ACC, W, N)                                                             \
     do {                                                                                             \
-        const void *kfn = reinterpret_cast<const void *>(&apply_tiles<Real, CT, LIN, ACC, W>);        \
+        const void *kfn = reinterpret_cast<const void *>(&apply_tiles<Real, CT, PATH, ACC, W>);       \
         const int per_cu = resident_per_cu(kfn, Shape<CT, Real>::NT, lds, dm->wg_per_cu);            \
         const int resident = ((ncu * per_cu + 7) / 8) * 8;                                           \
-        hipLaunchKernelGGL((apply_tiles<Real, CT, LIN, ACC, W>), dim3(std::min(N, resident)),          \
+        hipLaunchKernelGGL((apply_tiles<Real, CT, PATH, ACC, W>), dim3(std::min(N, resident)),         \
                            dim3(Shape<CT, Real>::NT), lds, dm->stream, A);                           \
     } while (0)
-#define PLX_LAUNCH(CT, LIN, ACC, LIST, N)                                                            \
+#define PLX_LAUNCH(CT, PATH, ACC, LIST, N)                                                           \
     do {                                                                                             \
         A.tile_list = LIST;                                                                          \
         A.num_list = N;                                                                              \
         if (N > 0) {                                                                                 \
-            if (minw <= 2) PLX_LAUNCH_W(CT, LIN, ACC, 2, N);                                           \
-            else if (minw == 3) PLX_LAUNCH_W(CT, LIN, ACC, 3, N);                                      \
-            else PLX_LAUNCH_W(CT, LIN, ACC, 4, N);                                                     \
+            if (minw <= 2) PLX_LAUNCH_W(CT, PATH, ACC, 2, N);                                          \
+            else if (minw == 3) PLX_LAUNCH_W(CT, PATH, ACC, 3, N);                                     \
+            else PLX_LAUNCH_W(CT, PATH, ACC, 4, N);                                                    \
         }                                                                                            \
     } while (0)
     const int minw = dm->min_waves;
     if (dm->cell_type == PLX_HEX_Q1) {
+        // affine tiles: the factorized kernel (or the quadrature-point one on affine cells); the
+        // rest: the quadrature-point kernel with the Jacobian at each point
         if (linear) {
-            // affine tiles: modal kernel; the rest: quadrature-point kernel
             if (atomic) {
-                PLX_LAUNCH(PLX_HEX_Q1, true, kAccAtomic, kp->d_tiles_affine, kp->num_tiles_affine);
-                PLX_LAUNCH(PLX_HEX_Q1, false, kAccAtomic, kp->d_tiles_general, kp->num_tiles_general);
+                PLX_LAUNCH(PLX_HEX_Q1, kPathFactored, kAccAtomic, kp->d_tiles_affine, kp->num_tiles_affine);
+                PLX_LAUNCH(PLX_HEX_Q1, kPathQpGeneral, kAccAtomic, kp->d_tiles_general, kp->num_tiles_general);
             } else {
-                PLX_LAUNCH(PLX_HEX_Q1, true, kAccColored, kp->d_tiles_affine, kp->num_tiles_affine);
-                PLX_LAUNCH(PLX_HEX_Q1, false, kAccColored, kp->d_tiles_general, kp->num_tiles_general);
+                PLX_LAUNCH(PLX_HEX_Q1, kPathFactored, kAccColored, kp->d_tiles_affine, kp->num_tiles_affine);
+                PLX_LAUNCH(PLX_HEX_Q1, kPathQpGeneral, kAccColored, kp->d_tiles_general, kp->num_tiles_general);
             }
         } else {
-            if (atomic) PLX_LAUNCH(PLX_HEX_Q1, false, kAccAtomic, nullptr, kp->num_tiles);
-            else PLX_LAUNCH(PLX_HEX_Q1, false, kAccColored, nullptr, kp->num_tiles);
+            if (atomic) {
+                PLX_LAUNCH(PLX_HEX_Q1, kPathQpAffine, kAccAtomic, kp->d_tiles_affine, kp->num_tiles_affine);
+                PLX_LAUNCH(PLX_HEX_Q1, kPathQpGeneral, kAccAtomic, kp->d_tiles_general, kp->num_tiles_general);
+            } else {
+                PLX_LAUNCH(PLX_HEX_Q1, kPathQpAffine, kAccColored, kp->d_tiles_affine, kp->num_tiles_affine);
+                PLX_LAUNCH(PLX_HEX_Q1, kPathQpGeneral, kAccColored, kp->d_tiles_general, kp->num_tiles_general);
+            }
         }
     } else {
         if (linear) {
-            if (atomic) PLX_LAUNCH(PLX_TET_P2, true, kAccAtomic, nullptr, kp->num_tiles);
-            else PLX_LAUNCH(PLX_TET_P2, true, kAccColored, nullptr, kp->num_tiles);
+            if (atomic) PLX_LAUNCH(PLX_TET_P2, kPathFactored, kAccAtomic, nullptr, kp->num_tiles);
+            else PLX_LAUNCH(PLX_TET_P2, kPathFactored, kAccColored, nullptr, kp->num_tiles);
         } else {
-            if (atomic) PLX_LAUNCH(PLX_TET_P2, false, kAccAtomic, nullptr, kp->num_tiles);
-            else PLX_LAUNCH(PLX_TET_P2, false, kAccColored, nullptr, kp->num_tiles);
+            if (atomic) PLX_LAUNCH(PLX_TET_P2, kPathQpAffine, kAccAtomic, nullptr, kp->num_tiles);
+            else PLX_LAUNCH(PLX_TET_P2, kPathQpAffine, kAccColored, nullptr, kp->num_tiles);
         }
     }
 #undef PLX_LAUNCH_W

@@ -180,154 +180,11 @@ __host__ __device__ inline void iso_flux_pre(const Real G[9], const Real Ji[9], 
 }
 
 // --------------------------------------------------------------------------------------------
-// Q1 hexahedron, 2x2x2 Gauss, sum factorization.  Lexicographic corner index l = i + 2j + 4k
-// maps to PyLith's vertex order through kLex.
+// Q1 hexahedron, 2x2x2 Gauss.  Lexicographic corner index l = i + 2j + 4k maps to PyLith's
+// vertex order through q1_lex.
 // --------------------------------------------------------------------------------------------
 __host__ __device__ constexpr int q1_lex(int l) {
     return l == 2 ? 3 : l == 3 ? 2 : l == 6 ? 7 : l == 7 ? 6 : l;
-}
-
-// Gradient of a trilinear field F (lexicographic corners) at the 8 Gauss points.  Each
-// reference derivative is constant along its own direction, so it has 4 distinct values:
-//   dX[a][b]: d/dxi at (eta=qa, zeta=qb); dY[a][b]: d/deta at (xi=qa, zeta=qb);
-//   dZ[a][b]: d/dzeta at (xi=qa, eta=qb).
-template <typename Real>
-__host__ __device__ inline void q1_interp2(const Real e[2][2], Real out[2][2]) {
-    const Real ga = Real(0.78867513459481288225);  // N0(-1/sqrt3) = N1(+1/sqrt3)
-    const Real gb = Real(0.21132486540518711775);  // N1(-1/sqrt3) = N0(+1/sqrt3)
-    Real t[2][2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        t[0][k] = ga * e[0][k] + gb * e[1][k];
-        t[1][k] = gb * e[0][k] + ga * e[1][k];
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        out[a][0] = ga * t[a][0] + gb * t[a][1];
-        out[a][1] = gb * t[a][0] + ga * t[a][1];
-    }
-}
-
-template <typename Real>
-__host__ __device__ inline void q1_ref_grad(const Real F[8], Real dX[2][2], Real dY[2][2],
-                                            Real dZ[2][2]) {
-    Real e[2][2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) e[j][k] = Real(0.5) * (F[1 + 2 * j + 4 * k] - F[2 * j + 4 * k]);
-    q1_interp2<Real>(e, dX);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) e[i][k] = Real(0.5) * (F[i + 2 + 4 * k] - F[i + 4 * k]);
-    q1_interp2<Real>(e, dY);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) e[i][j] = Real(0.5) * (F[i + 2 * j + 4] - F[i + 2 * j]);
-    q1_interp2<Real>(e, dZ);
-}
-
-// Transpose of q1_ref_grad: R[l] += sum_q sum_k S_k(q) dphi_l/dxi_k(q), given the flux sums
-// SX[a][b] = sum over the xi-points of P_xi at (eta=a, zeta=b), and likewise SY, SZ.
-// (q1_interp2 is symmetric, so it is its own transpose.)
-template <typename Real>
-__host__ __device__ inline void q1_ref_grad_t(const Real SX[2][2], const Real SY[2][2],
-                                              const Real SZ[2][2], Real R[8]) {
-    Real h[2][2];
-    q1_interp2<Real>(SX, h);
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            R[1 + 2 * j + 4 * k] += Real(0.5) * h[j][k];
-            R[2 * j + 4 * k] -= Real(0.5) * h[j][k];
-        }
-    q1_interp2<Real>(SY, h);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            R[i + 2 + 4 * k] += Real(0.5) * h[i][k];
-            R[i + 4 * k] -= Real(0.5) * h[i][k];
-        }
-    q1_interp2<Real>(SZ, h);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            R[i + 2 * j + 4] += Real(0.5) * h[i][j];
-            R[i + 2 * j] -= Real(0.5) * h[i][j];
-        }
-}
-
-// Element vector of one Q1 cell (lexicographic corners): re[c][l].
-// pre (optional): a stress added to sigma at each quadrature point q = qi + 2 qj + 4 qk, six
-// components (xx, yy, zz, xy, yz, xz), component k of point q at pre[(8 k + q) * pstride], scaled
-// by prescale -- the state-dependent part of a viscoelastic stress.
-template <typename Real>
-__host__ __device__ inline void q1_element(const double X[3][8], const Real U[3][8], Real lam,
-                                           Real mu, Real re[3][8], const double *pre = nullptr,
-                                           size_t pstride = 0, Real prescale = Real(0)) {
-    double jX[3][2][2], jY[3][2][2], jZ[3][2][2];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) q1_ref_grad<double>(X[d], jX[d], jY[d], jZ[d]);
-    Real uX[3][2][2], uY[3][2][2], uZ[3][2][2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) q1_ref_grad<Real>(U[c], uX[c], uY[c], uZ[c]);
-    Real SX[3][2][2], SY[3][2][2], SZ[3][2][2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) SX[c][a][b] = SY[c][a][b] = SZ[c][a][b] = Real(0);
-#pragma unroll
-    for (int qk = 0; qk < 2; ++qk)
-#pragma unroll
-        for (int qj = 0; qj < 2; ++qj)
-#pragma unroll
-            for (int qi = 0; qi < 2; ++qi) {
-                double J[9], Jid[9];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    J[3 * d + 0] = jX[d][qj][qk];
-                    J[3 * d + 1] = jY[d][qi][qk];
-                    J[3 * d + 2] = jZ[d][qi][qj];
-                }
-                const double det = inv3(J, Jid);
-                Real Ji[9], G[9], P[9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) Ji[i] = (Real)Jid[i];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    G[3 * c + 0] = uX[c][qj][qk];
-                    G[3 * c + 1] = uY[c][qi][qk];
-                    G[3 * c + 2] = uZ[c][qi][qj];
-                }
-                if (pre) {
-                    const int q = qi + 2 * qj + 4 * qk;
-                    Real ps[6];
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) ps[k] = prescale * (Real)pre[(size_t)(8 * k + q) * pstride];
-                    iso_flux_pre<Real>(G, Ji, lam, mu, (Real)det, ps, P);
-                } else {
-                    iso_flux<Real>(G, Ji, lam, mu, (Real)det, P);
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    SX[c][qj][qk] += P[3 * c + 0];
-                    SY[c][qi][qk] += P[3 * c + 1];
-                    SZ[c][qi][qj] += P[3 * c + 2];
-                }
-            }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int l = 0; l < 8; ++l) re[c][l] = Real(0);
-        q1_ref_grad_t<Real>(SX[c], SY[c], SZ[c], re[c]);
-    }
 }
 
 // Reference gradients of a P2 field at the four vertices, one solution component at a time:
@@ -671,6 +528,107 @@ __host__ __device__ inline void q1_element_affine(const double J[9], const UAcc 
     R[0][7] = lm * (N[0] * a0 + N[3] * a1 + N[5] * a2) + mt * a0;
     R[1][7] = lm * (N[3] * a0 + N[1] * a1 + N[4] * a2) + mt * a1;
     R[2][7] = lm * (N[5] * a0 + N[4] * a1 + N[2] * a2) + mt * a2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) iwht8<Real>(R[c], re[c]);
+}
+
+// Q1 element at the 2x2x2 Gauss points in the modal basis: u = sum_b T_b m_b with T the Walsh-
+// Hadamard transform of the corner values over 8 and m_b the products of the reference
+// coordinates in b (b = xi + 2 eta + 4 zeta bits), so the reference gradient at a point is
+//   du/dxi = T1 + T3 eta + T5 zeta + T7 eta zeta   (and cyclically for eta, zeta),
+// and the test sum collects the modal moments R_b = sum_q P(q) : grad m_b(q), re = iwht8(R).
+// The pointwise flux is evaluated at every point, with the optional per-point stress `pre` added
+// to sigma: six components (xx, yy, zz, xy, yz, xz), component k of point q = qi + 2 qj + 4 qk
+// at pre[(8 k + q) * pstride], scaled by prescale (the state-dependent part of a viscoelastic
+// stress).  AFFINE: the cell's constant Jacobian (edge vectors at corner 0); otherwise the
+// Jacobian at each point from the modal coefficients of the coordinates.  re[c][l] in
+// lexicographic corner order.
+template <typename Real, bool AFFINE, typename XAcc, typename UAcc>
+__host__ __device__ inline void q1_element_qp(const XAcc &X, const UAcc &U, Real lam, Real mu, Real re[3][8],
+                                              const double *pre = nullptr, size_t pstride = 0,
+                                              Real prescale = Real(0)) {
+    constexpr double g = 0.57735026918962576451;  // 1 / sqrt(3)
+    Real T[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Real u[8];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) u[l] = U(c, l);
+        wht8<Real>(u, T[c]);
+#pragma unroll
+        for (int b = 1; b < 8; ++b) T[c][b] *= Real(0.125);
+    }
+    double TX[3][8], Ja[9], Jia[9], deta = 0.0;
+    if constexpr (AFFINE) {
+        q1_affine_jacobian(X, Ja);
+        deta = inv3(Ja, Jia);
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double x[8];
+#pragma unroll
+            for (int l = 0; l < 8; ++l) x[l] = X(d, l);
+            wht8<double>(x, TX[d]);
+#pragma unroll
+            for (int b = 1; b < 8; ++b) TX[d][b] *= 0.125;
+        }
+    }
+    Real R[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) R[c][b] = Real(0);
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        const double xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g, zeta = (q & 4) ? g : -g;
+        double Jid[9], det;
+        if constexpr (AFFINE) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Jid[i] = Jia[i];
+            det = deta;
+        } else {
+            double J[9];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                J[3 * d + 0] = TX[d][1] + TX[d][3] * eta + TX[d][5] * zeta + TX[d][7] * (eta * zeta);
+                J[3 * d + 1] = TX[d][2] + TX[d][3] * xi + TX[d][6] * zeta + TX[d][7] * (xi * zeta);
+                J[3 * d + 2] = TX[d][4] + TX[d][5] * xi + TX[d][6] * eta + TX[d][7] * (xi * eta);
+            }
+            det = inv3(J, Jid);
+        }
+        const Real x = (Real)xi, y = (Real)eta, z = (Real)zeta, yz = (Real)(eta * zeta),
+                   xz = (Real)(xi * zeta), xy = (Real)(xi * eta);
+        Real Ji[9], G[9], P[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Ji[i] = (Real)Jid[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G[3 * c + 0] = T[c][1] + T[c][3] * y + T[c][5] * z + T[c][7] * yz;
+            G[3 * c + 1] = T[c][2] + T[c][3] * x + T[c][6] * z + T[c][7] * xz;
+            G[3 * c + 2] = T[c][4] + T[c][5] * x + T[c][6] * y + T[c][7] * xy;
+        }
+        // weight 1, and the 1/8 of the test modes
+        const Real scale = (Real)(0.125 * det);
+        if (pre) {
+            Real ps[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) ps[k] = prescale * (Real)pre[(size_t)(8 * k + q) * pstride];
+            iso_flux_pre<Real>(G, Ji, lam, mu, scale, ps, P);
+        } else {
+            iso_flux<Real>(G, Ji, lam, mu, scale, P);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const Real p0 = P[3 * c + 0], p1 = P[3 * c + 1], p2 = P[3 * c + 2];
+            R[c][1] += p0;
+            R[c][2] += p1;
+            R[c][4] += p2;
+            R[c][3] += y * p0 + x * p1;
+            R[c][5] += z * p0 + x * p2;
+            R[c][6] += z * p1 + y * p2;
+            R[c][7] += yz * p0 + xz * p1 + xy * p2;
+        }
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) iwht8<Real>(R[c], re[c]);
 }

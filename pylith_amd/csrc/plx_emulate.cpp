@@ -89,18 +89,23 @@ int emulate(const plx::MeshView &mv, const double *coords, const plx::HostPlan &
                         U[d][l] = sU[3 * v + d];
                     }
                 }
-                bool done = false;
-                if (linear) {
-                    double TX[3][8];
-                    for (int d = 0; d < 3; ++d) plx::wht8<double>(X[d], TX[d]);
-                    if (plx::q1_affine(TX)) {
+                // the kernels' paths: affine cells factorized (or at the quadrature points), the
+                // rest at the quadrature points with the Jacobian at each point
+                double TX[3][8];
+                for (int d = 0; d < 3; ++d) plx::wht8<double>(X[d], TX[d]);
+                const plx::ArrayAccX xacc{X};
+                const plx::ArrayAcc3Q1<Real> uacc{U};
+                if (plx::q1_affine(TX)) {
+                    if (linear) {
                         double J[9];
-                        plx::q1_affine_jacobian(plx::ArrayAccX{X}, J);
-                        plx::q1_element_affine<Real>(J, plx::ArrayAcc3Q1<Real>{U}, lam, mu, rl);
-                        done = true;
+                        plx::q1_affine_jacobian(xacc, J);
+                        plx::q1_element_affine<Real>(J, uacc, lam, mu, rl);
+                    } else {
+                        plx::q1_element_qp<Real, true>(xacc, uacc, lam, mu, rl);
                     }
+                } else {
+                    plx::q1_element_qp<Real, false>(xacc, uacc, lam, mu, rl);
                 }
-                if (!done) plx::q1_element<Real>(X, U, lam, mu, rl);
                 for (int l = 0; l < 8; ++l)
                     for (int c = 0; c < 3; ++c) out[c * nb + plx::q1_lex(l)] = rl[c][l];
             } else {

@@ -93,6 +93,21 @@ def hex_box(n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), brick=8, ordered=True, sh
                 np.zeros(closure.shape[0], np.int32), colors)
 
 
+def jittered(mesh, amp, seed):
+    """The same mesh with every node off the bounding box moved by a seeded uniform offset of up
+    to `amp` (in coordinate units) per component: hexes become non-affine (Q1 only: the nodes
+    are the vertices)."""
+    if mesh.cell_type != HEX_Q1:
+        raise ValueError("jittered: Q1 hex meshes only")
+    rng = np.random.default_rng(seed)
+    x = mesh.coords.copy()
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    tol = 1e-9 * float(np.max(hi - lo))
+    inner = np.all((x > lo + tol) & (x < hi - tol), axis=1)
+    x[inner] += rng.uniform(-amp, amp, (int(inner.sum()), 3))
+    return Mesh(mesh.cell_type, mesh.closure, mesh.cone, x, x, mesh.shape, mesh.labels, mesh.colors)
+
+
 def renumbered(mesh, seed):
     """The same mesh with cells, nodes and vertices renumbered by seeded random permutations (a
     mesh whose numbering carries no locality, as a general mesh file or partitioner may give)."""
