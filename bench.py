@@ -524,14 +524,17 @@ def bench_c5_sweep(torch):
     return out
 
 
-def bench_p2(torch, mg, n_edge, steps, warmup, renumber_seed=None):
+def bench_p2(torch, mg, n_edge, steps, warmup, renumber_seed=None, algorithm=None):
     """P2 FP64 residual apply on an n^3 x 6 box (same synthetic inputs as the headline); with
-    `renumber_seed` the cells, nodes and vertices are renumbered at random first."""
+    `renumber_seed` the cells, nodes and vertices are renumbered at random first, and
+    `algorithm` forces an element path."""
     mesh = mg.tet_p2_box(n_edge, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
     if renumber_seed is not None:
         mesh = mg.renumbered(mesh, renumber_seed)
     aux, sizes = mg.elastic_aux(mesh.num_cells, seed=SEED, vs_sigma=0.2, vp_vs=1.732)
     dm = setup(mesh, aux, sizes)
+    if algorithm is not None:
+        dm.set_kernel_options(algorithm)
     x = torch.from_numpy(mg.sinusoid_field(mesh.node_coords, SEED)).cuda()
     r = torch.zeros_like(x)
     f = lambda: dm.compute_residual(0, 0, x, r, mode=1)  # noqa: E731
@@ -540,8 +543,9 @@ def bench_p2(torch, mg, n_edge, steps, warmup, renumber_seed=None):
     B = p2_bytes(mesh.num_nodes, mesh.num_vertices, mesh.num_cells, 8)
     dm.close()
     return {"cells": mesh.num_cells, "dofs": 3 * mesh.num_nodes, "gdofs": 3 * mesh.num_nodes / sec / 1e9,
-            "ms": sec * 1e3, "kernel_ms": kms, "algorithmic_bytes": B,
-            "hbm_frac_kernel": B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "ms": sec * 1e3, "kernel_ms": kms or None, "algorithmic_bytes": B,
+            "hbm_frac_kernel": B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS if kms else None,
+            "hbm_frac_apply": B / sec / 1e9 / HBM_PEAK_GBS}
 
 
 def bench_q1(torch, mg, n_edge, steps, warmup, renumber_seed=None, jitter=None, algorithm=None):
@@ -575,8 +579,8 @@ def bench_q1(torch, mg, n_edge, steps, warmup, renumber_seed=None, jitter=None, 
     sj = time_applies(lambda: hdm.compute_jacobian_action(0, 0, hx, jv, mode=1), steps, warmup, torch)
     out = {"gdofs": 3 * hm.num_nodes / s / 1e9, "ms": s * 1e3, "ms_median": med, "ms_p99": p99,
            "ms_graph": gms, "gdofs_graph": 3 * hm.num_nodes / (gms * 1e-3) / 1e9,
-           "kernel_ms": km, "halo_reduce_ms": hm_ms,
-           "algorithmic_bytes": hb, "hbm_frac_kernel": hb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "kernel_ms": km or None, "halo_reduce_ms": hm_ms or None,
+           "algorithmic_bytes": hb, "hbm_frac_kernel": hb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS if km else None,
            "hbm_frac_apply": hb / s / 1e9 / HBM_PEAK_GBS,
            "jacobian_action_gdofs": 3 * hm.num_nodes / sj / 1e9}
     hdm.close()
@@ -786,6 +790,14 @@ def main():
         torch.cuda.empty_cache()
         extra["q1_fp64_residual_192cube_nonaffine"] = bench_q1(torch, mg, args.q1_large, max(10, args.steps // 2),
                                                                args.warmup, jitter=0.2)
+        torch.cuda.empty_cache()
+        # the same operators through the PetscPointFn-form device path (the plug-in route for
+        # kernels written to PyLith's pointwise signature: a thread per cell, colour passes)
+        print("[bench] PetscPointFn-form path", file=sys.stderr, flush=True)
+        extra["p2_fp64_residual_pointfn"] = bench_p2(torch, mg, 64, 5, 2, algorithm=L.ALGO_POINTFN)
+        torch.cuda.empty_cache()
+        extra["q1_fp64_residual_192cube_pointfn"] = bench_q1(torch, mg, args.q1_large, 5, 2,
+                                                             algorithm=L.ALGO_POINTFN)
         torch.cuda.empty_cache()
         result["extra"] = extra
     if not args.no_systems:

@@ -127,12 +127,14 @@ struct Shape<PLX_HEX_Q1, Real> {
     // RECW: uint4 words per cell record (8 x uint16 closure)
     static constexpr int NT = 256, CPT = 2, NB = 8, NV = 8, PFN = 3, PFV = 0, RECW = 1;
 };
-// P2 FP64: 3-wave workgroups with two cells per thread, so that two workgroups fit a CU at the
-// kernel's register count (2 waves per SIMD); FP32 (4 waves per SIMD) keeps 6-wave workgroups.
+// P2 FP64: 4-wave workgroups over the 384-cell tile (cells 0-255 on the first pass, 256-383 on
+// the second), two workgroups per CU at the kernel's register count (2 waves per SIMD: every
+// wave slot of the CU busy; measured 6 % faster than 3-wave workgroups with two full passes);
+// FP32 (4 waves per SIMD) keeps 6-wave workgroups.
 // record: 10 x uint16 closure, 4 x uint16 cone, 2 padding = 2 uint4 words
 template <>
 struct Shape<PLX_TET_P2, double> {
-    static constexpr int NT = 192, CPT = 2, NB = 10, NV = 4, PFN = 4, PFV = 1, RECW = 2;
+    static constexpr int NT = 256, CPT = 2, NB = 10, NV = 4, PFN = 3, PFV = 1, RECW = 2;
 };
 template <>
 struct Shape<PLX_TET_P2, float> {
