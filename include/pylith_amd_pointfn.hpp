@@ -321,43 +321,172 @@ __global__ void __launch_bounds__(256) zero_closure(plx_pointfn_args A) {
     }
 }
 
-// f += sum_q w|J| (f0 . phi + f1 : grad phi) over the cells [c0, c1) of one colour.
-template <class F0, class F1, int CT>
-__global__ void __launch_bounds__(128) residual_cells(plx_pointfn_args A, int c0, int c1) {
-    constexpr int NB = Basis<CT>::NB, NV = Basis<CT>::NV, NQ = Basis<CT>::NQ;
-    const int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c1) return;
-    const int cell = A.cells[i];
-    double X[NV][3], U[NB][3], Ut[NB][3];
-    gather_cell<CT>(A, cell, X, A.u, U);
-    gather_cell<CT>(A, cell, X, A.u_t, Ut);
-    int aOff[16], aOff_x[16];
-    for (int k = 0; k < A.num_aux && k < 16; ++k) {
-        aOff[k] = A.aux_off[k];
-        aOff_x[k] = 3 * A.aux_off[k];
+// ---------------------------------------------------------------------------------------------
+// Residual and Jacobian action through the structure of the bases (one thread per cell).
+//   P2 (affine map): J^-1 once per cell; at a point the values from the 10 basis values and the
+//   reference gradient from the barycentric form of the basis gradients; the f1 test sum
+//   collects physical moments M_v = sum_q w|J| l_v(q) f1(q) at the four vertices (the P2
+//   gradients are linear in the barycentric coordinates), r_b = sum_v (M_v J^-T) : dref_b(v).
+//   Q1: the modal (Walsh-Hadamard) form u = sum_b T_b m_b, m_b the products of the reference
+//   coordinates in b; values, reference gradients and the Jacobian at a point from the modal
+//   coefficients, and the test sums collect modal moments R_b = sum_q (f0 m_b + P : grad m_b).
+// Both are the quadrature sums of the tables (identical up to rounding).  The kernels are
+// inlined: a term's inputs it never reads (s, s_t, x, ...) cost nothing.
+// ---------------------------------------------------------------------------------------------
+struct CellP2 {
+    static constexpr int NB = 10, NV = 4, NQ = 14;
+    double Ji[9], det;
+    __device__ void setup(const double X[4][3]) {
+        double J[9];
+        for (int d = 0; d < 3; ++d)
+            for (int k = 0; k < 3; ++k) J[3 * d + k] = X[k + 1][d] - X[0][d];
+        det = inverse3(J, Ji);
     }
-    double zero[3 * kMaxAuxValues];
-    for (int k = 0; k < 3 * kMaxAuxValues; ++k) zero[k] = 0.0;
-    double re[NB][3];
-    for (int b = 0; b < NB; ++b) re[b][0] = re[b][1] = re[b][2] = 0.0;
-    for (int q = 0; q < NQ; ++q) {
-        PointData<CT> P;
-        point_data<CT>(X, q, P);
-        double s[3], s_x[9], s_t[3], st_x[9];
-        eval_field<CT>(P, U, s, s_x);
-        eval_field<CT>(P, Ut, s_t, st_x);
-        const double *a = aux_record(A, cell, q, NQ);
-        double f0[3] = {0, 0, 0}, f1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        call_fn<F0>(A, aOff, aOff_x, a, zero, s, s_t, s_x, P.x, f0);
-        call_fn<F1>(A, aOff, aOff_x, a, zero, s, s_t, s_x, P.x, f1);
-        for (int b = 0; b < NB; ++b)
-            for (int c = 0; c < 3; ++c)
-                re[b][c] += P.wdet * (f0[c] * P.phi[b] + f1[3 * c] * P.grad[b][0] + f1[3 * c + 1] * P.grad[b][1] +
-                                      f1[3 * c + 2] * P.grad[b][2]);
+    // basis values and the reference gradient of a nodal field at barycentric point l
+    __device__ static void values(const double l[4], double phi[10]) {
+        const int edges[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+        for (int v = 0; v < 4; ++v) phi[v] = l[v] * (2.0 * l[v] - 1.0);
+        for (int e = 0; e < 6; ++e) phi[4 + e] = 4.0 * l[edges[e][0]] * l[edges[e][1]];
     }
-    for (int b = 0; b < NB; ++b) {
-        const size_t g = (size_t)A.closure[(size_t)cell * NB + b];
-        for (int c = 0; c < 3; ++c) A.f[3 * g + c] += re[b][c];
+    __device__ static void ref_grad(const double l[4], const double V[10][3], double G[9]) {
+        const double d0 = 4.0 * l[0] - 1.0, d1 = 4.0 * l[1] - 1.0, d2 = 4.0 * l[2] - 1.0, d3 = 4.0 * l[3] - 1.0;
+        for (int c = 0; c < 3; ++c) {
+            const double v0 = V[0][c], e01 = V[4][c], e02 = V[5][c], e03 = V[6][c], e12 = V[7][c], e13 = V[8][c],
+                         e23 = V[9][c];
+            const double base = -d0 * v0;
+            G[3 * c + 0] = base + d1 * V[1][c] +
+                           4.0 * ((l[0] - l[1]) * e01 - l[2] * e02 - l[3] * e03 + l[2] * e12 + l[3] * e13);
+            G[3 * c + 1] = base + d2 * V[2][c] +
+                           4.0 * (-l[1] * e01 + (l[0] - l[2]) * e02 - l[3] * e03 + l[1] * e12 + l[3] * e23);
+            G[3 * c + 2] = base + d3 * V[3][c] +
+                           4.0 * (-l[1] * e01 - l[2] * e02 + (l[0] - l[3]) * e03 + l[1] * e13 + l[2] * e23);
+        }
+    }
+};
+
+// re += the P2 element vector of the moments: f1 part from M[v][c * 3 + d] (physical),
+// f0 part R0[b][c] given per node.
+__device__ inline void p2_finish(const double Ji[9], const double M[4][9], double re[10][3]) {
+    double Mr[4][9], S[9];  // reference moments (component c, direction k) and their sum
+    for (int v = 0; v < 4; ++v)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k)
+                Mr[v][3 * c + k] = M[v][3 * c] * Ji[3 * k] + M[v][3 * c + 1] * Ji[3 * k + 1] +
+                                   M[v][3 * c + 2] * Ji[3 * k + 2];
+    for (int i = 0; i < 9; ++i) S[i] = Mr[0][i] + Mr[1][i] + Mr[2][i] + Mr[3][i];
+    // vertex a: (4 M_a - S) . e_a; edge (a, b): 4 (M_a . e_b + M_b . e_a); e_0 = -(1, 1, 1)
+    for (int c = 0; c < 3; ++c) {
+        double me0[4];
+        for (int v = 0; v < 4; ++v) me0[v] = -(Mr[v][3 * c] + Mr[v][3 * c + 1] + Mr[v][3 * c + 2]);
+        const double se0 = -(S[3 * c] + S[3 * c + 1] + S[3 * c + 2]);
+        re[0][c] += 4.0 * me0[0] - se0;
+        re[1][c] += 4.0 * Mr[1][3 * c + 0] - S[3 * c + 0];
+        re[2][c] += 4.0 * Mr[2][3 * c + 1] - S[3 * c + 1];
+        re[3][c] += 4.0 * Mr[3][3 * c + 2] - S[3 * c + 2];
+        re[4][c] += 4.0 * (Mr[0][3 * c + 0] + me0[1]);
+        re[5][c] += 4.0 * (Mr[0][3 * c + 1] + me0[2]);
+        re[6][c] += 4.0 * (Mr[0][3 * c + 2] + me0[3]);
+        re[7][c] += 4.0 * (Mr[1][3 * c + 1] + Mr[2][3 * c + 0]);
+        re[8][c] += 4.0 * (Mr[1][3 * c + 2] + Mr[3][3 * c + 0]);
+        re[9][c] += 4.0 * (Mr[2][3 * c + 2] + Mr[3][3 * c + 1]);
+    }
+}
+
+// Q1 modal transform: T[b] = sum over corners (PyLith vertex order) of F times the signs of the
+// corner's coordinates in b, b = xi + 2 eta + 4 zeta bits (butterflies in lexicographic order).
+// The transform is its own transpose, and applied twice gives 8 times the input.
+__device__ inline void q1_modal(const double F[8], double T[8]) {
+    const double f[8] = {F[0], F[1], F[3], F[2], F[4], F[5], F[7], F[6]};  // lexicographic corners
+    double a[8], t[8];
+    for (int p = 0; p < 4; ++p) {
+        a[2 * p] = f[2 * p] + f[2 * p + 1];
+        a[2 * p + 1] = f[2 * p + 1] - f[2 * p];
+    }
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < 2; ++i) {
+            t[i + 4 * k] = a[i + 4 * k] + a[i + 2 + 4 * k];
+            t[i + 2 + 4 * k] = a[i + 2 + 4 * k] - a[i + 4 * k];
+        }
+    for (int i = 0; i < 4; ++i) {
+        T[i] = t[i] + t[i + 4];
+        T[i + 4] = t[i + 4] - t[i];
+    }
+}
+
+// nodal values (PyLith vertex order) of the modal sums: r_v = sum_b R_b S_vb
+__device__ inline void q1_unmodal(const double R[8], double r[8]) {
+    double a[8], t[8];
+    for (int i = 0; i < 4; ++i) {  // zeta
+        a[i] = R[i] - R[i + 4];
+        a[i + 4] = R[i] + R[i + 4];
+    }
+    for (int k = 0; k < 2; ++k)  // eta
+        for (int i = 0; i < 2; ++i) {
+            t[i + 4 * k] = a[i + 4 * k] - a[i + 2 + 4 * k];
+            t[i + 2 + 4 * k] = a[i + 4 * k] + a[i + 2 + 4 * k];
+        }
+    double lx[8];
+    for (int p = 0; p < 4; ++p) {  // xi
+        lx[2 * p] = t[2 * p] - t[2 * p + 1];
+        lx[2 * p + 1] = t[2 * p] + t[2 * p + 1];
+    }
+    r[0] = lx[0];
+    r[1] = lx[1];
+    r[2] = lx[3];
+    r[3] = lx[2];
+    r[4] = lx[4];
+    r[5] = lx[5];
+    r[6] = lx[7];
+    r[7] = lx[6];
+}
+
+// Point q of the 2x2x2 rule: modal functions m_b, reference point; weight 1.
+struct Q1Point {
+    double xi, eta, zeta, m[8];
+    __device__ Q1Point(int q) {
+        const double g = 0.57735026918962576451;
+        xi = (q & 1) ? g : -g;
+        eta = (q & 2) ? g : -g;
+        zeta = (q & 4) ? g : -g;
+        m[0] = 1.0;
+        m[1] = xi;
+        m[2] = eta;
+        m[3] = xi * eta;
+        m[4] = zeta;
+        m[5] = xi * zeta;
+        m[6] = eta * zeta;
+        m[7] = xi * eta * zeta;
+    }
+    // value and reference gradient of a field from its modal coefficients over 8
+    __device__ double value(const double T[8]) const {
+        double v = 0.0;
+        for (int b = 0; b < 8; ++b) v += T[b] * m[b];
+        return v;
+    }
+    __device__ void grad(const double T[8], double g[3]) const {
+        g[0] = T[1] + T[3] * eta + T[5] * zeta + T[7] * m[6];
+        g[1] = T[2] + T[3] * xi + T[6] * zeta + T[7] * m[5];
+        g[2] = T[4] + T[5] * xi + T[6] * eta + T[7] * m[3];
+    }
+    // R_b += P . grad m_b (reference flux P[k])
+    __device__ void add_flux(const double P[3], double R[8]) const {
+        R[1] += P[0];
+        R[2] += P[1];
+        R[4] += P[2];
+        R[3] += eta * P[0] + xi * P[1];
+        R[5] += zeta * P[0] + xi * P[2];
+        R[6] += zeta * P[1] + eta * P[2];
+        R[7] += m[6] * P[0] + m[5] * P[1] + m[3] * P[2];
+    }
+};
+
+// Modal coefficients over 8 of the three components of a nodal field V[8][3].
+__device__ inline void q1_modal3(const double V[8][3], double T[3][8]) {
+    for (int c = 0; c < 3; ++c) {
+        double f[8];
+        for (int v = 0; v < 8; ++v) f[v] = V[v][c];
+        q1_modal(f, T[c]);
+        for (int b = 0; b < 8; ++b) T[c][b] *= 0.125;
     }
 }
 
@@ -376,37 +505,29 @@ __device__ inline void point_jacobian(const plx_pointfn_args &A, const int *aOff
     call_jac<J3>(A, aOff, aOff_x, a, zero, s, s_t, s_x, x, g3);
 }
 
-// f += J(u) v over the cells [c0, c1) of one colour.
-template <class J0, class J1, class J2, class J3, int CT>
-__global__ void __launch_bounds__(128) action_cells(plx_pointfn_args A, int c0, int c1) {
-    constexpr int NB = Basis<CT>::NB, NV = Basis<CT>::NV, NQ = Basis<CT>::NQ;
-    const int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c1) return;
-    const int cell = A.cells[i];
-    double X[NV][3], U[NB][3], Ut[NB][3], V[NB][3];
-    gather_cell<CT>(A, cell, X, A.u, U);
-    gather_cell<CT>(A, cell, X, A.u_t, Ut);
-    gather_cell<CT>(A, cell, X, A.v, V);
-    int aOff[16], aOff_x[16];
-    for (int k = 0; k < A.num_aux && k < 16; ++k) {
-        aOff[k] = A.aux_off[k];
-        aOff_x[k] = 3 * A.aux_off[k];
+// A term's output at a point: the residual's f0 / f1, or the action's t0 / t1 built from the
+// pointwise Jacobians and the direction v.
+template <class F0, class F1>
+struct ResidualTerms {
+    static constexpr bool has0 = !std::is_same<F0, None>::value;
+    __device__ static void eval(const plx_pointfn_args &A, const int *aOff, const int *aOff_x, const double *a,
+                                const double *zero, const double s[3], const double s_t[3], const double s_x[9],
+                                const double *, const double *, const double x[3], double f0[3], double f1[9]) {
+        for (int c = 0; c < 3; ++c) f0[c] = 0.0;
+        for (int k = 0; k < 9; ++k) f1[k] = 0.0;
+        call_fn<F0>(A, aOff, aOff_x, a, zero, s, s_t, s_x, x, f0);
+        call_fn<F1>(A, aOff, aOff_x, a, zero, s, s_t, s_x, x, f1);
     }
-    double zero[3 * kMaxAuxValues];
-    for (int k = 0; k < 3 * kMaxAuxValues; ++k) zero[k] = 0.0;
-    double re[NB][3];
-    for (int b = 0; b < NB; ++b) re[b][0] = re[b][1] = re[b][2] = 0.0;
-    for (int q = 0; q < NQ; ++q) {
-        PointData<CT> P;
-        point_data<CT>(X, q, P);
-        double s[3], s_x[9], s_t[3], st_x[9], v[3], v_x[9];
-        eval_field<CT>(P, U, s, s_x);
-        eval_field<CT>(P, Ut, s_t, st_x);
-        eval_field<CT>(P, V, v, v_x);
+};
+
+template <class J0, class J1, class J2, class J3>
+struct ActionTerms {
+    static constexpr bool has0 = !std::is_same<J0, None>::value || !std::is_same<J1, None>::value;
+    __device__ static void eval(const plx_pointfn_args &A, const int *aOff, const int *aOff_x, const double *a,
+                                const double *zero, const double s[3], const double s_t[3], const double s_x[9],
+                                const double *v, const double *v_x, const double x[3], double t0[3], double t1[9]) {
         double g0[9], g1[27], g2[27], g3[81];
-        point_jacobian<J0, J1, J2, J3>(A, aOff, aOff_x, aux_record(A, cell, q, NQ), zero, s, s_t, s_x, P.x, g0, g1,
-                                       g2, g3);
-        double t0[3], t1[9];
+        point_jacobian<J0, J1, J2, J3>(A, aOff, aOff_x, a, zero, s, s_t, s_x, x, g0, g1, g2, g3);
         for (int f = 0; f < 3; ++f) {
             t0[f] = 0.0;
             for (int g = 0; g < 3; ++g) {
@@ -422,15 +543,163 @@ __global__ void __launch_bounds__(128) action_cells(plx_pointfn_args A, int c0, 
                 t1[3 * f + d] = acc;
             }
         }
-        for (int b = 0; b < NB; ++b)
-            for (int c = 0; c < 3; ++c)
-                re[b][c] += P.wdet * (t0[c] * P.phi[b] + t1[3 * c] * P.grad[b][0] + t1[3 * c + 1] * P.grad[b][1] +
-                                      t1[3 * c + 2] * P.grad[b][2]);
     }
-    for (int b = 0; b < NB; ++b) {
-        const size_t g = (size_t)A.closure[(size_t)cell * NB + b];
+};
+
+// Zero arrays for the a_t / a_x arguments (piecewise-constant aux without time derivative).
+static __constant__ double kZeroAux[3 * kMaxAuxValues];
+
+// f += the element vectors of the cells [c0, c1) of one colour: TERMS gives (f0, f1) at a point
+// from the solution u (and, for the action, the direction v).
+template <class TERMS, bool ACTION>
+__device__ inline void integrate_p2(const plx_pointfn_args &A, int cell) {
+    double X[4][3], U[10][3], Ut[10][3], V[10][3];
+    gather_cell<PLX_TET_P2>(A, cell, X, A.u, U);
+    if (A.u_t) gather_cell<PLX_TET_P2>(A, cell, X, A.u_t, Ut);
+    if (ACTION) gather_cell<PLX_TET_P2>(A, cell, X, A.v, V);
+    int aOff[16], aOff_x[16];
+    for (int k = 0; k < A.num_aux && k < 16; ++k) {
+        aOff[k] = A.aux_off[k];
+        aOff_x[k] = 3 * A.aux_off[k];
+    }
+    CellP2 C;
+    C.setup(X);
+    double M[4][9], re[10][3];
+    for (int v = 0; v < 4; ++v)
+        for (int i = 0; i < 9; ++i) M[v][i] = 0.0;
+    for (int b = 0; b < 10; ++b) re[b][0] = re[b][1] = re[b][2] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < CellP2::NQ; ++q) {
+        double l[4], w, phi[10];
+        Basis<PLX_TET_P2>::rule(q, l, w);
+        CellP2::values(l, phi);
+        const double wdet = w * C.det;
+        double s[3] = {0, 0, 0}, s_t[3] = {0, 0, 0}, x[3], G[9], s_x[9];
+        for (int b = 0; b < 10; ++b)
+            for (int c = 0; c < 3; ++c) s[c] += U[b][c] * phi[b];
+        if (A.u_t)
+            for (int b = 0; b < 10; ++b)
+                for (int c = 0; c < 3; ++c) s_t[c] += Ut[b][c] * phi[b];
+        for (int d = 0; d < 3; ++d) x[d] = l[0] * X[0][d] + l[1] * X[1][d] + l[2] * X[2][d] + l[3] * X[3][d];
+        CellP2::ref_grad(l, U, G);
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+                s_x[3 * c + d] = G[3 * c] * C.Ji[d] + G[3 * c + 1] * C.Ji[3 + d] + G[3 * c + 2] * C.Ji[6 + d];
+        double v[3] = {0, 0, 0}, v_x[9];
+        if (ACTION) {
+            for (int b = 0; b < 10; ++b)
+                for (int c = 0; c < 3; ++c) v[c] += V[b][c] * phi[b];
+            CellP2::ref_grad(l, V, G);
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d)
+                    v_x[3 * c + d] = G[3 * c] * C.Ji[d] + G[3 * c + 1] * C.Ji[3 + d] + G[3 * c + 2] * C.Ji[6 + d];
+        }
+        double f0[3], f1[9];
+        TERMS::eval(A, aOff, aOff_x, aux_record(A, cell, q, CellP2::NQ), kZeroAux, s, s_t, s_x, v, v_x, x, f0, f1);
+        for (int i = 0; i < 9; ++i) {
+            const double wf = wdet * f1[i];
+            M[0][i] += l[0] * wf;
+            M[1][i] += l[1] * wf;
+            M[2][i] += l[2] * wf;
+            M[3][i] += l[3] * wf;
+        }
+        if (TERMS::has0)
+            for (int b = 0; b < 10; ++b)
+                for (int c = 0; c < 3; ++c) re[b][c] += wdet * f0[c] * phi[b];
+    }
+    p2_finish(C.Ji, M, re);
+    for (int b = 0; b < 10; ++b) {
+        const size_t g = (size_t)A.closure[(size_t)cell * 10 + b];
         for (int c = 0; c < 3; ++c) A.f[3 * g + c] += re[b][c];
     }
+}
+
+template <class TERMS, bool ACTION>
+__device__ inline void integrate_q1(const plx_pointfn_args &A, int cell) {
+    double X[8][3], TX[3][8], T[3][8], Tt[3][8], TV[3][8];
+    {
+        double W[8][3];
+        gather_cell<PLX_HEX_Q1>(A, cell, X, A.u, W);
+        q1_modal3(W, T);
+        if (A.u_t) {
+            gather_cell<PLX_HEX_Q1>(A, cell, X, A.u_t, W);
+            q1_modal3(W, Tt);
+        }
+        if (ACTION) {
+            gather_cell<PLX_HEX_Q1>(A, cell, X, A.v, W);
+            q1_modal3(W, TV);
+        }
+        q1_modal3(X, TX);
+    }
+    int aOff[16], aOff_x[16];
+    for (int k = 0; k < A.num_aux && k < 16; ++k) {
+        aOff[k] = A.aux_off[k];
+        aOff_x[k] = 3 * A.aux_off[k];
+    }
+    double R[3][8];
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 8; ++b) R[c][b] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        const Q1Point P(q);
+        double J[9], Ji[9];
+        for (int d = 0; d < 3; ++d) P.grad(TX[d], J + 3 * d);
+        const double wdet = inverse3(J, Ji);  // weight 1
+        double s[3], s_t[3] = {0, 0, 0}, x[3], s_x[9];
+        for (int c = 0; c < 3; ++c) {
+            double G[3];
+            s[c] = P.value(T[c]);
+            if (A.u_t) s_t[c] = P.value(Tt[c]);
+            P.grad(T[c], G);
+            for (int d = 0; d < 3; ++d) s_x[3 * c + d] = G[0] * Ji[d] + G[1] * Ji[3 + d] + G[2] * Ji[6 + d];
+            x[c] = P.value(TX[c]);
+        }
+        double v[3] = {0, 0, 0}, v_x[9];
+        if (ACTION)
+            for (int c = 0; c < 3; ++c) {
+                double G[3];
+                v[c] = P.value(TV[c]);
+                P.grad(TV[c], G);
+                for (int d = 0; d < 3; ++d) v_x[3 * c + d] = G[0] * Ji[d] + G[1] * Ji[3 + d] + G[2] * Ji[6 + d];
+            }
+        double f0[3], f1[9];
+        TERMS::eval(A, aOff, aOff_x, aux_record(A, cell, q, 8), kZeroAux, s, s_t, s_x, v, v_x, x, f0, f1);
+        for (int c = 0; c < 3; ++c) {
+            // reference flux P_k = w|J| sum_d f1[c][d] Jinv[k][d]
+            double Pk[3];
+            for (int k = 0; k < 3; ++k)
+                Pk[k] = wdet * (f1[3 * c] * Ji[3 * k] + f1[3 * c + 1] * Ji[3 * k + 1] + f1[3 * c + 2] * Ji[3 * k + 2]);
+            P.add_flux(Pk, R[c]);
+            if (TERMS::has0)
+                for (int b = 0; b < 8; ++b) R[c][b] += wdet * f0[c] * P.m[b];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        double r[8];
+        q1_unmodal(R[c], r);  // the modes of the test functions carry the factor 1/8
+        for (int v = 0; v < 8; ++v) {
+            const size_t g = (size_t)A.closure[(size_t)cell * 8 + v];
+            A.f[3 * g + c] += 0.125 * r[v];
+        }
+    }
+}
+
+// f += sum_q w|J| (f0 . phi + f1 : grad phi) over the cells [c0, c1) of one colour.
+template <class F0, class F1, int CT>
+__global__ void __launch_bounds__(128) residual_cells(plx_pointfn_args A, int c0, int c1) {
+    const int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c1) return;
+    if constexpr (CT == PLX_TET_P2) integrate_p2<ResidualTerms<F0, F1>, false>(A, A.cells[i]);
+    else integrate_q1<ResidualTerms<F0, F1>, false>(A, A.cells[i]);
+}
+
+// f += J(u) v over the cells [c0, c1) of one colour.
+template <class J0, class J1, class J2, class J3, int CT>
+__global__ void __launch_bounds__(128) action_cells(plx_pointfn_args A, int c0, int c1) {
+    const int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c1) return;
+    if constexpr (CT == PLX_TET_P2) integrate_p2<ActionTerms<J0, J1, J2, J3>, true>(A, A.cells[i]);
+    else integrate_q1<ActionTerms<J0, J1, J2, J3>, true>(A, A.cells[i]);
 }
 
 // values += the element blocks of the cells [c0, c1) of one colour; thread = (cell, row node a).
