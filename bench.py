@@ -648,11 +648,12 @@ def main():
     gms = graph_apply_ms(dm, apply64, torch)
     B = p2_bytes(mesh.num_nodes, mesh.num_vertices, mesh.num_cells, 8)
     achieved = B / (kms * 1e-3) / 1e9
-    traffic = None
+    traffic = traffic_build = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic = tj.get("p2_fp64_residual_64cube")
+        traffic_build = tj.get("_build")
     result = {
         "metric": "residual-apply GDoF/s (% HBM roofline)",
         "value": ndofs / sec / 1e9,
@@ -687,6 +688,8 @@ def main():
             "frac_apply_kernels": B / ((kms + hms) * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "frac_apply_wall": B / sec / 1e9 / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_basis": "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 of this kernel (profiles/traffic.json, "
+                             f"build {traffic_build})",
             "algorithmic_bytes": B,
             "kernel_avg_ms": kms,
             "kernel_launches_timed": kcount,

@@ -2,9 +2,12 @@
 
     python -m pylith_amd.profiling stats  <results.db>            per-kernel dispatch statistics
     python -m pylith_amd.profiling counter <results.db> [kernel]  per-kernel counter averages
+    python -m pylith_amd.profiling copy                           the calibration workload
 
 The Python-side counterpart of PyLith's profiling helpers (pylith/utils/profiling.py): it reads
-what the profiler recorded and prints compact tables; it launches nothing.
+what the profiler recorded and prints compact tables.  `copy` is the one command that runs GPU
+work: three device-to-device copies of 2^27 doubles (1 GiB read and 1 GiB written each), run
+under `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE) to scale those counters to bytes.
 """
 import sqlite3
 import sys
@@ -37,7 +40,21 @@ def _short(name, width=70):
     return name if len(name) <= width else name[:width - 3] + "..."
 
 
+def copy_workload(n=1 << 27, reps=3):
+    """Device copies of n doubles (the counters' calibration: bytes moved are known)."""
+    import torch
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {reps} x {8 * n} bytes", flush=True)
+
+
 def main(argv):
+    if len(argv) == 2 and argv[1] == "copy":
+        copy_workload()
+        return 0
     if len(argv) < 3 or argv[1] not in ("stats", "counter"):
         print(__doc__)
         return 2
