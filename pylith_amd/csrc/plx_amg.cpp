@@ -32,19 +32,25 @@ void parallel_chunks(int n, int T, F f) {
 // Inverse of a b x b block (b <= 6) by Gauss-Jordan with partial pivoting; a singular block
 // (zero pivot) yields the identity (its rows are decoupled dummy dofs).
 void block_inverse(const double *A, int b, double *Ai) {
+    // rows equilibrated to unit max norm first (solve (S A) X = S): a partly constrained node's
+    // block holds an identity row of 1 next to stiffness rows many orders larger
     alignas(64) double M[6][12];
-    double scale = 0.0;
-    for (int i = 0; i < b; ++i)
+    bool singular = false;
+    for (int i = 0; i < b; ++i) {
+        double rs = 0.0;
+        for (int j = 0; j < b; ++j) rs = std::max(rs, std::fabs(A[i * b + j]));
+        singular = singular || !(rs > 0.0);
+        const double s = rs > 0.0 ? 1.0 / rs : 0.0;
         for (int j = 0; j < b; ++j) {
-            M[i][j] = A[i * b + j];
-            M[i][b + j] = i == j ? 1.0 : 0.0;
-            scale = std::max(scale, std::fabs(A[i * b + j]));
+            M[i][j] = A[i * b + j] * s;
+            M[i][b + j] = i == j ? s : 0.0;
         }
+    }
     for (int c = 0; c < b; ++c) {
         int p = c;
         for (int r = c + 1; r < b; ++r)
             if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
-        if (!(std::fabs(M[p][c]) > 1e-14 * scale)) {
+        if (singular || !(std::fabs(M[p][c]) > 1e-14)) {
             for (int i = 0; i < b * b; ++i) Ai[i] = (i % (b + 1) == 0) ? 1.0 : 0.0;
             return;
         }
@@ -148,7 +154,8 @@ void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const do
 }  // namespace
 
 int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *val0, const double *coords,
-              const uint8_t *fixed0, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err) {
+              const uint8_t *fixed0, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err,
+              double theta) {
     h->levels.clear();
     h->coarse_inv.clear();
     // current level operator
@@ -189,6 +196,23 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
             for (int c = 0; c < b; ++c) all = all && fixed[(size_t)i * b + c];
             excl[i] = all;
         }
+        // strong couplings: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F) (theta = 0: the whole
+        // node graph)
+        std::vector<uint8_t> strong(Acol.size(), 1);
+        if (theta > 0.0) {
+            std::vector<double> dn(n, 0.0);
+            auto fro = [&](size_t e) {
+                double t = 0.0;
+                for (int k = 0; k < bb; ++k) t += Aval[e * bb + k] * Aval[e * bb + k];
+                return std::sqrt(t);
+            };
+            for (int i = 0; i < n; ++i)
+                for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+                    if (Acol[e] == i) dn[i] = fro((size_t)e);
+            for (int i = 0; i < n; ++i)
+                for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+                    strong[e] = Acol[e] == i || fro((size_t)e) >= theta * std::sqrt(dn[i] * dn[Acol[e]]);
+        }
         std::vector<int> agg(n, -1);
         int na = 0;
         for (int i = 0; i < n; ++i) {
@@ -196,18 +220,18 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
             bool free_nbhd = true;
             for (int e = Aptr[i]; e < Aptr[i + 1] && free_nbhd; ++e) {
                 const int j = Acol[e];
-                if (!excl[j] && agg[j] >= 0) free_nbhd = false;
+                if (strong[e] && !excl[j] && agg[j] >= 0) free_nbhd = false;
             }
             if (!free_nbhd) continue;
             for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
-                if (!excl[Acol[e]]) agg[Acol[e]] = na;
+                if (strong[e] && !excl[Acol[e]]) agg[Acol[e]] = na;
             agg[i] = na++;
         }
         for (int i = 0; i < n; ++i) {
             if (excl[i] || agg[i] >= 0) continue;
             for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
                 const int j = Acol[e];
-                if (!excl[j] && agg[j] >= 0) {
+                if (strong[e] && !excl[j] && agg[j] >= 0) {
                     agg[i] = agg[j];
                     break;
                 }

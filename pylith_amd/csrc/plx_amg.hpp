@@ -36,8 +36,11 @@ struct AmgHierarchy {
 };
 
 // Build the hierarchy of the 3x3-block operator (ptr, col, val) on n nodes.  fixed[3 n] marks
-// identity rows/columns (constrained dofs, positive split nodes); coords [n][3].
+// identity rows/columns (constrained dofs, positive split nodes); coords [n][3].  Aggregates
+// grow over the strong couplings of each level, ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F)
+// (theta = 0: every coupling of the node graph).
 int amg_setup(int n, const int32_t *ptr, const int32_t *col, const double *val, const double *coords,
-              const uint8_t *fixed, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err);
+              const uint8_t *fixed, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err,
+              double theta = 0.0);
 
 }  // namespace plx

@@ -717,6 +717,7 @@ struct plx_gf_s {
     // cache lines of the neighbour gathers): perm[row] is the DM node of a row
     std::vector<int32_t> perm;
     int32_t *d_perm = nullptr;
+    std::vector<double> xyz;  // [nrows][3] node coordinates in row order (multigrid rigid-body modes)
     double *d_val64 = nullptr;
     float *d_val32 = nullptr;
     double *d_Binv64 = nullptr;
@@ -737,6 +738,9 @@ struct plx_gf_s {
     std::vector<double> probe_us;  // per-RHS per-iteration time of each probed width
     // smoothed-aggregation multigrid preconditioner (PLX_GF_PC_AMG)
     int pc = 0;
+    // strength threshold of the aggregation: Q1 aggregates over the whole node graph; the P2
+    // graph (vertex and edge nodes, up to 65 neighbours a row) over its strong couplings
+    double amg_theta = 0.0;
     struct Level {
         int n = 0, b = 0, nc = 0;
         int32_t *Aptr = nullptr, *Acol = nullptr, *Pptr = nullptr, *Pcol = nullptr, *Rptr = nullptr, *Rcol = nullptr;
@@ -1090,13 +1094,11 @@ static int gf_amg_setup(plx_gf g) {
                 }
             fixed[3 * (size_t)i + c] = ident;
         }
-    // node coordinates in row order (the aggregation's rigid-body modes)
-    std::vector<double> xyz(3 * (size_t)nn);
-    for (int i = 0; i < nn; ++i)
-        for (int d = 0; d < 3; ++d) xyz[3 * (size_t)i + d] = dm->coords[3 * (size_t)g->perm[i] + d];
     AmgHierarchy h;
     std::string msg;
-    if (amg_setup(nn, ptr.data(), col.data(), val.data(), xyz.data(), fixed.data(), 768, 8, &h, &msg))
+    if (g->xyz.size() != 3 * (size_t)nn) return set_error(dm, 1, "node coordinates missing");
+    if (amg_setup(nn, ptr.data(), col.data(), val.data(), g->xyz.data(), fixed.data(), 768, 8, &h, &msg,
+                  g->amg_theta))
         return set_error(dm, 1, msg.c_str());
     gf_amg_free(g);
     auto upf = [&](float **d, const std::vector<double> &v) -> int {
@@ -1145,6 +1147,144 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
              const std::vector<int32_t> &inc_ptr, const std::vector<int32_t> &inc,
              const std::vector<int32_t> &mcl, const std::vector<int32_t> &diag_pos,
              const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos);
+int gf_alloc_work(plx_gf g);
+
+// Handle over an operator assembled on the host side (row_ptr / col / val in row order, diagonal
+// block of row i at diag_pos[i]): upload, inverse diagonal blocks, FP32 copies, workspace.
+static int gf_setup_assembled(plx_gf g, const std::vector<int32_t> &row_ptr, const std::vector<int32_t> &col,
+                              const std::vector<int32_t> &diag_pos, const std::vector<double> &val) {
+    plx_dm dm = g->dm;
+    const int nn = g->nrows;
+    auto up = [&](auto **d, const auto &h) -> int {
+        using T = typename std::decay<decltype(h)>::type::value_type;
+        PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, h.size()) * sizeof(T)));
+        PLX_HIP(dm, hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    };
+    int32_t *d_diag = nullptr;
+    int err = 0;
+    if ((err = up(&g->d_row_ptr, row_ptr)) || (err = up(&g->d_col, col)) || (err = up(&g->d_perm, g->perm)) ||
+        (err = up(&g->d_val64, val)) || (err = up(&d_diag, diag_pos)))
+        return err;
+    PLX_HIP(dm, hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    PLX_HIP(dm, hipMalloc(&g->d_val32, (size_t)g->nnzb * 9 * sizeof(float)));
+    PLX_HIP(dm, hipMalloc(&g->d_Binv64, 9 * (size_t)nn * sizeof(double)));
+    PLX_HIP(dm, hipMalloc(&g->d_Binv32, 9 * (size_t)nn * sizeof(float)));
+    hipLaunchKernelGGL(gf_diag_kernel, dim3((nn + 255) / 256), dim3(256), 0, g->stream, nn, d_diag, g->d_val64,
+                       g->d_Binv64, g->d_Binv32, g->d_val32, g->nnzb);
+    PLX_HIP(dm, hipGetLastError());
+    PLX_HIP(dm, hipStreamSynchronize(g->stream));
+    (void)hipFree(d_diag);
+    return gf_alloc_work(g);
+}
+
+// Operator handle from the assembled Jacobian of the keys (plx_compute_jacobian, linearised at
+// zero displacement), negated into the positive operator -J of the Q1 path: the path for P2
+// tetrahedra, whose element matrices the Q1 kernels above do not form.  Rows are the mesh
+// nodes in Morton order of their coordinates (an edge node sits at its edge's midpoint); a
+// constrained dof gets an identity row and a zero column.  The handle serves products and the
+// multigrid preconditioner; fault impulses need the Q1 handle.
+static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan *> &keys) {
+    if (dm->fault) return set_error(dm, 1, "Green's functions with a fault are implemented for Q1 hexes");
+    const int nn = dm->num_nodes, nb = dm->nb, nv = dm->nv;
+    int64_t nnzb = 0;
+    int err = plx_jacobian_pattern(dm, &nnzb, nullptr, nullptr);
+    if (err) return err;
+    std::vector<int32_t> rp0(nn + 1), col0((size_t)nnzb);
+    if ((err = plx_jacobian_pattern(dm, &nnzb, rp0.data(), col0.data()))) return err;
+    std::vector<double> val0((size_t)nnzb * 9);
+    {
+        double *d_val = nullptr, *d_x = nullptr;
+        if (hipMalloc(&d_val, val0.size() * sizeof(double)) != hipSuccess ||
+            hipMalloc(&d_x, std::max<int64_t>(1, dm->local_size) * sizeof(double)) != hipSuccess) {
+            (void)hipFree(d_val);
+            return set_error(dm, 1, "out of device memory for the assembled operator");
+        }
+        if (hipMemsetAsync(d_x, 0, dm->local_size * sizeof(double), dm->stream) != hipSuccess)
+            err = set_error(dm, 76, "hipMemsetAsync failed");
+        for (size_t k = 0; k < keys.size() && !err; ++k)
+            err = plx_compute_jacobian(dm, keys[k]->label_value, keys[k]->part, 0.0, 0.0, d_x, nullptr, d_val,
+                                       k == 0 ? PLX_INSERT_VALUES : PLX_ADD_VALUES);
+        if (!err && (hipStreamSynchronize(dm->stream) != hipSuccess ||
+                     hipMemcpy(val0.data(), d_val, val0.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+            err = set_error(dm, 76, "downloading the assembled operator failed");
+        (void)hipFree(d_val);
+        (void)hipFree(d_x);
+        if (err) return err;
+    }
+    // node coordinates: the vertex of a vertex node, the midpoint of an edge node's edge (P2
+    // closure order: 4 vertices, then edges 01 02 03 12 13 23)
+    static const int kEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    std::vector<double> nx(3 * (size_t)nn, 0.0);
+    for (int c = 0; c < dm->num_cells; ++c) {
+        const int32_t *cl = &dm->closure[(size_t)c * nb];
+        const int32_t *cv = &dm->cone[(size_t)c * nv];
+        for (int b = 0; b < nb; ++b) {
+            const double *p = nullptr, *q = nullptr;
+            if (b < nv) {
+                p = q = &dm->coords[3 * (size_t)cv[b]];
+            } else if (dm->cell_type == PLX_TET_P2) {
+                p = &dm->coords[3 * (size_t)cv[kEdge[b - 4][0]]];
+                q = &dm->coords[3 * (size_t)cv[kEdge[b - 4][1]]];
+            } else {
+                return set_error(dm, 1, "no node coordinates for this cell type");
+            }
+            for (int d = 0; d < 3; ++d) nx[3 * (size_t)cl[b] + d] = 0.5 * (p[d] + q[d]);
+        }
+    }
+    std::vector<int32_t> perm = morton_permutation(nx.data(), nn), iperm(nn);
+    for (int i = 0; i < nn; ++i) iperm[perm[i]] = i;
+    std::vector<uint8_t> conmask(3 * (size_t)nn, 0);
+    for (int32_t d : dm->con.dofs) {
+        if (d < 0 || d >= 3 * nn) return set_error(dm, 1, "constrained dof out of range");
+        conmask[d] = 1;
+    }
+    // rows in Morton order, columns renumbered and ascending
+    std::vector<int32_t> row_ptr(nn + 1, 0), col((size_t)nnzb), diag_pos(nn, -1);
+    std::vector<double> val((size_t)nnzb * 9);
+    std::vector<std::pair<int32_t, int32_t>> rowv;
+    for (int r = 0; r < nn; ++r) {
+        const int i = perm[r];
+        rowv.clear();
+        for (int e = rp0[i]; e < rp0[i + 1]; ++e) rowv.push_back({iperm[col0[e]], e});
+        std::sort(rowv.begin(), rowv.end());
+        int64_t o = row_ptr[r];
+        for (const auto &ce : rowv) {
+            const int j = perm[ce.first];
+            col[o] = ce.first;
+            if (ce.first == r) diag_pos[r] = (int32_t)o;
+            for (int f = 0; f < 3; ++f)
+                for (int h = 0; h < 3; ++h) {
+                    double v = -val0[9 * (size_t)ce.second + 3 * f + h];  // the operator -J
+                    if (conmask[3 * (size_t)i + f] || conmask[3 * (size_t)j + h]) v = (i == j && f == h) ? 1.0 : 0.0;
+                    val[9 * (size_t)o + 3 * f + h] = v;
+                }
+            ++o;
+        }
+        row_ptr[r + 1] = (int32_t)o;
+        if (diag_pos[r] < 0) return set_error(dm, 1, "assembled operator without a diagonal block");
+    }
+    std::vector<double>().swap(val0);
+    plx_gf g = new plx_gf_s();
+    g->dm = dm;
+    g->kp = keys[0];
+    g->keys = keys;
+    g->nrows = nn;
+    g->ndof = 3 * (int64_t)nn;
+    g->nnzb = nnzb;
+    g->op_epoch = dm->op_epoch;
+    g->amg_theta = 0.1;
+    g->xyz.resize(3 * (size_t)nn);
+    for (int r = 0; r < nn; ++r)
+        for (int d = 0; d < 3; ++d) g->xyz[3 * (size_t)r + d] = nx[3 * (size_t)perm[r] + d];
+    g->perm = std::move(perm);
+    if ((err = gf_setup_assembled(g, row_ptr, col, diag_pos, val))) {
+        plx_gf_destroy(g);
+        return err;
+    }
+    *out = g;
+    return 0;
+}
 
 // One V-cycle on a single column, z = M r (3 x num_nodes FP32 vectors in DM node order), queued
 // on `stream`; used by the matrix-free solver as its multigrid preconditioner.
@@ -1169,7 +1309,6 @@ extern "C" {
 int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     if (!out || !dm) return 1;
     *out = nullptr;
-    if (dm->cell_type != PLX_HEX_Q1) return set_error(dm, 1, "Green's functions are implemented for Q1 hexes");
     std::vector<KeyPlan *> keys;
     if (label_value == PLX_ALL_KEYS) {
         for (KeyPlan *k : dm->keys)
@@ -1184,6 +1323,7 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
         allcells.insert(allcells.end(), k->cells.begin(), k->cells.end());
     }
     if (allcells.size() != (size_t)dm->num_cells) return set_error(dm, 1, "the keys must cover every cell");
+    if (dm->cell_type != PLX_HEX_Q1) return gf_create_assembled(out, dm, keys);
     const int nn = dm->num_nodes, nc = (int)allcells.size();
     // handle rows: the DM's nodes in Morton order of their coordinates (Q1: the nodes are the
     // vertices), perm[row] = node, iperm[node] = row
@@ -1256,6 +1396,11 @@ int plx_gf_create(plx_gf *out, plx_dm dm, int label_value, int part) {
     g->ndof = 3 * (int64_t)nn;
     g->nnzb = (int64_t)col.size();
     g->op_epoch = dm->op_epoch;
+    if (dm->coords.size() == 3 * (size_t)nn) {  // else no multigrid (it needs the coordinates)
+        g->xyz.resize(3 * (size_t)nn);
+        for (int r = 0; r < nn; ++r)
+            for (int d = 0; d < 3; ++d) g->xyz[3 * (size_t)r + d] = dm->coords[3 * (size_t)perm[r] + d];
+    }
     g->perm = std::move(perm);
     const int err = gf_setup(g, nc, row_ptr, col, inc_ptr, inc, mcl, diag_pos, conmask, is_pos);
     if (err) {
@@ -1316,6 +1461,12 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
     for (void *p : {(void *)d_inc_ptr, (void *)d_inc, (void *)d_mcl, (void *)d_diag, (void *)d_con, (void *)d_pos,
                     (void *)d_Ke})
         (void)hipFree(p);
+    return gf_alloc_work(g);
+}
+
+// Reduction scalars and single-vector workspace of a handle whose operator is in place.
+int gf_alloc_work(plx_gf g) {
+    plx_dm dm = g->dm;
     PLX_HIP(dm, hipMalloc(&g->scal, G_COUNT * 64 * sizeof(double)));
     PLX_HIP(dm, hipMalloc(&g->prz, (size_t)kGfRed * 64 * sizeof(double)));
     PLX_HIP(dm, hipMalloc(&g->prr, (size_t)kGfRed * 64 * sizeof(double)));

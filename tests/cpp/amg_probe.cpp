@@ -16,6 +16,7 @@ AmgHierarchy H;
 std::vector<int32_t> A0p, A0c;
 std::vector<double> A0v;
 std::string last_error;
+double theta = 0.0;  // strength-of-connection threshold of the next build
 
 void spmm(int nr, int br, int bc, const int32_t *p, const int32_t *c, const double *v, const double *x,
           double *y, bool acc) {
@@ -86,9 +87,11 @@ int amg_probe_build(int n, const int32_t *ptr, const int32_t *col, const double 
     A0p.assign(ptr, ptr + n + 1);
     A0c.assign(col, col + ptr[n]);
     A0v.assign(val, val + (size_t)ptr[n] * 9);
-    if (plx::amg_setup(n, ptr, col, val, coords, fixed, coarse_max, max_levels, &H, &last_error)) return -1;
+    if (plx::amg_setup(n, ptr, col, val, coords, fixed, coarse_max, max_levels, &H, &last_error, theta)) return -1;
     return (int)H.levels.size();
 }
+
+void amg_probe_set_theta(double t) { theta = t; }
 
 const char *amg_probe_error() { return last_error.c_str(); }
 

@@ -59,15 +59,22 @@ def test_solve_matches_dense_direct(oracle, kind, method):
     assert err <= 1e-9, (err, st)
 
 
+@pytest.mark.parametrize("kind", ["hex", "tet"])
 @pytest.mark.parametrize("method", [L.KSP_CG, L.KSP_FGMRES_IR, L.KSP_IR])
-def test_multigrid_solve_matches_dense_direct(oracle, method):
-    m = mg.hex_box(6, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+def test_multigrid_solve_matches_dense_direct(oracle, kind, method):
+    """The V-cycle preconditioner: Q1 from the element-matrix assembly, P2 from the assembled
+    Jacobian (plx_compute_jacobian) renumbered in Morton order; 1,331 P2 nodes keep a
+    multi-level hierarchy (the coarsest level holds at most 768 nodes)."""
+    if kind == "hex":
+        m = mg.hex_box(6, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+    else:
+        m = mg.tet_p2_box(5, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
     aux, sizes = mg.elastic_aux(m.num_cells, seed=4, vs_sigma=0.2, vp_vs=1.732)
     dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
     ref = dense_solution(oracle, m, aux, sizes, dofs, vals)
     dm, solver = build(m, aux, sizes, dofs)
     solver.set_preconditioner("amg")
-    print(f"[solver-amg] method={method} dofs={3 * m.num_nodes}", flush=True)
+    print(f"[solver-amg] kind={kind} method={method} dofs={3 * m.num_nodes}", flush=True)
     x = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
     dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
     st = solver.solve(x, method=method, rtol=1e-12, maxit=5000)
@@ -101,6 +108,28 @@ def test_multigrid_cuts_iterations_on_config1():
     ref = sols["pbjacobi"]
     for pc in ("amg", "amg-rebuilt"):
         assert np.abs(sols[pc] - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_multigrid_cuts_iterations_on_p2():
+    """16^3 x 6 P2 tets (lognormal moduli): FP64 CG with the V-cycle of the assembled P2
+    operator against point-block Jacobi; both reach the same solution."""
+    m = mg.tet_p2_box(16, (0, 0, 0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=5, vs_sigma=0.2, vp_vs=1.732)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    dm, solver = build(m, aux, sizes, dofs)
+    sols, its = {}, {}
+    for pc in ("pbjacobi", "amg"):
+        if pc == "amg":
+            solver.set_preconditioner("amg")
+        x = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+        dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
+        st = solver.solve(x, method=L.KSP_CG, rtol=1e-10, maxit=20000)
+        print(f"[solver-amg] p2 16^3 pc={pc} {st}", flush=True)
+        assert st["converged"], st
+        sols[pc], its[pc] = x.cpu().numpy(), st["outer_iterations"]
+    assert its["amg"] * 4 <= its["pbjacobi"], its
+    ref = sols["pbjacobi"]
+    assert np.abs(sols["amg"] - ref).max() <= 1e-8 * np.abs(ref).max()
 
 
 def test_config1_fp32_refinement_matches_fp64(oracle):
