@@ -79,9 +79,29 @@ inline void gemm_b6(int b, double alpha, const double *A, const double *X, doubl
         }
 }
 
+// y (B x 6) += A (B x B) X (B x 6), fixed sizes (unrolled and vectorised)
+template <int B>
+inline void gemm_bx6(const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ y) {
+    for (int i = 0; i < B; ++i)
+        for (int k = 0; k < B; ++k) {
+            const double a = A[i * B + k];
+            for (int j = 0; j < 6; ++j) y[i * 6 + j] += a * X[k * 6 + j];
+        }
+}
+
+// o (6 x 6) += P^T X for P, X (B x 6)
+template <int B>
+inline void gemm_t6(const double *__restrict__ P, const double *__restrict__ X, double *__restrict__ o) {
+    for (int k = 0; k < B; ++k)
+        for (int r = 0; r < 6; ++r) {
+            const double p = P[k * 6 + r];
+            for (int q = 0; q < 6; ++q) o[r * 6 + q] += p * X[k * 6 + q];
+        }
+}
+
 struct Csr6 {  // block rows of b x 6 blocks, sorted columns
-    std::vector<int32_t> ptr, col;
-    std::vector<double> val;
+    AmgIdx ptr, col;
+    AmgVal val;
 };
 
 // Row-parallel construction of a block-CSR matrix: row(i, col, val, tid) appends row i's
@@ -98,32 +118,40 @@ void build_rows(int n, int bs, Csr6 *out, RowFn row) {
             P.ptr.push_back((int32_t)P.col.size());
         }
     });
-    out->ptr.assign(1, 0);
-    out->col.clear();
-    out->val.clear();
-    size_t nnz = 0;
-    for (auto &P : part) nnz += P.col.size();
-    out->ptr.reserve(n + 1);
-    out->col.reserve(nnz);
-    out->val.reserve(nnz * bs);
-    for (auto &P : part) {
-        if (P.ptr.empty()) continue;
-        const int32_t base = (int32_t)out->col.size();
-        for (size_t r = 1; r < P.ptr.size(); ++r) out->ptr.push_back(base + P.ptr[r]);
-        out->col.insert(out->col.end(), P.col.begin(), P.col.end());
-        out->val.insert(out->val.end(), P.val.begin(), P.val.end());
+    // each part's place in the result, then the parts copied in parallel
+    std::vector<size_t> coff(T + 1, 0);
+    std::vector<int> roff(T + 1, 0);
+    for (int t = 0; t < T; ++t) {
+        coff[t + 1] = coff[t] + part[t].col.size();
+        roff[t + 1] = roff[t] + (part[t].ptr.empty() ? 0 : (int)part[t].ptr.size() - 1);
     }
+    out->ptr.assign(n + 1, 0);
+    out->col.resize(coff[T]);
+    out->val.resize(coff[T] * bs);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            Csr6 &P = part[t];
+            if (P.ptr.empty()) return;
+            for (size_t r = 1; r < P.ptr.size(); ++r) out->ptr[roff[t] + r] = (int32_t)(coff[t] + P.ptr[r]);
+            std::copy(P.col.begin(), P.col.end(), out->col.begin() + coff[t]);
+            std::copy(P.val.begin(), P.val.end(), out->val.begin() + coff[t] * bs);
+            AmgIdx().swap(P.ptr);
+            AmgIdx().swap(P.col);
+            AmgVal().swap(P.val);
+        });
+    for (auto &x : th) x.join();
 }
 
 // Sparse product rows: out_i = sum_j A_ij * M_j for M with b x 6 blocks.
 void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const double *Aval, const Csr6 &M,
                 int ncols, Csr6 *out) {
     const int T = num_threads();
-    std::vector<std::vector<int>> marks(T, std::vector<int>(ncols, -1)), lists(T);
+    std::vector<std::vector<int>> marks(T, std::vector<int>(ncols, -1)), lists(T), orders(T);
     std::vector<std::vector<double>> accs(T);
     const int bb = b * 6;
-    build_rows(n, bb, out, [&](int i, std::vector<int32_t> &ocol, std::vector<double> &oval, int t) {
-        std::vector<int> &mark = marks[t], &list = lists[t];
+    build_rows(n, bb, out, [&](int i, AmgIdx &ocol, AmgVal &oval, int t) {
+        std::vector<int> &mark = marks[t], &list = lists[t], &order = orders[t];
         std::vector<double> &acc = accs[t];
         list.clear();
         acc.clear();
@@ -137,10 +165,11 @@ void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const do
                     list.push_back(c);
                     acc.resize(acc.size() + bb, 0.0);
                 }
-                gemm_b6(b, 1.0, A, M.val.data() + (size_t)f * bb, acc.data() + (size_t)mark[c] * bb);
+                if (b == 3) gemm_bx6<3>(A, M.val.data() + (size_t)f * bb, acc.data() + (size_t)mark[c] * bb);
+                else gemm_bx6<6>(A, M.val.data() + (size_t)f * bb, acc.data() + (size_t)mark[c] * bb);
             }
         }
-        std::vector<int> order(list.size());
+        order.resize(list.size());
         for (size_t u = 0; u < order.size(); ++u) order[u] = (int)u;
         std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
         for (int u : order) {
@@ -160,8 +189,12 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
     h->coarse_inv.clear();
     // current level operator
     int n = n0, b = 3;
-    std::vector<int32_t> Aptr(ptr0, ptr0 + n0 + 1), Acol(col0, col0 + ptr0[n0]);
-    std::vector<double> Aval(val0, val0 + (size_t)ptr0[n0] * 9);
+    // the current level's operator: the caller's arrays on level 0, the Galerkin products below
+    AmgIdx Aptr_v, Acol_v;
+    AmgVal Aval_v;
+    const int32_t *Ap = ptr0, *Ac = col0;
+    const double *Av = val0;
+    size_t nnz = (size_t)ptr0[n0];
     std::vector<uint8_t> fixed(fixed0, fixed0 + 3 * (size_t)n0);
     std::vector<double> B;  // near-null space [n*b][6], built per aggregate on level 0
     const int T = num_threads();
@@ -176,13 +209,13 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         parallel_chunks(n, T, [&](int lo, int hi, int t) {
             for (int i = lo; i < hi; ++i) {
                 int d = -1;
-                for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
-                    if (Acol[e] == i) d = e;
+                for (int e = Ap[i]; e < Ap[i + 1]; ++e)
+                    if (Ac[e] == i) d = e;
                 if (d < 0) {
                     no_diag[t] = 1;
                     return;
                 }
-                block_inverse(Aval.data() + (size_t)d * bb, b, L.Dinv.data() + (size_t)i * bb);
+                block_inverse(Av + (size_t)d * bb, b, L.Dinv.data() + (size_t)i * bb);
             }
         });
         if (std::find(no_diag.begin(), no_diag.end(), 1) != no_diag.end()) {
@@ -196,41 +229,42 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
             for (int c = 0; c < b; ++c) all = all && fixed[(size_t)i * b + c];
             excl[i] = all;
         }
-        // strong couplings: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F) (theta = 0: the whole
-        // node graph)
-        std::vector<uint8_t> strong(Acol.size(), 1);
-        if (theta > 0.0) {
+        // strong couplings: ||A_ij||_F >= th sqrt(||A_ii||_F ||A_jj||_F), th = theta on the finest
+        // level and theta / 2 below (th = 0: the whole node graph)
+        const double th = lev == 0 ? theta : 0.5 * theta;
+        std::vector<uint8_t> strong(nnz, 1);
+        if (th > 0.0) {
             std::vector<double> dn(n, 0.0);
             auto fro = [&](size_t e) {
                 double t = 0.0;
-                for (int k = 0; k < bb; ++k) t += Aval[e * bb + k] * Aval[e * bb + k];
+                for (int k = 0; k < bb; ++k) t += Av[e * bb + k] * Av[e * bb + k];
                 return std::sqrt(t);
             };
             for (int i = 0; i < n; ++i)
-                for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
-                    if (Acol[e] == i) dn[i] = fro((size_t)e);
+                for (int e = Ap[i]; e < Ap[i + 1]; ++e)
+                    if (Ac[e] == i) dn[i] = fro((size_t)e);
             for (int i = 0; i < n; ++i)
-                for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
-                    strong[e] = Acol[e] == i || fro((size_t)e) >= theta * std::sqrt(dn[i] * dn[Acol[e]]);
+                for (int e = Ap[i]; e < Ap[i + 1]; ++e)
+                    strong[e] = Ac[e] == i || fro((size_t)e) >= th * std::sqrt(dn[i] * dn[Ac[e]]);
         }
         std::vector<int> agg(n, -1);
         int na = 0;
         for (int i = 0; i < n; ++i) {
             if (excl[i] || agg[i] >= 0) continue;
             bool free_nbhd = true;
-            for (int e = Aptr[i]; e < Aptr[i + 1] && free_nbhd; ++e) {
-                const int j = Acol[e];
+            for (int e = Ap[i]; e < Ap[i + 1] && free_nbhd; ++e) {
+                const int j = Ac[e];
                 if (strong[e] && !excl[j] && agg[j] >= 0) free_nbhd = false;
             }
             if (!free_nbhd) continue;
-            for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
-                if (strong[e] && !excl[Acol[e]]) agg[Acol[e]] = na;
+            for (int e = Ap[i]; e < Ap[i + 1]; ++e)
+                if (strong[e] && !excl[Ac[e]]) agg[Ac[e]] = na;
             agg[i] = na++;
         }
         for (int i = 0; i < n; ++i) {
             if (excl[i] || agg[i] >= 0) continue;
-            for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
-                const int j = Acol[e];
+            for (int e = Ap[i]; e < Ap[i + 1]; ++e) {
+                const int j = Ac[e];
                 if (strong[e] && !excl[j] && agg[j] >= 0) {
                     agg[i] = agg[j];
                     break;
@@ -344,9 +378,9 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                     double acc = 0.0;
                     for (int i = lo; i < hi; ++i) {
                         double y[6] = {0, 0, 0, 0, 0, 0};
-                        for (int e = Aptr[i]; e < Aptr[i + 1]; ++e) {
-                            const double *A = &Aval[(size_t)e * bb];
-                            const double *x = &v[(size_t)Acol[e] * b];
+                        for (int e = Ap[i]; e < Ap[i + 1]; ++e) {
+                            const double *A = &Av[(size_t)e * bb];
+                            const double *x = &v[(size_t)Ac[e] * b];
                             for (int r = 0; r < b; ++r) {
                                 double sacc = 0.0;
                                 for (int c = 0; c < b; ++c) sacc += A[r * b + c] * x[c];
@@ -372,13 +406,40 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
             }
             L.lmax = 1.1 * lam;
         }
-        // smoothed prolongator P = P0 - omega D^-1 A P0
+        // smoothed prolongator P = P0 - omega D^-1 A_F P0.  With a threshold, A_F is the filtered
+        // operator: the strong blocks, the weak blocks of a row lumped onto its diagonal block
+        // (row sums kept, so translations stay in the range of P); it keeps P's stencil, and
+        // with it the coarse operator, from growing with the weak couplings
         Csr6 AP0, P;
-        spgemm_A_M(n, b, Aptr.data(), Acol.data(), Aval.data(), P0, na, &AP0);
+        if (th > 0.0) {
+            std::vector<int32_t> Fptr(n + 1, 0), Fcol;
+            std::vector<double> Fval;
+            Fcol.reserve(nnz);
+            Fval.reserve(nnz * bb);
+            for (int i = 0; i < n; ++i) {
+                double lump[36] = {};
+                size_t dpos = 0;
+                for (int e = Ap[i]; e < Ap[i + 1]; ++e) {
+                    const double *blk = &Av[(size_t)e * bb];
+                    if (Ac[e] == i) dpos = Fcol.size();
+                    if (strong[e]) {
+                        Fcol.push_back(Ac[e]);
+                        Fval.insert(Fval.end(), blk, blk + bb);
+                    } else {
+                        for (int k = 0; k < bb; ++k) lump[k] += blk[k];
+                    }
+                }
+                for (int k = 0; k < bb; ++k) Fval[dpos * bb + k] += lump[k];
+                Fptr[i + 1] = (int32_t)Fcol.size();
+            }
+            spgemm_A_M(n, b, Fptr.data(), Fcol.data(), Fval.data(), P0, na, &AP0);
+        } else {
+            spgemm_A_M(n, b, Ap, Ac, Av, P0, na, &AP0);
+        }
         const double omega = 4.0 / (3.0 * L.lmax);
         // P0's own aggregate is always among AP0's columns when row i is free (A_ii P0_i); a
         // fixed row keeps P = 0 there (its A row is the identity and P0 row is zero)
-        build_rows(n, b * 6, &P, [&](int i, std::vector<int32_t> &ocol, std::vector<double> &oval, int) {
+        build_rows(n, b * 6, &P, [&](int i, AmgIdx &ocol, AmgVal &oval, int) {
             for (int f = AP0.ptr[i]; f < AP0.ptr[i + 1]; ++f) {
                 const int c = AP0.col[f];
                 double blk[36] = {};
@@ -390,9 +451,9 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         });
         // Galerkin coarse operator Ac = P^T (A P)
         Csr6 AP;
-        spgemm_A_M(n, b, Aptr.data(), Acol.data(), Aval.data(), P, na, &AP);
+        spgemm_A_M(n, b, Ap, Ac, Av, P, na, &AP);
         // transpose of P: for coarse row a, the fine rows i with their b x 6 blocks
-        std::vector<int32_t> Rptr(na + 1, 0), Rrow(P.col.size()), Rsrc(P.col.size());
+        AmgIdx Rptr(na + 1, 0), Rrow(P.col.size()), Rsrc(P.col.size());
         for (int32_t c : P.col) Rptr[c + 1]++;
         for (int a = 0; a < na; ++a) Rptr[a + 1] += Rptr[a];
         {
@@ -405,10 +466,10 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         }
         Csr6 C;
         {
-            std::vector<std::vector<int>> marks(T, std::vector<int>(na, -1)), lists(T);
+            std::vector<std::vector<int>> marks(T, std::vector<int>(na, -1)), lists(T), orders(T);
             std::vector<std::vector<double>> accs(T);
-            build_rows(na, 36, &C, [&](int a, std::vector<int32_t> &ocol, std::vector<double> &oval, int t) {
-                std::vector<int> &mark = marks[t], &list = lists[t];
+            build_rows(na, 36, &C, [&](int a, AmgIdx &ocol, AmgVal &oval, int t) {
+                std::vector<int> &mark = marks[t], &list = lists[t], &order = orders[t];
                 std::vector<double> &acc = accs[t];
                 list.clear();
                 acc.clear();
@@ -424,15 +485,11 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                         }
                         const double *APic = &AP.val[(size_t)f * b * 6];  // b x 6
                         double *o = &acc[(size_t)mark[c] * 36];
-                        for (int r = 0; r < 6; ++r)
-                            for (int k = 0; k < b; ++k) {
-                                const double p = Pia[k * 6 + r];
-                                if (p == 0.0) continue;
-                                for (int q = 0; q < 6; ++q) o[r * 6 + q] += p * APic[k * 6 + q];
-                            }
+                        if (b == 3) gemm_t6<3>(Pia, APic, o);
+                        else gemm_t6<6>(Pia, APic, o);
                     }
                 }
-                std::vector<int> order(list.size());
+                order.resize(list.size());
                 for (size_t u = 0; u < order.size(); ++u) order[u] = (int)u;
                 std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
                 bool has_diag = false;
@@ -458,8 +515,8 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                 for (int c : list) mark[c] = -1;
             });
         }
-        std::vector<int32_t> &Cptr = C.ptr, &Ccol = C.col;
-        std::vector<double> &Cval = C.val;
+        AmgIdx &Cptr = C.ptr, &Ccol = C.col;
+        AmgVal &Cval = C.val;
         // store this level's transfer operators
         L.nc = na;
         L.Rval.resize((size_t)P.col.size() * 6 * b);
@@ -476,17 +533,21 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         L.Rptr = std::move(Rptr);
         L.Rcol = std::move(Rrow);
         if (lev > 0) {  // the next level's operator replaces these below
-            L.Aptr = std::move(Aptr);
-            L.Acol = std::move(Acol);
-            L.Aval = std::move(Aval);
+            L.Aptr = std::move(Aptr_v);
+            L.Acol = std::move(Acol_v);
+            L.Aval = std::move(Aval_v);
         }
         h->levels.push_back(std::move(L));
         // next level
         n = na;
         b = 6;
-        Aptr.swap(Cptr);
-        Acol.swap(Ccol);
-        Aval.swap(Cval);
+        Aptr_v.swap(Cptr);
+        Acol_v.swap(Ccol);
+        Aval_v.swap(Cval);
+        Ap = Aptr_v.data();
+        Ac = Acol_v.data();
+        Av = Aval_v.data();
+        nnz = Acol_v.size();
         B.swap(Bc);
         fixed.assign((size_t)n * 6, 0);
         if (6 * n <= coarse_max || (int)h->levels.size() >= max_levels) break;
@@ -499,9 +560,9 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
     }
     std::vector<double> D((size_t)m * m, 0.0), I((size_t)m * m, 0.0);
     for (int i = 0; i < n; ++i)
-        for (int e = Aptr[i]; e < Aptr[i + 1]; ++e)
+        for (int e = Ap[i]; e < Ap[i + 1]; ++e)
             for (int r = 0; r < 6; ++r)
-                for (int c = 0; c < 6; ++c) D[(size_t)(6 * i + r) * m + 6 * Acol[e] + c] = Aval[(size_t)e * 36 + r * 6 + c];
+                for (int c = 0; c < 6; ++c) D[(size_t)(6 * i + r) * m + 6 * Ac[e] + c] = Av[(size_t)e * 36 + r * 6 + c];
     for (int i = 0; i < m; ++i) I[(size_t)i * m + i] = 1.0;
     for (int c = 0; c < m; ++c) {
         int p = c;

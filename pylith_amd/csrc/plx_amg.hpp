@@ -10,22 +10,48 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace plx {
+
+// Allocator that default-initialises (leaves doubles and ints unset) on resize: the set-up's
+// large arrays are written in full right after they are sized, so a zero fill would only cost
+// a serial pass over gigabytes.
+template <typename T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <typename U>
+    DefaultInitAlloc(const DefaultInitAlloc<U> &) noexcept {}
+    template <typename U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U *p, Args &&...args) {
+        ::new (static_cast<void *>(p)) U(std::forward<Args>(args)...);
+    }
+};
+using AmgIdx = std::vector<int32_t, DefaultInitAlloc<int32_t>>;
+using AmgVal = std::vector<double, DefaultInitAlloc<double>>;
 
 struct AmgLevel {
     int n = 0;       // block rows
     int b = 0;       // block size (3 on the finest level, 6 below)
     int nc = 0;      // block rows of the next level
     // operator of this level (block CSR, b x b blocks row-major); level 0 uses the caller's copy
-    std::vector<int32_t> Aptr, Acol;
-    std::vector<double> Aval;
+    AmgIdx Aptr, Acol;
+    AmgVal Aval;
     // prolongator (n rows, b x 6 blocks) and its transpose (nc rows, 6 x b blocks)
-    std::vector<int32_t> Pptr, Pcol, Rptr, Rcol;
-    std::vector<double> Pval, Rval;
-    std::vector<double> Dinv;  // [n][b][b] inverse diagonal blocks
+    AmgIdx Pptr, Pcol, Rptr, Rcol;
+    AmgVal Pval, Rval;
+    AmgVal Dinv;  // [n][b][b] inverse diagonal blocks
     double lmax = 0.0;         // upper bound of the spectrum of D^-1 A
 };
 
@@ -37,8 +63,9 @@ struct AmgHierarchy {
 
 // Build the hierarchy of the 3x3-block operator (ptr, col, val) on n nodes.  fixed[3 n] marks
 // identity rows/columns (constrained dofs, positive split nodes); coords [n][3].  Aggregates
-// grow over the strong couplings of each level, ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F)
-// (theta = 0: every coupling of the node graph).
+// grow over the strong couplings of each level, ||A_ij||_F >= th sqrt(||A_ii||_F ||A_jj||_F)
+// with th = theta on the finest level and theta / 2 below, and the prolongator is smoothed with
+// the operator filtered to those couplings (theta = 0: every coupling of the node graph).
 int amg_setup(int n, const int32_t *ptr, const int32_t *col, const double *val, const double *coords,
               const uint8_t *fixed, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err,
               double theta = 0.0);

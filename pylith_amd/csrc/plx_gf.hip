@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "plx_amg.hpp"
@@ -530,6 +531,27 @@ __global__ void __launch_bounds__(256) amg_bsr1(int nrows, const int32_t *__rest
     }
 }
 
+// r = b - t (the fine residual over a matrix-free product t = A x)
+__global__ void __launch_bounds__(256) amg_sub(int64_t n, const float *__restrict__ b, const float *__restrict__ t,
+                                               float *r) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) r[i] = b[i] - t[i];
+}
+
+// z = x + s Dinv (b - t), 3 x 3 blocks (the fine post-smoothing over a matrix-free t = A x)
+__global__ void __launch_bounds__(256) amg_post0(int n, const float *__restrict__ Dinv, const float *__restrict__ b,
+                                                 const float *__restrict__ t, const float *__restrict__ x, float *z,
+                                                 float s) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float *Di = Dinv + 9 * (size_t)i;
+        float rv[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) rv[q] = b[3 * (size_t)i + q] - t[3 * (size_t)i + q];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            z[3 * (size_t)i + a] = x[3 * (size_t)i + a] + s * (Di[3 * a] * rv[0] + Di[3 * a + 1] * rv[1] + Di[3 * a + 2] * rv[2]);
+    }
+}
+
 // coarsest level, one column: a wave per row of the dense inverse
 __global__ void __launch_bounds__(256) amg_dense1(int m, const float *__restrict__ Ainv, const float *__restrict__ Bv,
                                                   float *X) {
@@ -718,6 +740,11 @@ struct plx_gf_s {
     std::vector<int32_t> perm;
     int32_t *d_perm = nullptr;
     std::vector<double> xyz;  // [nrows][3] node coordinates in row order (multigrid rigid-body modes)
+    // host copy of the row-ordered operator and its identity-row dofs, kept by the assembled
+    // (P2) path for the multigrid set-up (which frees it) instead of a download
+    std::vector<int32_t> h_ptr, h_col;
+    std::vector<double> h_val;
+    std::vector<uint8_t> h_fixed;
     double *d_val64 = nullptr;
     float *d_val32 = nullptr;
     double *d_Binv64 = nullptr;
@@ -752,6 +779,12 @@ struct plx_gf_s {
     int coarse_m = 0;
     float *coarse_inv = nullptr, *cb = nullptr, *cx = nullptr;
     float *vr1 = nullptr, *vz1 = nullptr;  // single-column V-cycle operands in row order
+    // the finest level's transfer operators and inverse diagonal blocks with DM-ordered fine
+    // indices, for the single-column V-cycle over a matrix-free fine operator (fine_op)
+    int32_t *Rcol_dm = nullptr, *Pptr_dm = nullptr, *Pcol_dm = nullptr;
+    float *Pval_dm = nullptr, *Dinv_dm = nullptr;
+    GfFineOp fine_op = nullptr;
+    void *fine_ctx = nullptr;
     int amg_kmax = 0;
     uint64_t op_epoch = 0;  // the DM's operator epoch the assembled copies were made at
     // continuous batching (plx_gf_slots_*): per-column solves that start and retire on their own
@@ -1063,9 +1096,12 @@ static void gf_amg_free(plx_gf g) {
                         (void *)L.bv, (void *)L.xv})
             if (p) (void)hipFree(p);
     g->lv.clear();
-    for (void *p : {(void *)g->coarse_inv, (void *)g->cb, (void *)g->cx, (void *)g->vr1, (void *)g->vz1})
+    for (void *p : {(void *)g->coarse_inv, (void *)g->cb, (void *)g->cx, (void *)g->vr1, (void *)g->vz1,
+                    (void *)g->Rcol_dm, (void *)g->Pptr_dm, (void *)g->Pcol_dm, (void *)g->Pval_dm,
+                    (void *)g->Dinv_dm})
         if (p) (void)hipFree(p);
-    g->coarse_inv = g->cb = g->cx = g->vr1 = g->vz1 = nullptr;
+    g->coarse_inv = g->cb = g->cx = g->vr1 = g->vz1 = g->Pval_dm = g->Dinv_dm = nullptr;
+    g->Rcol_dm = g->Pptr_dm = g->Pcol_dm = nullptr;
     g->coarse_m = 0;
 }
 
@@ -1074,26 +1110,39 @@ static void gf_amg_free(plx_gf g) {
 static int gf_amg_setup(plx_gf g) {
     plx_dm dm = g->dm;
     const int nn = g->nrows;
-    std::vector<int32_t> ptr(nn + 1), col(g->nnzb);
-    std::vector<double> val((size_t)g->nnzb * 9);
-    PLX_HIP(dm, hipMemcpy(ptr.data(), g->d_row_ptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    PLX_HIP(dm, hipMemcpy(col.data(), g->d_col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    PLX_HIP(dm, hipMemcpy(val.data(), g->d_val64, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int32_t> ptr, col;
+    std::vector<double> val;
+    std::vector<uint8_t> fixed;
+    if (!g->h_val.empty()) {  // the assembled path's host copy
+        ptr.swap(g->h_ptr);
+        col.swap(g->h_col);
+        val.swap(g->h_val);
+        fixed.swap(g->h_fixed);
+    } else {
+        ptr.resize(nn + 1);
+        col.resize(g->nnzb);
+        val.resize((size_t)g->nnzb * 9);
+        PLX_HIP(dm, hipMemcpy(ptr.data(), g->d_row_ptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PLX_HIP(dm, hipMemcpy(col.data(), g->d_col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PLX_HIP(dm, hipMemcpy(val.data(), g->d_val64, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
     // fixed dofs: identity rows of the merged operator (constrained dofs, positive split nodes)
-    std::vector<uint8_t> fixed(3 * (size_t)nn, 0);
-    for (int i = 0; i < nn; ++i)
-        for (int c = 0; c < 3; ++c) {
-            bool ident = true;
-            for (int e = ptr[i]; e < ptr[i + 1] && ident; ++e)
-                for (int d = 0; d < 3; ++d) {
-                    const double v = val[(size_t)e * 9 + c * 3 + d];
-                    if (v != ((col[e] == i && d == c) ? 1.0 : 0.0)) {
-                        ident = false;
-                        break;
+    if (fixed.empty()) {
+        fixed.assign(3 * (size_t)nn, 0);
+        for (int i = 0; i < nn; ++i)
+            for (int c = 0; c < 3; ++c) {
+                bool ident = true;
+                for (int e = ptr[i]; e < ptr[i + 1] && ident; ++e)
+                    for (int d = 0; d < 3; ++d) {
+                        const double v = val[(size_t)e * 9 + c * 3 + d];
+                        if (v != ((col[e] == i && d == c) ? 1.0 : 0.0)) {
+                            ident = false;
+                            break;
+                        }
                     }
-                }
-            fixed[3 * (size_t)i + c] = ident;
-        }
+                fixed[3 * (size_t)i + c] = ident;
+            }
+    }
     AmgHierarchy h;
     std::string msg;
     if (g->xyz.size() != 3 * (size_t)nn) return set_error(dm, 1, "node coordinates missing");
@@ -1101,13 +1150,13 @@ static int gf_amg_setup(plx_gf g) {
                   g->amg_theta))
         return set_error(dm, 1, msg.c_str());
     gf_amg_free(g);
-    auto upf = [&](float **d, const std::vector<double> &v) -> int {
+    auto upf = [&](float **d, const auto &v) -> int {
         std::vector<float> f(v.begin(), v.end());
         PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, f.size()) * sizeof(float)));
         PLX_HIP(dm, hipMemcpy(*d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
         return 0;
     };
-    auto upi = [&](int32_t **d, const std::vector<int32_t> &v) -> int {
+    auto upi = [&](int32_t **d, const auto &v) -> int {
         PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, v.size()) * sizeof(int32_t)));
         PLX_HIP(dm, hipMemcpy(*d, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         return 0;
@@ -1140,6 +1189,27 @@ static int gf_amg_setup(plx_gf g) {
         return err;
     PLX_HIP(dm, hipMalloc((void **)&g->vr1, (size_t)g->ndof * sizeof(float)));
     PLX_HIP(dm, hipMalloc((void **)&g->vz1, (size_t)g->ndof * sizeof(float)));
+    {  // the finest level in DM order: Dinv and P rows moved to their nodes, R columns renamed
+        const AmgLevel &H = h.levels[0];
+        std::vector<int32_t> iperm(nn);
+        for (int r = 0; r < nn; ++r) iperm[g->perm[r]] = r;
+        std::vector<double> dinv(H.Dinv.size());
+        for (int v = 0; v < nn; ++v)
+            std::copy(&H.Dinv[9 * (size_t)iperm[v]], &H.Dinv[9 * (size_t)iperm[v]] + 9, &dinv[9 * (size_t)v]);
+        std::vector<int32_t> rcol(H.Rcol.size());
+        for (size_t k = 0; k < rcol.size(); ++k) rcol[k] = g->perm[H.Rcol[k]];
+        std::vector<int32_t> pptr(nn + 1, 0), pcol(H.Pcol.size());
+        std::vector<double> pval(H.Pval.size());
+        for (int v = 0; v < nn; ++v) {
+            const int r = iperm[v], e0 = H.Pptr[r], e1 = H.Pptr[r + 1];
+            pptr[v + 1] = pptr[v] + (e1 - e0);
+            std::copy(&H.Pcol[e0], &H.Pcol[e1], &pcol[pptr[v]]);
+            std::copy(&H.Pval[18 * (size_t)e0], &H.Pval[18 * (size_t)e1], &pval[18 * (size_t)pptr[v]]);
+        }
+        if ((err = upf(&g->Dinv_dm, dinv)) || (err = upi(&g->Rcol_dm, rcol)) || (err = upi(&g->Pptr_dm, pptr)) ||
+            (err = upi(&g->Pcol_dm, pcol)) || (err = upf(&g->Pval_dm, pval)))
+            return err;
+    }
     return 0;
 }
 
@@ -1242,28 +1312,39 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
     // rows in Morton order, columns renumbered and ascending
     std::vector<int32_t> row_ptr(nn + 1, 0), col((size_t)nnzb), diag_pos(nn, -1);
     std::vector<double> val((size_t)nnzb * 9);
-    std::vector<std::pair<int32_t, int32_t>> rowv;
-    for (int r = 0; r < nn; ++r) {
-        const int i = perm[r];
-        rowv.clear();
-        for (int e = rp0[i]; e < rp0[i + 1]; ++e) rowv.push_back({iperm[col0[e]], e});
-        std::sort(rowv.begin(), rowv.end());
-        int64_t o = row_ptr[r];
-        for (const auto &ce : rowv) {
-            const int j = perm[ce.first];
-            col[o] = ce.first;
-            if (ce.first == r) diag_pos[r] = (int32_t)o;
-            for (int f = 0; f < 3; ++f)
-                for (int h = 0; h < 3; ++h) {
-                    double v = -val0[9 * (size_t)ce.second + 3 * f + h];  // the operator -J
-                    if (conmask[3 * (size_t)i + f] || conmask[3 * (size_t)j + h]) v = (i == j && f == h) ? 1.0 : 0.0;
-                    val[9 * (size_t)o + 3 * f + h] = v;
+    for (int r = 0; r < nn; ++r) row_ptr[r + 1] = row_ptr[r] + (rp0[perm[r] + 1] - rp0[perm[r]]);
+    {  // rows are independent: chunks of them on host threads
+        const int T = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                std::vector<std::pair<int32_t, int32_t>> rowv;
+                const int r0 = (int)((int64_t)nn * t / T), r1 = (int)((int64_t)nn * (t + 1) / T);
+                for (int r = r0; r < r1; ++r) {
+                    const int i = perm[r];
+                    rowv.clear();
+                    for (int e = rp0[i]; e < rp0[i + 1]; ++e) rowv.push_back({iperm[col0[e]], e});
+                    std::sort(rowv.begin(), rowv.end());
+                    int64_t o = row_ptr[r];
+                    for (const auto &ce : rowv) {
+                        const int j = perm[ce.first];
+                        col[o] = ce.first;
+                        if (ce.first == r) diag_pos[r] = (int32_t)o;
+                        for (int f = 0; f < 3; ++f)
+                            for (int h = 0; h < 3; ++h) {
+                                double v = -val0[9 * (size_t)ce.second + 3 * f + h];  // the operator -J
+                                if (conmask[3 * (size_t)i + f] || conmask[3 * (size_t)j + h])
+                                    v = (i == j && f == h) ? 1.0 : 0.0;
+                                val[9 * (size_t)o + 3 * f + h] = v;
+                            }
+                        ++o;
+                    }
                 }
-            ++o;
-        }
-        row_ptr[r + 1] = (int32_t)o;
-        if (diag_pos[r] < 0) return set_error(dm, 1, "assembled operator without a diagonal block");
+            });
+        for (auto &x : th) x.join();
     }
+    for (int r = 0; r < nn; ++r)
+        if (diag_pos[r] < 0) return set_error(dm, 1, "assembled operator without a diagonal block");
     std::vector<double>().swap(val0);
     plx_gf g = new plx_gf_s();
     g->dm = dm;
@@ -1282,14 +1363,59 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
         plx_gf_destroy(g);
         return err;
     }
+    g->h_fixed.resize(3 * (size_t)nn);
+    for (int r = 0; r < nn; ++r)
+        for (int c = 0; c < 3; ++c) g->h_fixed[3 * (size_t)r + c] = conmask[3 * (size_t)g->perm[r] + c];
+    g->h_ptr = std::move(row_ptr);
+    g->h_col = std::move(col);
+    g->h_val = std::move(val);
     *out = g;
     return 0;
 }
 
 // One V-cycle on a single column, z = M r (3 x num_nodes FP32 vectors in DM node order), queued
 // on `stream`; used by the matrix-free solver as its multigrid preconditioner.
+int gf_set_fine_operator(plx_gf g, GfFineOp op, void *ctx) {
+    if (!g) return 1;
+    g->fine_op = op;
+    g->fine_ctx = ctx;
+    return 0;
+}
+
+// The V-cycle with the finest level in DM order over the matrix-free operator:
+//   x = s D^-1 r;  t = A x;  x += P V(R (r - t));  t = A x;  z = x + s D^-1 (r - t)
+// (the same cycle as amg_level on level 0; constrained dofs stay zero: their rows of D^-1 are
+// identity rows, their rows of P zero, and r is zero there)
+static int gf_vcycle1_matfree(plx_gf g, const float *r, float *z) {
+    const plx_gf_s::Level &L = g->lv[0];
+    const int nl = (int)g->lv.size();
+    const float s = (float)(2.0 / (L.lmax + L.lmax / 30.0));
+    float *x = L.d, *r1 = L.r, *t = g->vr1;
+    float *nb = nl > 1 ? g->lv[1].bv : g->cb;
+    float *nx = nl > 1 ? g->lv[1].xv : g->cx;
+    hipLaunchKernelGGL((amg_jacobi0<3, 1>), gf_grid((int64_t)L.n), dim3(256), 0, g->stream, L.n, g->Dinv_dm, r, x, s);
+    int err = g->fine_op(g->fine_ctx, x, t);
+    if (err) return err;
+    hipLaunchKernelGGL(amg_sub, gf_grid(g->ndof), dim3(256), 0, g->stream, g->ndof, r, (const float *)t, r1);
+    amg_bsr_launch<6, 3, 1, kBsrPlain, 64>(g, L.nc, L.Rptr, g->Rcol_dm, L.Rval, r1, nullptr, nullptr, nb, 0.f, 0);
+    amg_vcycle<1>(g, 1, nb, nx);
+    amg_bsr_launch<3, 6, 1, kBsrPlain, 8>(g, L.n, g->Pptr_dm, g->Pcol_dm, g->Pval_dm, nx, nullptr, nullptr, x, 0.f, 1);
+    if ((err = g->fine_op(g->fine_ctx, x, t))) return err;
+    hipLaunchKernelGGL(amg_post0, gf_grid(L.n), dim3(256), 0, g->stream, L.n, (const float *)g->Dinv_dm, r,
+                       (const float *)t, (const float *)x, z, s);
+    PLX_HIP(g->dm, hipGetLastError());
+    return 0;
+}
+
 int gf_vcycle1(plx_gf g, hipStream_t stream, const float *r, float *z) {
     if (g->lv.empty()) return set_error(g->dm, 1, "multigrid hierarchy not built");
+    if (g->fine_op) {
+        hipStream_t saved = g->stream;
+        g->stream = stream;
+        const int err = gf_vcycle1_matfree(g, r, z);
+        g->stream = saved;
+        return err;
+    }
     hipStream_t saved = g->stream;
     g->stream = stream;
     hipLaunchKernelGGL(gf_permute_block<float>, gf_grid(g->ndof), dim3(256), 0, stream, g->ndof, 1, g->d_perm, true,

@@ -247,8 +247,13 @@ int launch_fault_jacobi(plx_dm dm, const double *r, double *z);
 int launch_fault_slip_rhs(plx_dm dm, double *lag_out);
 void free_fault(FaultData *f);
 
-// Multigrid V-cycle of a Green's-function handle on one column (plx_gf.hip).
+// Multigrid V-cycle of a Green's-function handle on one column (plx_gf.hip), r and z in DM
+// node order.  gf_set_fine_operator gives it a matrix-free fine-level operator y = A x (FP32,
+// DM order, queued on the V-cycle's stream, constrained rows zero): the fine level then smooths
+// with it instead of the assembled operator (null: back to the assembled one).
 int gf_vcycle1(plx_gf g, hipStream_t stream, const float *r, float *z);
+using GfFineOp = int (*)(void *ctx, const float *x, float *y);
+int gf_set_fine_operator(plx_gf g, GfFineOp op, void *ctx);
 
 // Vector kernels (plx_vector.hip).
 int launch_l2g(plx_dm dm, int scalar_type, const void *loc, void *glob, int mode);

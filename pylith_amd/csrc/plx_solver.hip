@@ -411,6 +411,12 @@ static int dot64(plx_solver s, const double *a, const double *b, double *out) {
     return read_scalar(s, S_RZ, out);
 }
 
+// The V-cycle's fine-level operator: the masked matrix-free FP32 apply (no fault: x is not
+// touched; with a fault the V-cycle keeps the assembled merged operator).
+static int amg_fine_apply(void *ctx, const float *x, float *y) {
+    return masked_apply<float>((plx_solver)ctx, const_cast<float *>(x), y);
+}
+
 // z = M r with the multigrid V-cycle (FP32; an FP64 pair goes through the FP32 work vectors,
 // which no FP64 iteration uses).
 template <typename Real>
@@ -803,6 +809,7 @@ int plx_solver_set_preconditioner(plx_solver s, int kind) {
     plx_gf g = nullptr;
     int err = plx_gf_create(&g, dm, s->label_value, s->part);
     if (!err) err = plx_gf_set_preconditioner(g, PLX_GF_PC_AMG);
+    if (!err && !s->fault) err = gf_set_fine_operator(g, amg_fine_apply, s);
     if (err) {
         if (g) plx_gf_destroy(g);
         return err;

@@ -73,9 +73,9 @@ void vcycle(int l, const double *bv, double *x) {
     for (size_t i = 0; i < nd; ++i) x[i] = xl[i] + z[i];
 }
 
-template <typename T>
-int copy_out(const std::vector<T> &v, void *dst) {
-    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+template <typename V>
+int copy_out(const V &v, void *dst) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
     return (int)v.size();
 }
 }  // namespace
