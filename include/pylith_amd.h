@@ -335,8 +335,9 @@ int plx_solver_set_inner(plx_solver solver, double rtol, int maxit, int restart)
 /* Preconditioner of every PCG the solver runs (the FP64 CG and the FP32 inner PCG): the inverse
  * 3x3 point blocks (default; PyLith's vpbjacobi fine smoother) or one FP32 V-cycle of
  * smoothed-aggregation multigrid on the assembled operator (the GAMG role of
- * materials/Elasticity.cc:307-318; Q1 keys covering every cell).  plx_solver_setup rebuilds
- * the hierarchy from the current operator. */
+ * materials/Elasticity.cc:307-318; keys covering every cell, Q1 or P2).  Without a fault the
+ * finest level of the cycle smooths with the solver's matrix-free FP32 operator.
+ * plx_solver_setup rebuilds the hierarchy from the current operator. */
 enum { PLX_PC_PBJACOBI = 0, PLX_PC_AMG = 1 };
 int plx_solver_set_preconditioner(plx_solver solver, int kind);
 int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, double rtol,
@@ -345,10 +346,11 @@ int plx_solver_solve(plx_solver solver, int method, const double *b, double *x, 
 /* ---------------------------------------------------------------------------------------------
  * Batched Green's functions: GreensFns (problems/GreensFns.cc:236-331) with the impulses of
  * FaultCohesiveImpulses (faults/FaultCohesiveImpulses.cc:263-300) -- one linear solve per
- * fault-slip impulse, all sharing the operator.  The operator of the key (Q1, the key covering
+ * fault-slip impulse, all sharing the operator.  The operator of the key (the key covering
  * every cell) is assembled once at create as 3x3-block CSR over the nodes, with the DM's
  * constrained dofs and, when the DM has a fault, the split nodes merged (the constraint-
- * eliminated operator); plx_gf_solve needs a planar fault.  Impulses are solved k at a time
+ * eliminated operator); Q1 from its element matrices, P2 (no fault) from the assembled
+ * Jacobian (plx_compute_jacobian); plx_gf_solve needs a planar fault (Q1).  Impulses are solved k at a time
  * (k in {8, 16, 32, 64}, autotuned when 0) by FP64 iterative refinement around an FP32 block
  * PCG, each block solved to ||b - A x|| <= rtol ||b|| per impulse in FP64.
  *   slip [num_impulses][num_lagrange][3] (device, fault frame), u [num_impulses][local size]
