@@ -191,13 +191,17 @@ def test_captured_iterations_follow_changes_of_the_dm(pc):
         assert np.abs(eager - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("kind", ["hex", "tet"])
 @pytest.mark.parametrize("pc", ["pbjacobi", "amg"])
 @pytest.mark.parametrize("method", [L.KSP_CG, L.KSP_FGMRES_IR])
-def test_two_materials_solve_matches_dense_direct(oracle, method, pc):
+def test_two_materials_solve_matches_dense_direct(oracle, method, pc, kind):
     """A solver over every key (two materials, labels 1 and 2): the operator is the sum of the
     keys' operators; against a dense direct solve with each cell's own moduli."""
     from pylith_amd.dm import DM, Solver
-    m = mg.hex_box(6, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+    if kind == "hex":
+        m = mg.hex_box(6, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+    else:
+        m = mg.tet_p2_box(4, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
     m.labels = (m.coords[m.cone[:, 0], 2] >= 3000.0).astype(np.int32) + 1
     aux1, sizes = mg.elastic_aux(m.num_cells, seed=4, vs_sigma=0.2, vp_vs=1.732)
     aux2, _ = mg.elastic_aux(m.num_cells, vs=1500.0, vp=2800.0)
@@ -220,7 +224,7 @@ def test_two_materials_solve_matches_dense_direct(oracle, method, pc):
     dm.set_constraints(dofs)
     solver = Solver(dm, label_value=L.ALL_KEYS)
     solver.set_preconditioner(pc)
-    print(f"[solver-2mat] method={method} pc={pc}", flush=True)
+    print(f"[solver-2mat] kind={kind} method={method} pc={pc}", flush=True)
     x = torch.zeros(nd, dtype=torch.float64, device="cuda")
     dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
     st = solver.solve(x, method=method, rtol=1e-12, maxit=5000)
