@@ -618,8 +618,9 @@ def main():
                     help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
-              "multi-GPU path (sharding is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs on a single MI355X and has no "
+              "multi-GPU path (a mesh partition per GPU with a halo exchange of shared nodes "
+              "would be the way to shard it).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -43,7 +43,8 @@ enum plx_insert_mode { PLX_ADD_VALUES = 0, PLX_INSERT_VALUES = 1 };
 enum plx_equation_part { PLX_LHS = 0, PLX_RHS = 1, PLX_LHS_LUMPED_INV = 2, PLX_LHS_WEIGHTED = 3 };
 
 /* Device twins of PyLith pointwise kernels.  A host integration maps the PetscPointFn /
- * PetscPointJacFn address it registers to one of these ids. */
+ * PetscPointJacFn address it registers to one of these ids; kernels outside this list are
+ * compiled for the device against pylith_amd_pointfn.hpp and registered by address. */
 enum plx_pointfn {
     PLX_FN_NONE = 0,
     PLX_F1V_ISOLINEAR_ELASTICITY_3D = 1,      /* IsotropicLinearElasticity3D::f1v_infinitesimalStrain  (IsotropicLinearElasticity.hh:803) */
