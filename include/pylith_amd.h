@@ -111,8 +111,9 @@ int plx_dm_get_sizes(plx_dm dm, int64_t *local_size, int64_t *global_size);
  *     back to quadrature points.  Identical results up to rounding;
  *   PLX_ALGO_POINTFN: the generic quadrature-point path of include/pylith_amd_pointfn.hpp,
  *     which calls the device kernels in the PetscPointFn / PetscPointJacFn parameter list (the
- *     library's own twins of the reference kernels, written in that form; one thread per cell,
- *     colour-ordered scatter, bitwise reproducible).  Elastic, reference-state and body-force /
+ *     library's own twins of the reference kernels, written in that form; LDS-staged tiles with
+ *     the accumulation below, or one thread per cell colour by colour under
+ *     PLX_ACCUMULATE_COLORED, bitwise reproducible).  Elastic, reference-state and body-force /
  *     gravity keys; generalized Maxwell keys keep the tile kernels;
  *   PLX_ALGO_AUTO (default): FACTORED where it applies.
  *   PLX_ACCUMULATE_COLORED: element vectors enter the tile's LDS node buffer colour by colour, a
@@ -249,6 +250,20 @@ typedef struct {
     int mode;                        /* ADD_VALUES / INSERT_VALUES */
     const int32_t *row_ptr, *col;    /* DEVICE Jacobian pattern (assembly launchers) */
     double *values;                  /* DEVICE Jacobian values [nnzb][3][3] */
+    /* The key's tile plan (residual and action launchers): each tile's cells, its nodes (owned
+     * ones first, then the shared ones whose partial sums leave through the halo slots) and its
+     * vertices; DEVICE arrays.  deterministic = 1 (PLX_ACCUMULATE_COLORED) asks for the colour
+     * passes, whose summation order is fixed. */
+    int deterministic;
+    int num_tiles, tile_max_nodes, tile_max_verts;
+    const int32_t *tile_cell_ptr, *tile_node_ptr, *tile_vert_ptr;  /* [num_tiles + 1] */
+    const int32_t *tile_nown, *tile_halo_off;                      /* [num_tiles] */
+    const int32_t *tile_nodes, *tile_verts;  /* mesh node / vertex of each tile slot */
+    const int32_t *tile_cells;               /* [num_cells] the key's cells in tile order */
+    const uint16_t *lclosure, *lcone;        /* [nodes_per_cell | verts_per_cell][num_cells] tile slots */
+    int num_shared;
+    const int32_t *shared_node, *shared_ptr, *shared_slot;  /* shared nodes and their halo slots */
+    double *halo;                            /* [halo slots][3] partial sums */
 } plx_pointfn_args;
 typedef int (*plx_pointfn_launcher)(const plx_pointfn_args *args, void *hip_stream);
 /* residual: f (+)= sum w|J| (f0 . phi + f1 : grad phi).  action: f (+)= J(u) v.  assemble:

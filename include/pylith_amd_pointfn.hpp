@@ -38,9 +38,11 @@
 //
 // Integration: Q1 hexahedra with 2x2x2 Gauss points (PyLith's vertex order: 0-3 one face
 // cyclic, 4-7 the opposite face), P2 tetrahedra with the 14-point degree-5 rule (vertices, then
-// edge nodes (0,1), (0,2), (0,3), (1,2), (1,3), (2,3)).  One thread per cell; the cells of one
-// colour share no node, so the scatter-add needs no atomics and the summation order is fixed
-// (bitwise reproducible).
+// edge nodes (0,1), (0,2), (0,3), (1,2), (1,3), (2,3)).  Residual and action run over the key's
+// tiles (nodes staged in LDS, element vectors added in LDS, shared nodes through halo slots);
+// with args->deterministic they run one thread per cell colour by colour instead: the cells of
+// one colour share no node, so the scatter-add needs no atomics and the summation order is
+// fixed (bitwise reproducible).  The assembled Jacobian always takes the colour passes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -551,12 +553,12 @@ static __constant__ double kZeroAux[3 * kMaxAuxValues];
 
 // f += the element vectors of the cells [c0, c1) of one colour: TERMS gives (f0, f1) at a point
 // from the solution u (and, for the action, the direction v).
+// Element vector re[b][c] of one P2 cell from its vertex coordinates and nodal values (Ut read
+// only when A.u_t is set, V only for the action).
 template <class TERMS, bool ACTION>
-__device__ inline void integrate_p2(const plx_pointfn_args &A, int cell) {
-    double X[4][3], U[10][3], Ut[10][3], V[10][3];
-    gather_cell<PLX_TET_P2>(A, cell, X, A.u, U);
-    if (A.u_t) gather_cell<PLX_TET_P2>(A, cell, X, A.u_t, Ut);
-    if (ACTION) gather_cell<PLX_TET_P2>(A, cell, X, A.v, V);
+__device__ inline void integrate_p2_cell(const plx_pointfn_args &A, int cell, const double X[4][3],
+                                         const double U[10][3], const double Ut[10][3], const double V[10][3],
+                                         double re[10][3]) {
     int aOff[16], aOff_x[16];
     for (int k = 0; k < A.num_aux && k < 16; ++k) {
         aOff[k] = A.aux_off[k];
@@ -564,7 +566,7 @@ __device__ inline void integrate_p2(const plx_pointfn_args &A, int cell) {
     }
     CellP2 C;
     C.setup(X);
-    double M[4][9], re[10][3];
+    double M[4][9];
     for (int v = 0; v < 4; ++v)
         for (int i = 0; i < 9; ++i) M[v][i] = 0.0;
     for (int b = 0; b < 10; ++b) re[b][0] = re[b][1] = re[b][2] = 0.0;
@@ -608,29 +610,32 @@ __device__ inline void integrate_p2(const plx_pointfn_args &A, int cell) {
                 for (int c = 0; c < 3; ++c) re[b][c] += wdet * f0[c] * phi[b];
     }
     p2_finish(C.Ji, M, re);
+}
+
+template <class TERMS, bool ACTION>
+__device__ inline void integrate_p2(const plx_pointfn_args &A, int cell) {
+    double X[4][3], U[10][3], Ut[10][3], V[10][3], re[10][3];
+    gather_cell<PLX_TET_P2>(A, cell, X, A.u, U);
+    if (A.u_t) gather_cell<PLX_TET_P2>(A, cell, X, A.u_t, Ut);
+    if (ACTION) gather_cell<PLX_TET_P2>(A, cell, X, A.v, V);
+    integrate_p2_cell<TERMS, ACTION>(A, cell, X, U, Ut, V, re);
     for (int b = 0; b < 10; ++b) {
         const size_t g = (size_t)A.closure[(size_t)cell * 10 + b];
         for (int c = 0; c < 3; ++c) A.f[3 * g + c] += re[b][c];
     }
 }
 
+// Element vector re[v][c] of one Q1 cell from its corner coordinates and nodal values (Ut read
+// only when A.u_t is set, V only for the action).
 template <class TERMS, bool ACTION>
-__device__ inline void integrate_q1(const plx_pointfn_args &A, int cell) {
-    double X[8][3], TX[3][8], T[3][8], Tt[3][8], TV[3][8];
-    {
-        double W[8][3];
-        gather_cell<PLX_HEX_Q1>(A, cell, X, A.u, W);
-        q1_modal3(W, T);
-        if (A.u_t) {
-            gather_cell<PLX_HEX_Q1>(A, cell, X, A.u_t, W);
-            q1_modal3(W, Tt);
-        }
-        if (ACTION) {
-            gather_cell<PLX_HEX_Q1>(A, cell, X, A.v, W);
-            q1_modal3(W, TV);
-        }
-        q1_modal3(X, TX);
-    }
+__device__ inline void integrate_q1_cell(const plx_pointfn_args &A, int cell, const double X[8][3],
+                                         const double U[8][3], const double Ut[8][3], const double V[8][3],
+                                         double re[8][3]) {
+    double TX[3][8], T[3][8], Tt[3][8], TV[3][8];
+    q1_modal3(U, T);
+    if (A.u_t) q1_modal3(Ut, Tt);
+    if (ACTION) q1_modal3(V, TV);
+    q1_modal3(X, TX);
     int aOff[16], aOff_x[16];
     for (int k = 0; k < A.num_aux && k < 16; ++k) {
         aOff[k] = A.aux_off[k];
@@ -677,11 +682,108 @@ __device__ inline void integrate_q1(const plx_pointfn_args &A, int cell) {
     for (int c = 0; c < 3; ++c) {
         double r[8];
         q1_unmodal(R[c], r);  // the modes of the test functions carry the factor 1/8
-        for (int v = 0; v < 8; ++v) {
-            const size_t g = (size_t)A.closure[(size_t)cell * 8 + v];
-            A.f[3 * g + c] += 0.125 * r[v];
+        for (int v = 0; v < 8; ++v) re[v][c] = 0.125 * r[v];
+    }
+}
+
+template <class TERMS, bool ACTION>
+__device__ inline void integrate_q1(const plx_pointfn_args &A, int cell) {
+    double X[8][3], U[8][3], Ut[8][3], V[8][3], re[8][3];
+    gather_cell<PLX_HEX_Q1>(A, cell, X, A.u, U);
+    if (A.u_t) gather_cell<PLX_HEX_Q1>(A, cell, X, A.u_t, Ut);
+    if (ACTION) gather_cell<PLX_HEX_Q1>(A, cell, X, A.v, V);
+    integrate_q1_cell<TERMS, ACTION>(A, cell, X, U, Ut, V, re);
+    for (int v = 0; v < 8; ++v) {
+        const size_t g = (size_t)A.closure[(size_t)cell * 8 + v];
+        for (int c = 0; c < 3; ++c) A.f[3 * g + c] += re[v][c];
+    }
+}
+
+// One tile per workgroup: the tile's vertex coordinates and nodal values are staged in LDS,
+// each thread integrates cells of the tile through the pointwise kernels and adds their element
+// vectors into an LDS node accumulator (FP64 LDS atomics), then the tile writes its owned nodes
+// and leaves the shared nodes' partial sums in their halo slots (reduce_shared adds them).
+// Consecutive tiles run on one XCD (workgroups are dispatched round-robin over the 8 XCDs), so
+// nodes shared by neighbouring tiles meet in that XCD's L2.
+constexpr int kTileThreads = 256;
+
+template <class TERMS, bool ACTION, int CT>
+__global__ void __launch_bounds__(kTileThreads) tile_cells(plx_pointfn_args A) {
+    constexpr int NB = Basis<CT>::NB, NV = Basis<CT>::NV;
+    extern __shared__ double pf_lds[];
+    const int tid = threadIdx.x;
+    const int n = A.num_tiles, blk = blockIdx.x;
+    const int per = n >> 3, rem = n & 7, xcd = blk & 7;
+    const int tile = xcd * per + (xcd < rem ? xcd : rem) + (blk >> 3);
+    const int c0 = A.tile_cell_ptr[tile], nc = A.tile_cell_ptr[tile + 1] - c0;
+    const int n0 = A.tile_node_ptr[tile], nn = A.tile_node_ptr[tile + 1] - n0;
+    const int v0 = A.tile_vert_ptr[tile], nvt = A.tile_vert_ptr[tile + 1] - v0;
+    const int nown = A.tile_nown[tile];
+    const bool has_t = A.u_t != nullptr;
+    double *sX = pf_lds;
+    double *sU = sX + 3 * A.tile_max_verts;
+    double *sUt = sU + 3 * A.tile_max_nodes;
+    double *sV = sUt + (has_t ? 3 * A.tile_max_nodes : 0);
+    double *sR = sV + (ACTION ? 3 * A.tile_max_nodes : 0);
+    for (int k = tid; k < nvt; k += kTileThreads) {
+        const size_t g = (size_t)A.tile_verts[v0 + k];
+        for (int d = 0; d < 3; ++d) sX[3 * k + d] = A.coords[3 * g + d];
+    }
+    for (int k = tid; k < nn; k += kTileThreads) {
+        const size_t g = (size_t)A.tile_nodes[n0 + k];
+        for (int c = 0; c < 3; ++c) {
+            sU[3 * k + c] = A.u ? A.u[3 * g + c] : 0.0;  // an action may come without a state (zero)
+            if (has_t) sUt[3 * k + c] = A.u_t[3 * g + c];
+            if (ACTION) sV[3 * k + c] = A.v[3 * g + c];
+            sR[3 * k + c] = 0.0;
         }
     }
+    __syncthreads();
+    for (int i = tid; i < nc; i += kTileThreads) {
+        const size_t p = (size_t)(c0 + i);
+        const int cell = A.tile_cells[p];
+        int lc[NB];
+        double X[NV][3], U[NB][3], Ut[NB][3], V[NB][3], re[NB][3];
+        for (int v = 0; v < NV; ++v) {
+            const int lv = A.lcone[(size_t)v * A.num_cells + p];
+            for (int d = 0; d < 3; ++d) X[v][d] = sX[3 * lv + d];
+        }
+        for (int b = 0; b < NB; ++b) {
+            lc[b] = A.lclosure[(size_t)b * A.num_cells + p];
+            for (int c = 0; c < 3; ++c) {
+                U[b][c] = sU[3 * lc[b] + c];
+                Ut[b][c] = has_t ? sUt[3 * lc[b] + c] : 0.0;
+                V[b][c] = ACTION ? sV[3 * lc[b] + c] : 0.0;
+            }
+        }
+        if constexpr (CT == PLX_TET_P2) integrate_p2_cell<TERMS, ACTION>(A, cell, X, U, Ut, V, re);
+        else integrate_q1_cell<TERMS, ACTION>(A, cell, X, U, Ut, V, re);
+        for (int b = 0; b < NB; ++b)
+            for (int c = 0; c < 3; ++c) atomicAdd(&sR[3 * lc[b] + c], re[b][c]);
+    }
+    __syncthreads();
+    const bool insert = A.mode == PLX_INSERT_VALUES;
+    for (int k = tid; k < nown; k += kTileThreads) {
+        const size_t g = (size_t)A.tile_nodes[n0 + k];
+        for (int c = 0; c < 3; ++c) A.f[3 * g + c] = insert ? sR[3 * k + c] : A.f[3 * g + c] + sR[3 * k + c];
+    }
+    const size_t hoff = (size_t)A.tile_halo_off[tile];
+    for (int k = nown + tid; k < nn; k += kTileThreads) {
+        const size_t sl = hoff + (size_t)(k - nown);
+        for (int c = 0; c < 3; ++c) A.halo[3 * sl + c] = sR[3 * k + c];
+    }
+}
+
+// Shared nodes: the tiles' partial sums added in slot (tile) order.
+static __global__ void __launch_bounds__(256) reduce_shared(plx_pointfn_args A) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= A.num_shared) return;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int j = A.shared_ptr[s]; j < A.shared_ptr[s + 1]; ++j)
+        for (int c = 0; c < 3; ++c) a[c] += A.halo[3 * (size_t)A.shared_slot[j] + c];
+    const size_t g = 3 * (size_t)A.shared_node[s];
+    const bool insert = A.mode == PLX_INSERT_VALUES;
+    for (int c = 0; c < 3; ++c) A.f[g + c] = insert ? a[c] : A.f[g + c] + a[c];
 }
 
 // f += sum_q w|J| (f0 . phi + f1 : grad phi) over the cells [c0, c1) of one colour.
@@ -777,8 +879,29 @@ inline bool pointfn_args_ok(const plx_pointfn_args *A) {
            (A->cell_type == PLX_HEX_Q1 || A->cell_type == PLX_TET_P2);
 }
 
+// The tile path applies when the key's plan is there and the caller did not ask for fixed
+// summation order.
+inline bool use_tiles(const plx_pointfn_args *A) {
+    return !A->deterministic && A->num_tiles > 0 && A->tile_cell_ptr && A->tile_nodes && A->lclosure &&
+           A->lcone && A->tile_cells && (A->num_shared == 0 || (A->halo && A->shared_ptr));
+}
+
+template <class TERMS, bool ACTION, int CT>
+inline int launch_tiles(const plx_pointfn_args *A, hipStream_t s) {
+    const int fields = 2 + (A->u_t ? 1 : 0) + (ACTION ? 1 : 0);  // u, accumulator, u_t, v
+    const size_t lds = sizeof(double) * 3 * ((size_t)A->tile_max_verts + (size_t)fields * A->tile_max_nodes);
+    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void *>(&tile_cells<TERMS, ACTION, CT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 76;
+    hipLaunchKernelGGL((tile_cells<TERMS, ACTION, CT>), dim3(A->num_tiles), dim3(kTileThreads), lds, s, *A);
+    if (A->num_shared > 0)
+        hipLaunchKernelGGL(reduce_shared, dim3((A->num_shared + 255) / 256), dim3(256), 0, s, *A);
+    return hipGetLastError() == hipSuccess ? 0 : 76;
+}
+
 template <class F0, class F1, int CT>
 inline int launch_residual_ct(const plx_pointfn_args *A, hipStream_t s) {
+    if (use_tiles(A)) return launch_tiles<ResidualTerms<F0, F1>, false, CT>(A, s);
     if (A->mode == PLX_INSERT_VALUES && A->num_cells > 0)
         hipLaunchKernelGGL(zero_closure<CT>, dim3((A->num_cells + 255) / 256), dim3(256), 0, s, *A);
     for (int k = 0; k < A->num_colors; ++k) {
@@ -799,6 +922,7 @@ int launch_residual(const plx_pointfn_args *A, void *stream) {
 
 template <class J0, class J1, class J2, class J3, int CT>
 inline int launch_action_ct(const plx_pointfn_args *A, hipStream_t s) {
+    if (use_tiles(A)) return launch_tiles<ActionTerms<J0, J1, J2, J3>, true, CT>(A, s);
     if (A->mode == PLX_INSERT_VALUES && A->num_cells > 0)
         hipLaunchKernelGGL(zero_closure<CT>, dim3((A->num_cells + 255) / 256), dim3(256), 0, s, *A);
     for (int k = 0; k < A->num_colors; ++k) {

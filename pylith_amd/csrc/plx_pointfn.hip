@@ -310,6 +310,25 @@ static int make_args(plx_dm dm, KeyPlan *kp, plx_pointfn_args *A) {
     A->aux = kp->d_aux_raw;
     A->num_constants = (int)kp->constants.size();
     A->constants = kp->d_constants;
+    A->deterministic = dm->accumulate == PLX_ACCUMULATE_COLORED;
+    A->num_tiles = kp->num_tiles;
+    A->tile_max_nodes = kp->max_nodes;
+    A->tile_max_verts = kp->max_verts;
+    A->tile_cell_ptr = kp->d_tile_cell_ptr;
+    A->tile_node_ptr = kp->d_tile_node_ptr;
+    A->tile_vert_ptr = kp->d_tile_vert_ptr;
+    A->tile_nown = kp->d_tile_nown;
+    A->tile_halo_off = kp->d_tile_halo_off;
+    A->tile_nodes = kp->d_tile_nodes;
+    A->tile_verts = kp->d_tile_verts;
+    A->tile_cells = kp->d_cells;
+    A->lclosure = kp->d_lclosure;
+    A->lcone = kp->d_lcone;
+    A->num_shared = kp->num_shared;
+    A->shared_node = kp->d_shared_node;
+    A->shared_ptr = kp->d_shared_ptr;
+    A->shared_slot = kp->d_shared_slot;
+    A->halo = static_cast<double *>(kp->d_halo);
     return 0;
 }
 

@@ -4,7 +4,8 @@ Jacobian (DMPlexComputeJacobianByKey, IntegratorDomain.cc:489-490), against the 
 
 * The library's twins of the reference kernels written in that form (PLX_ALGO_POINTFN): the
   elastic, reference-state, body-force and gravity residuals and the elastic Jacobian action on
-  non-affine Q1 and on P2 meshes; bitwise reproducible.
+  non-affine Q1 and on P2 meshes, through the tile path and through the colour passes (bitwise
+  reproducible).
 * Test-only kernels in a library of their own (tests/cpp/pointfn_test_kernels.hip), compiled
   against the public header only and registered through plx_dm_add_*_pointfn: residual, action and
   assembled Jacobian, reading the point x, time, constants, solution values and aux offsets.
@@ -86,17 +87,23 @@ def test_builtin_kernels_in_pointfn_form(oracle, kind):
     dm.add_residual(0, 0, 0, F1)
     dm.add_jacobian(0, 0, 0, 0, 0, JF3)
     dm.set_auxiliary(0, 0, aux, sizes)
-    dm.set_kernel_options(L.ALGO_POINTFN, 0, 0)
-    r = torch.zeros(3 * nn, dtype=torch.float64, device="cuda")
-    dm.compute_residual(0, 0, dev(u), r, mode=L.INSERT_VALUES)
-    r2 = torch.full_like(r, 3.0)
-    dm.compute_residual(0, 0, dev(u), r2, mode=L.INSERT_VALUES)
-    jv = torch.full_like(r, 1.0)
-    dm.compute_jacobian_action(0, 0, dev(u), jv, mode=L.ADD_VALUES)
-    torch.cuda.synchronize()
-    assert relerr(r.cpu().numpy(), P.residual(u, F1)) <= TOL
-    assert torch.equal(r, r2)  # colour-ordered: the same bits every call
-    assert relerr(jv.cpu().numpy() - 1.0, P.jacobian_action(u, JF3)) <= TOL
+    ref, jref = P.residual(u, F1), P.jacobian_action(u, JF3)
+    # colour passes (bitwise reproducible) and the tile path (LDS accumulation, halo slots)
+    for acc in (L.ACCUMULATE_COLORED, L.ACCUMULATE_AUTO):
+        dm.set_kernel_options(L.ALGO_POINTFN, acc, 0)
+        r = torch.zeros(3 * nn, dtype=torch.float64, device="cuda")
+        dm.compute_residual(0, 0, dev(u), r, mode=L.INSERT_VALUES)
+        r2 = torch.full_like(r, 3.0)
+        dm.compute_residual(0, 0, dev(u), r2, mode=L.INSERT_VALUES)
+        jv = torch.full_like(r, 1.0)
+        dm.compute_jacobian_action(0, 0, dev(u), jv, mode=L.ADD_VALUES)
+        torch.cuda.synchronize()
+        assert relerr(r.cpu().numpy(), ref) <= TOL, acc
+        if acc == L.ACCUMULATE_COLORED:
+            assert torch.equal(r, r2)  # colour-ordered: the same bits every call
+        else:
+            assert relerr(r2.cpu().numpy(), ref) <= TOL
+        assert relerr(jv.cpu().numpy() - 1.0, jref) <= TOL, acc
     dm.close()
     # reference state with gravity and body force: aux [density, body_force, gravity, ref stress,
     # ref strain, mu, K]
