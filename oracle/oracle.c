@@ -307,9 +307,10 @@ static void f0_test(int dim, int numS, int numA, const int sOff[], const int sOf
                     const double s_t[], const double s_x[], const int aOff[], const int aOff_x[],
                     const double a[], const double a_t[], const double a_x[], double t, const double x[],
                     int numConstants, const double constants[], double f0[]) {
-    (void)numS; (void)numA; (void)sOff_x; (void)s_t; (void)s_x; (void)aOff_x; (void)a_t; (void)a_x;
+    (void)numS; (void)numA; (void)sOff_x; (void)s_x; (void)aOff_x; (void)a_t; (void)a_x;
     (void)numConstants;
-    for (int i = 0; i < dim; ++i) f0[i] += constants[1] * s[sOff[0] + i] + a[aOff[0]] * (constants[0] * x[i] + t);
+    for (int i = 0; i < dim; ++i)
+        f0[i] += constants[1] * s[sOff[0] + i] + a[aOff[0]] * (constants[0] * x[i] + t + s_t[sOff[0] + i]);
 }
 
 static void f1_test(int dim, int numS, int numA, const int sOff[], const int sOff_x[], const double s[],
@@ -332,9 +333,9 @@ static void jf0_test(int dim, int numS, int numA, const int sOff[], const int sO
                      const double s_t[], const double s_x[], const int aOff[], const int aOff_x[],
                      const double a[], const double a_t[], const double a_x[], double t, double s_tshift,
                      const double x[], int numConstants, const double constants[], double J[]) {
-    (void)numS; (void)numA; (void)sOff; (void)sOff_x; (void)s; (void)s_t; (void)s_x; (void)aOff; (void)aOff_x;
-    (void)a; (void)a_t; (void)a_x; (void)t; (void)s_tshift; (void)x; (void)numConstants;
-    for (int i = 0; i < dim; ++i) J[i * dim + i] += constants[1];
+    (void)numS; (void)numA; (void)sOff; (void)sOff_x; (void)s; (void)s_t; (void)s_x; (void)aOff_x;
+    (void)a_t; (void)a_x; (void)t; (void)x; (void)numConstants;
+    for (int i = 0; i < dim; ++i) J[i * dim + i] += constants[1] + s_tshift * a[aOff[0]];
 }
 
 static void jf3_test(int dim, int numS, int numA, const int sOff[], const int sOff_x[], const double s[],

@@ -23,7 +23,7 @@ __device__ void f0_test(const PylithInt dim, const PylithInt numS, const PylithI
                         const PylithReal t, const PylithScalar x[], const PylithInt numConstants,
                         const PylithScalar constants[], PylithScalar f0[]) {
     for (PylithInt i = 0; i < dim; ++i)
-        f0[i] += constants[1] * s[sOff[0] + i] + a[aOff[0]] * (constants[0] * x[i] + t);
+        f0[i] += constants[1] * s[sOff[0] + i] + a[aOff[0]] * (constants[0] * x[i] + t + s_t[sOff[0] + i]);
 }
 
 __device__ void f1_test(const PylithInt dim, const PylithInt numS, const PylithInt numA, const PylithInt sOff[],
@@ -48,7 +48,7 @@ __device__ void jf0_test(const PylithInt dim, const PylithInt numS, const Pylith
                          const PylithScalar a[], const PylithScalar a_t[], const PylithScalar a_x[],
                          const PylithReal t, const PylithReal s_tshift, const PylithScalar x[],
                          const PylithInt numConstants, const PylithScalar constants[], PylithScalar Jf0[]) {
-    for (PylithInt i = 0; i < dim; ++i) Jf0[i * dim + i] += constants[1];
+    for (PylithInt i = 0; i < dim; ++i) Jf0[i * dim + i] += constants[1] + s_tshift * a[aOff[0]];
 }
 
 __device__ void jf3_test(const PylithInt dim, const PylithInt numS, const PylithInt numA, const PylithInt sOff[],

@@ -171,6 +171,16 @@ def test_caller_kernels_through_launchers(oracle, kind):
     assert relerr(r.cpu().numpy(), ref) <= TOL
     jref = P.jacobian_action(u, T_JF3, jf0=T_JF0, t=t)
     assert relerr(jv.cpu().numpy(), jref) <= TOL
+    # with a time derivative (the test f0 reads s_t): tile path and colour passes
+    ut = rng.standard_normal(n) * 1e-2
+    ref_t = P.residual(u, T_F1, f0=T_F0, t=t, locX_t=ut)
+    for acc in (L.ACCUMULATE_AUTO, L.ACCUMULATE_COLORED):
+        dm.set_kernel_options(L.ALGO_AUTO, acc, 0)
+        rt = torch.zeros_like(r)
+        dm.compute_residual(0, 0, dev(u), rt, mode=L.INSERT_VALUES, t=t, locX_t=dev(ut))
+        torch.cuda.synchronize()
+        assert relerr(rt.cpu().numpy(), ref_t) <= TOL, acc
+    dm.set_kernel_options(L.ALGO_AUTO, L.ACCUMULATE_AUTO, 0)
     # residual is affine in u: F(u) - F(0) = J u
     r0 = torch.zeros_like(r)
     dm.compute_residual(0, 0, torch.zeros_like(r), r0, mode=L.INSERT_VALUES, t=t)

@@ -297,3 +297,36 @@ def test_tangential_directions(oracle):
     # normal along the first reference direction switches to the second
     t1, t2 = oracle.tangential_directions([0, 0, 1], [0, 1, 0], [0, 0, 1])
     assert np.allclose(np.abs(t1), [1, 0, 0])
+
+
+@pytest.mark.parametrize("kind", ["q1", "p2"])
+def test_time_derivative_enters_the_test_f0(oracle, kind):
+    """The test-only f0 adds a0 * s_t (a0 the first aux subfield): a uniform unit u_t in one
+    component changes that component's residual by sum_i int a0 phi_i = sum_cells a0_c vol_c
+    (partition of unity), and nothing else; Jf0 carries s_tshift * a0 on its diagonal."""
+    if kind == "q1":
+        m = mg.hex_box((3, 2, 2), (0, 0, 0), (3000.0, 2000.0, 2000.0), brick=2, shear=0.2)
+    else:
+        m = mg.tet_p2_box(2, (0, 0, 0), (2000.0, 2000.0, 2000.0), brick=1)
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=3, vs_sigma=0.2, vp_vs=1.8)
+    aux[:, 0] *= np.random.default_rng(5).uniform(0.5, 1.5, m.num_cells)  # density varies by cell
+    consts = [2.5e-3, 3.0e7, 0.3]
+    P = oracle.Problem(m, aux, sizes, constants=consts)
+    # congruent cells (a sheared brick grid / the six tets of each cube): equal volumes
+    vol = np.full(m.num_cells, (3000.0 if kind == "q1" else 2000.0) * 2000.0 * 2000.0 / m.num_cells)
+    n = 3 * m.num_nodes
+    u = np.random.default_rng(4).standard_normal(n) * 1e-3
+    T_F0, T_F1, T_JF0 = 90, 91, 92
+    r0 = P.residual(u, T_F1, f0=T_F0, t=0.4)
+    for comp in range(3):
+        ut = np.zeros(n)
+        ut[comp::3] = 1.0
+        d = P.residual(u, T_F1, f0=T_F0, t=0.4, locX_t=ut) - r0
+        assert abs(d[comp::3].sum() - (aux[:, 0] * vol).sum()) <= 1e-10 * (aux[:, 0] * vol).sum()
+        others = [k for k in range(3) if k != comp]
+        assert np.abs(d[others[0]::3]).max() == 0.0 and np.abs(d[others[1]::3]).max() == 0.0
+    # the s_tshift term of Jf0: J(s_tshift) - J(0) applied to a uniform field, summed, = s_tshift int a0
+    e = np.zeros(n)
+    e[0::3] = 1.0
+    dj = P.jacobian_action(e, 0, jf0=T_JF0, t=0.4, s_tshift=2.0) - P.jacobian_action(e, 0, jf0=T_JF0, t=0.4)
+    assert abs(dj[0::3].sum() - 2.0 * (aux[:, 0] * vol).sum()) <= 1e-10 * (aux[:, 0] * vol).sum()
