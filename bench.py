@@ -613,7 +613,7 @@ def main():
     ap.add_argument("--no-systems", action="store_true",
                     help="skip the solver / viscoelastic / Green's-function / serving measurements")
     ap.add_argument("--only", choices=["c1solve", "c2solve", "c3", "c3amg", "c4", "c5", "c5b64", "c5b32", "c5sweep", "q1",
-                                       "p2large"],
+                                       "p2large", "pointfn"],
                     default=None,
                     help="run one system measurement alone and print its JSON")
     args = ap.parse_args()
@@ -640,7 +640,11 @@ def main():
               "c5b64": lambda: bench_c5(torch, max_block=64), "c5sweep": lambda: bench_c5_sweep(torch),
               "c5b32": lambda: bench_c5(torch, max_block=32),
               "q1": lambda: bench_q1(torch, mg, args.q1_large, args.steps, args.warmup),
-              "p2large": lambda: bench_p2(torch, mg, 96, args.steps, args.warmup)}[args.only]
+              "p2large": lambda: bench_p2(torch, mg, 96, args.steps, args.warmup),
+              "pointfn": lambda: {"p2_64cube": bench_p2(torch, mg, 64, args.steps, args.warmup,
+                                                        algorithm=L.ALGO_POINTFN),
+                                  "q1_192cube": bench_q1(torch, mg, args.q1_large, args.steps, args.warmup,
+                                                         algorithm=L.ALGO_POINTFN)}}[args.only]
         print(f"[bench] {args.only}", file=sys.stderr, flush=True)
         print(json.dumps({args.only: fn()}))
         return
