@@ -705,11 +705,17 @@ __device__ inline void integrate_q1(const plx_pointfn_args &A, int cell) {
 // and leaves the shared nodes' partial sums in their halo slots (reduce_shared adds them).
 // Consecutive tiles run on one XCD (workgroups are dispatched round-robin over the 8 XCDs), so
 // nodes shared by neighbouring tiles meet in that XCD's L2.
-constexpr int kTileThreads = 256;
+// Threads per tile workgroup: Q1 tiles hold up to 512 cells (2 per thread), P2 tiles up to 384
+// (3 per thread in 2-wave workgroups, ~4 resident per CU: 5 % faster than 256 threads with one
+// or two cells each).
+template <int CT>
+struct TileThreads {
+    static constexpr int NT = CT == PLX_TET_P2 ? 128 : 256;
+};
 
 template <class TERMS, bool ACTION, int CT>
-__global__ void __launch_bounds__(kTileThreads) tile_cells(plx_pointfn_args A) {
-    constexpr int NB = Basis<CT>::NB, NV = Basis<CT>::NV;
+__global__ void __launch_bounds__(TileThreads<CT>::NT) tile_cells(plx_pointfn_args A) {
+    constexpr int NB = Basis<CT>::NB, NV = Basis<CT>::NV, kTileThreads = TileThreads<CT>::NT;
     extern __shared__ double pf_lds[];
     const int tid = threadIdx.x;
     const int n = A.num_tiles, blk = blockIdx.x;
@@ -893,7 +899,7 @@ inline int launch_tiles(const plx_pointfn_args *A, hipStream_t s) {
     if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void *>(&tile_cells<TERMS, ACTION, CT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return 76;
-    hipLaunchKernelGGL((tile_cells<TERMS, ACTION, CT>), dim3(A->num_tiles), dim3(kTileThreads), lds, s, *A);
+    hipLaunchKernelGGL((tile_cells<TERMS, ACTION, CT>), dim3(A->num_tiles), dim3(TileThreads<CT>::NT), lds, s, *A);
     if (A->num_shared > 0)
         hipLaunchKernelGGL(reduce_shared, dim3((A->num_shared + 255) / 256), dim3(256), 0, s, *A);
     return hipGetLastError() == hipSuccess ? 0 : 76;
