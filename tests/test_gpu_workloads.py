@@ -11,6 +11,10 @@
   to 1e-12) is <= the solve tolerance, and every impulse's error against a 1e-14 FP64 solve is
   <= 1e-8.  A 20-impulse block on a 12 x 8 x 8 mesh matches a direct solve of the oracle's
   assembled saddle-point system.
+* C2: P2 64^3 x 6 (6.44 M dofs, lognormal moduli, shear Dirichlet values) solved by FGMRES with
+  the FP32 multigrid PCG inside FP64 refinement to 1e-10: the oracle's FP64 residual of that
+  solution on the free dofs is <= 1e-9 of the initial one, and it equals an FP64 CG solve to
+  1e-12 (multigrid-preconditioned) within 1e-8.
 * C5: the serving mix of 16^3 / 32^3 / 64^3 / 96^3 meshes: every request's FP64 residual is
   <= the serving tolerance (1e-9), its error against a 1e-13 FP64 solve is <= 1e-8, and
   requests solved alone equal the same requests solved in the mix to 1e-8.
@@ -274,3 +278,41 @@ def test_c5_serving_mix_16_to_96():
         assert d <= 1e-8, (sz, d)
     for s in solvers.values():
         s.close()
+
+
+def test_c2_fp32_refinement_against_tight_fp64(oracle):
+    from pylith_amd.dm import DM, Solver
+    m = mg.tet_p2_box(64, (0.0, 0.0, 0.0), (6000.0, 6000.0, 6000.0))
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=20261202, vs_sigma=0.2, vp_vs=1.732)
+    dofs, vals = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    dm = DM(m)
+    dm.add_residual(0, 0, 0, 1)
+    dm.add_jacobian(0, 0, 0, 0, 0, 2)
+    dm.set_auxiliary(0, 0, aux, sizes)
+    dm.set_constraints(dofs)
+    solver = Solver(dm)
+    solver.set_preconditioner("amg")
+    solver.set_inner(1e-3, 20000, 30)
+    print(f"[c2] dofs={3 * m.num_nodes} hierarchy built", flush=True)
+    sols = {}
+    for name, method, rtol in (("fp64_cg", L.KSP_CG, 1e-12), ("fp32_ir", L.KSP_FGMRES_IR, 1e-10)):
+        x = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+        dm.insert_boundary_values(torch.from_numpy(vals).cuda(), x)
+        st = solver.solve(x, method=method, rtol=rtol, maxit=20000)
+        print(f"[c2] {name} {st}", flush=True)
+        assert st["converged"], st
+        sols[name] = x.cpu().numpy()
+    a, b = sols["fp32_ir"], sols["fp64_cg"]
+    assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    # the oracle's FP64 residual of the refined solution vanishes on the unconstrained dofs
+    P = oracle.Problem(m, aux, sizes)
+    r = P.residual(a, 1)
+    u0 = np.zeros_like(a)
+    u0[dofs] = vals
+    r0 = P.residual(u0, 1)
+    free = np.setdiff1d(np.arange(a.size), dofs)
+    rel = np.linalg.norm(r[free]) / np.linalg.norm(r0[free])
+    print(f"[c2] oracle residual of the FP32-refined solution {rel:.3e}", flush=True)
+    assert rel <= 1e-9
+    solver.close()
+    dm.close()
