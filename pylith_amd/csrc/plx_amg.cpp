@@ -99,6 +99,74 @@ inline void gemm_t6(const double *__restrict__ P, const double *__restrict__ X, 
         }
 }
 
+// Inverse diagonal blocks of a b x b block operator; false when a row has no diagonal block.
+bool level_dinv(int n, int b, const int32_t *Ap, const int32_t *Ac, const double *Av, AmgVal *Dinv) {
+    const int T = num_threads(), bb = b * b;
+    Dinv->assign((size_t)n * bb, 0.0);
+    std::vector<uint8_t> no_diag(T, 0);
+    parallel_chunks(n, T, [&](int lo, int hi, int t) {
+        for (int i = lo; i < hi; ++i) {
+            int d = -1;
+            for (int e = Ap[i]; e < Ap[i + 1]; ++e)
+                if (Ac[e] == i) d = e;
+            if (d < 0) {
+                no_diag[t] = 1;
+                return;
+            }
+            block_inverse(Av + (size_t)d * bb, b, Dinv->data() + (size_t)i * bb);
+        }
+    });
+    return std::find(no_diag.begin(), no_diag.end(), 1) == no_diag.end();
+}
+
+// 1.1 x the largest eigenvalue of D^-1 A, 12 power iterations from a fixed random vector.
+double level_lmax(int n, int b, const int32_t *Ap, const int32_t *Ac, const double *Av, const AmgVal &Dinv) {
+    const int T = num_threads(), bb = b * b;
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd;
+    const size_t nd_ = (size_t)n * b;
+    std::vector<double> v(nd_), w(nd_);
+    for (auto &x : v) x = nd(rng);
+    double lam = 0.0;
+    std::vector<double> part(T);
+    double vn = 0.0;
+    for (double x : v) vn += x * x;
+    vn = std::sqrt(vn);
+    for (int it = 0; it < 12; ++it) {
+        std::fill(part.begin(), part.end(), 0.0);
+        parallel_chunks(n, T, [&](int lo, int hi, int t) {
+            double acc = 0.0;
+            for (int i = lo; i < hi; ++i) {
+                double y[6] = {0, 0, 0, 0, 0, 0};
+                for (int e = Ap[i]; e < Ap[i + 1]; ++e) {
+                    const double *A = &Av[(size_t)e * bb];
+                    const double *x = &v[(size_t)Ac[e] * b];
+                    for (int r = 0; r < b; ++r) {
+                        double sacc = 0.0;
+                        for (int c = 0; c < b; ++c) sacc += A[r * b + c] * x[c];
+                        y[r] += sacc;
+                    }
+                }
+                const double *Di = &Dinv[(size_t)i * bb];
+                for (int r = 0; r < b; ++r) {
+                    double sacc = 0.0;
+                    for (int c = 0; c < b; ++c) sacc += Di[r * b + c] * y[c];
+                    w[(size_t)i * b + r] = sacc / vn;
+                    acc += sacc * sacc;
+                }
+            }
+            part[t] = acc;
+        });
+        double wn = 0.0;
+        for (double x : part) wn += x;
+        wn = std::sqrt(wn) / vn;  // || D^-1 A v || / || v || with v scaled to unit norm
+        lam = wn;
+        v.swap(w);
+        vn = wn;
+    }
+    return 1.1 * lam;
+}
+
 struct Csr6 {  // block rows of b x 6 blocks, sorted columns
     AmgIdx ptr, col;
     AmgVal val;
@@ -204,21 +272,7 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
         L.b = b;
         const int bb = b * b;
         // inverse diagonal blocks
-        L.Dinv.assign((size_t)n * bb, 0.0);
-        std::vector<uint8_t> no_diag(T, 0);
-        parallel_chunks(n, T, [&](int lo, int hi, int t) {
-            for (int i = lo; i < hi; ++i) {
-                int d = -1;
-                for (int e = Ap[i]; e < Ap[i + 1]; ++e)
-                    if (Ac[e] == i) d = e;
-                if (d < 0) {
-                    no_diag[t] = 1;
-                    return;
-                }
-                block_inverse(Av + (size_t)d * bb, b, L.Dinv.data() + (size_t)i * bb);
-            }
-        });
-        if (std::find(no_diag.begin(), no_diag.end(), 1) != no_diag.end()) {
+        if (!level_dinv(n, b, Ap, Ac, Av, &L.Dinv)) {
             *err = "amg: missing diagonal block";
             return 1;
         }
@@ -361,51 +415,7 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
                 std::copy(&Q[(size_t)i * b * 6], &Q[(size_t)(i + 1) * b * 6], &P0.val[(size_t)P0.ptr[i] * b * 6]);
             }
         // spectrum bound of D^-1 A (power iteration, 1.1 x the estimate)
-        {
-            std::mt19937_64 rng(7);
-            std::normal_distribution<double> nd;
-            const size_t nd_ = (size_t)n * b;
-            std::vector<double> v(nd_), w(nd_);
-            for (auto &x : v) x = nd(rng);
-            double lam = 0.0;
-            std::vector<double> part(T);
-            double vn = 0.0;
-            for (double x : v) vn += x * x;
-            vn = std::sqrt(vn);
-            for (int it = 0; it < 12; ++it) {
-                std::fill(part.begin(), part.end(), 0.0);
-                parallel_chunks(n, T, [&](int lo, int hi, int t) {
-                    double acc = 0.0;
-                    for (int i = lo; i < hi; ++i) {
-                        double y[6] = {0, 0, 0, 0, 0, 0};
-                        for (int e = Ap[i]; e < Ap[i + 1]; ++e) {
-                            const double *A = &Av[(size_t)e * bb];
-                            const double *x = &v[(size_t)Ac[e] * b];
-                            for (int r = 0; r < b; ++r) {
-                                double sacc = 0.0;
-                                for (int c = 0; c < b; ++c) sacc += A[r * b + c] * x[c];
-                                y[r] += sacc;
-                            }
-                        }
-                        const double *Di = &L.Dinv[(size_t)i * bb];
-                        for (int r = 0; r < b; ++r) {
-                            double sacc = 0.0;
-                            for (int c = 0; c < b; ++c) sacc += Di[r * b + c] * y[c];
-                            w[(size_t)i * b + r] = sacc / vn;
-                            acc += sacc * sacc;
-                        }
-                    }
-                    part[t] = acc;
-                });
-                double wn = 0.0;
-                for (double x : part) wn += x;
-                wn = std::sqrt(wn) / vn;  // || D^-1 A v || / || v || with v scaled to unit norm
-                lam = wn;
-                v.swap(w);
-                vn = wn;
-            }
-            L.lmax = 1.1 * lam;
-        }
+        L.lmax = level_lmax(n, b, Ap, Ac, Av, L.Dinv);
         // smoothed prolongator P = P0 - omega D^-1 A_F P0.  With a threshold, A_F is the filtered
         // operator: the strong blocks, the weak blocks of a row lumped onto its diagonal block
         // (row sums kept, so translations stay in the range of P); it keeps P's stencil, and
@@ -594,6 +604,133 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
     }
     h->coarse_n = m;
     h->coarse_inv.swap(I);
+    return 0;
+}
+
+int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed, int nv,
+                 const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz, int coarse_max,
+                 int max_levels, AmgHierarchy *h, std::string *err, double theta) {
+    h->levels.clear();
+    h->coarse_inv.clear();
+    if (max_levels < 2) {
+        *err = "amg: a P2 hierarchy needs at least two levels";
+        return 1;
+    }
+    AmgLevel L0;
+    L0.n = n;
+    L0.b = 3;
+    L0.nc = nv;
+    L0.bc = 3;
+    if (!level_dinv(n, 3, ptr, col, val, &L0.Dinv)) {
+        *err = "amg: missing diagonal block";
+        return 1;
+    }
+    L0.lmax = level_lmax(n, 3, ptr, col, val, L0.Dinv);
+    // fixed vertex dofs: those of the vertex's own node (the node with that single parent)
+    std::vector<uint8_t> fixed1(3 * (size_t)nv, 0);
+    for (int i = 0; i < n; ++i) {
+        if (par_ptr[i + 1] - par_ptr[i] != 1) continue;
+        const int I = par[par_ptr[i]];
+        if (I < 0 || I >= nv) {
+            *err = "amg: P2 parent out of range";
+            return 1;
+        }
+        for (int c = 0; c < 3; ++c)
+            if (fixed[3 * (size_t)i + c]) fixed1[3 * (size_t)I + c] = 1;
+    }
+    // prolongator: diagonal 3 x 3 blocks, zero where the fine or the coarse dof is fixed
+    const int np = par_ptr[n];
+    L0.Pptr.assign(par_ptr, par_ptr + n + 1);
+    L0.Pcol.resize(np);
+    L0.Pval.assign((size_t)np * 9, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = par_ptr[i]; k < par_ptr[i + 1]; ++k) {
+            const int I = par[k];
+            if (I < 0 || I >= nv) {
+                *err = "amg: P2 parent out of range";
+                return 1;
+            }
+            L0.Pcol[k] = I;
+            for (int c = 0; c < 3; ++c)
+                L0.Pval[(size_t)k * 9 + 4 * c] = (fixed[3 * (size_t)i + c] || fixed1[3 * (size_t)I + c]) ? 0.0 : parw[k];
+        }
+    // restriction P^T: vertex I's children (fine rows) with the same diagonal blocks
+    L0.Rptr.assign(nv + 1, 0);
+    for (int k = 0; k < np; ++k) L0.Rptr[L0.Pcol[k] + 1]++;
+    for (int I = 0; I < nv; ++I) L0.Rptr[I + 1] += L0.Rptr[I];
+    L0.Rcol.resize(np);
+    L0.Rval.resize((size_t)np * 9);
+    AmgIdx rsrc(np);
+    {
+        std::vector<int32_t> fill(L0.Rptr.begin(), L0.Rptr.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int k = par_ptr[i]; k < par_ptr[i + 1]; ++k) {
+                const int at = fill[L0.Pcol[k]]++;
+                L0.Rcol[at] = i;
+                rsrc[at] = k;
+                std::copy(&L0.Pval[(size_t)k * 9], &L0.Pval[(size_t)(k + 1) * 9], &L0.Rval[(size_t)at * 9]);
+            }
+    }
+    // Galerkin product A1 = P^T A P: coarse row I over its children i, their rows and the parents
+    // of each column
+    Csr6 A1;
+    {
+        const int T = num_threads();
+        std::vector<std::vector<int>> marks(T, std::vector<int>(nv, -1)), lists(T), orders(T);
+        std::vector<std::vector<double>> accs(T);
+        build_rows(nv, 9, &A1, [&](int I, AmgIdx &ocol, AmgVal &oval, int t) {
+            std::vector<int> &mark = marks[t], &list = lists[t], &order = orders[t];
+            std::vector<double> &acc = accs[t];
+            list.clear();
+            acc.clear();
+            for (int u = L0.Rptr[I]; u < L0.Rptr[I + 1]; ++u) {
+                const int i = L0.Rcol[u];
+                const double *pi = &L0.Pval[(size_t)rsrc[u] * 9];
+                for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+                    const int j = col[e];
+                    const double *A = val + (size_t)e * 9;
+                    for (int k = par_ptr[j]; k < par_ptr[j + 1]; ++k) {
+                        const int J = L0.Pcol[k];
+                        const double *pj = &L0.Pval[(size_t)k * 9];
+                        if (mark[J] < 0) {
+                            mark[J] = (int)list.size();
+                            list.push_back(J);
+                            acc.resize(acc.size() + 9, 0.0);
+                        }
+                        double *o = &acc[(size_t)mark[J] * 9];
+                        for (int c = 0; c < 3; ++c)
+                            for (int d = 0; d < 3; ++d) o[3 * c + d] += pi[4 * c] * A[3 * c + d] * pj[4 * d];
+                    }
+                }
+            }
+            order.resize(list.size());
+            for (size_t u = 0; u < order.size(); ++u) order[u] = (int)u;
+            std::sort(order.begin(), order.end(), [&](int x, int y) { return list[x] < list[y]; });
+            for (int u : order) {
+                const int J = list[u];
+                double *o = &acc[(size_t)u * 9];
+                for (int c = 0; c < 3; ++c)
+                    for (int d = 0; d < 3; ++d)
+                        if (fixed1[3 * (size_t)I + c] || fixed1[3 * (size_t)J + d])
+                            o[3 * c + d] = (I == J && c == d && fixed1[3 * (size_t)I + c]) ? 1.0 : 0.0;
+                ocol.push_back(J);
+                oval.insert(oval.end(), o, o + 9);
+            }
+            for (int J : list) mark[J] = -1;
+        });
+    }
+    AmgHierarchy h1;
+    if (amg_setup(nv, A1.ptr.data(), A1.col.data(), A1.val.data(), vxyz, fixed1.data(), coarse_max, max_levels - 1,
+                  &h1, err, theta))
+        return 1;
+    // amg_setup leaves its finest operator with the caller: here it is level 1's
+    h1.levels[0].Aptr = std::move(A1.ptr);
+    h1.levels[0].Acol = std::move(A1.col);
+    h1.levels[0].Aval = std::move(A1.val);
+    h->levels.push_back(std::move(L0));
+    for (auto &L : h1.levels) h->levels.push_back(std::move(L));
+    h->coarse_n = h1.coarse_n;
+    h->coarse_inv.swap(h1.coarse_inv);
     return 0;
 }
 

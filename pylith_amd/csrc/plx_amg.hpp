@@ -45,10 +45,11 @@ struct AmgLevel {
     int n = 0;       // block rows
     int b = 0;       // block size (3 on the finest level, 6 below)
     int nc = 0;      // block rows of the next level
+    int bc = 6;      // block size of the next level (3 below a P2 -> P1 level)
     // operator of this level (block CSR, b x b blocks row-major); level 0 uses the caller's copy
     AmgIdx Aptr, Acol;
     AmgVal Aval;
-    // prolongator (n rows, b x 6 blocks) and its transpose (nc rows, 6 x b blocks)
+    // prolongator (n rows, b x bc blocks) and its transpose (nc rows, bc x b blocks)
     AmgIdx Pptr, Pcol, Rptr, Rcol;
     AmgVal Pval, Rval;
     AmgVal Dinv;  // [n][b][b] inverse diagonal blocks
@@ -69,5 +70,14 @@ struct AmgHierarchy {
 int amg_setup(int n, const int32_t *ptr, const int32_t *col, const double *val, const double *coords,
               const uint8_t *fixed, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err,
               double theta = 0.0);
+
+// P2 operators: a first level coarsens the P2 nodes onto the mesh vertices by the P1
+// interpolation (a vertex node takes its vertex's value, an edge node the mean of its two
+// vertices': parents par[par_ptr[i] ..] with weights parw), the Galerkin product gives the P1
+// operator, and amg_setup builds the rest of the hierarchy on it (vertex coordinates vxyz
+// [nv][3], threshold theta).  A vertex dof is fixed when its own node's dof is.
+int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed, int nv,
+                 const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz, int coarse_max,
+                 int max_levels, AmgHierarchy *h, std::string *err, double theta = 0.0);
 
 }  // namespace plx

@@ -745,6 +745,11 @@ struct plx_gf_s {
     std::vector<int32_t> h_ptr, h_col;
     std::vector<double> h_val;
     std::vector<uint8_t> h_fixed;
+    // P2 handles: the P1 interpolation onto the vertices that the multigrid's first level takes
+    // (parents of each row in vertex numbering, weights 1 or 1/2; vertex coordinates)
+    int p2_nv = 0;
+    std::vector<int32_t> p2_par_ptr, p2_par;
+    std::vector<double> p2_parw, p2_vxyz;
     double *d_val64 = nullptr;
     float *d_val32 = nullptr;
     double *d_Binv64 = nullptr;
@@ -770,6 +775,7 @@ struct plx_gf_s {
     double amg_theta = 0.0;
     struct Level {
         int n = 0, b = 0, nc = 0;
+        int bc = 6;  // block size of the next level (3 below the P2 -> P1 level)
         int32_t *Aptr = nullptr, *Acol = nullptr, *Pptr = nullptr, *Pcol = nullptr, *Rptr = nullptr, *Rcol = nullptr;
         float *Aval = nullptr, *Pval = nullptr, *Rval = nullptr, *Dinv = nullptr;
         float lmax = 0.f;
@@ -962,9 +968,15 @@ static void amg_level(plx_gf g, int l, const float *bv, float *xv) {
     float *nx = l + 1 < nl ? g->lv[l + 1].xv : g->cx;
     hipLaunchKernelGGL((amg_jacobi0<B, K>), gf_grid((int64_t)L.n * K), dim3(256), 0, g->stream, L.n, L.Dinv, bv, x, s);
     amg_bsr_launch<B, B, K, kBsrResid>(g, L.n, Ap, Ac, Av, x, bv, nullptr, r, 0.f, 0);
-    amg_bsr_launch<6, B, K, kBsrPlain, 64>(g, L.nc, L.Rptr, L.Rcol, L.Rval, r, nullptr, nullptr, nb, 0.f, 0);
+    if (B == 3 && L.bc == 3)  // the P2 -> P1 level: 3 x 3 transfer blocks (~15 children a vertex)
+        amg_bsr_launch<3, 3, K, kBsrPlain, 16>(g, L.nc, L.Rptr, L.Rcol, L.Rval, r, nullptr, nullptr, nb, 0.f, 0);
+    else
+        amg_bsr_launch<6, B, K, kBsrPlain, 64>(g, L.nc, L.Rptr, L.Rcol, L.Rval, r, nullptr, nullptr, nb, 0.f, 0);
     amg_vcycle<K>(g, l + 1, nb, nx);
-    amg_bsr_launch<B, 6, K, kBsrPlain, 8>(g, L.n, L.Pptr, L.Pcol, L.Pval, nx, nullptr, nullptr, x, 0.f, 1);
+    if (B == 3 && L.bc == 3)  // one or two parents a node
+        amg_bsr_launch<3, 3, K, kBsrPlain, 2>(g, L.n, L.Pptr, L.Pcol, L.Pval, nx, nullptr, nullptr, x, 0.f, 1);
+    else
+        amg_bsr_launch<B, 6, K, kBsrPlain, 8>(g, L.n, L.Pptr, L.Pcol, L.Pval, nx, nullptr, nullptr, x, 0.f, 1);
     amg_bsr_launch<B, B, K, kBsrPost>(g, L.n, Ap, Ac, Av, x, bv, L.Dinv, xv, s, 0);
 }
 
@@ -1146,9 +1158,13 @@ static int gf_amg_setup(plx_gf g) {
     AmgHierarchy h;
     std::string msg;
     if (g->xyz.size() != 3 * (size_t)nn) return set_error(dm, 1, "node coordinates missing");
-    if (amg_setup(nn, ptr.data(), col.data(), val.data(), g->xyz.data(), fixed.data(), 768, 8, &h, &msg,
-                  g->amg_theta))
-        return set_error(dm, 1, msg.c_str());
+    const int aerr = g->p2_nv > 0
+                         ? amg_setup_p2(nn, ptr.data(), col.data(), val.data(), fixed.data(), g->p2_nv,
+                                        g->p2_par_ptr.data(), g->p2_par.data(), g->p2_parw.data(),
+                                        g->p2_vxyz.data(), 768, 8, &h, &msg, g->amg_theta)
+                         : amg_setup(nn, ptr.data(), col.data(), val.data(), g->xyz.data(), fixed.data(), 768, 8,
+                                     &h, &msg, g->amg_theta);
+    if (aerr) return set_error(dm, 1, msg.c_str());
     gf_amg_free(g);
     auto upf = [&](float **d, const auto &v) -> int {
         std::vector<float> f(v.begin(), v.end());
@@ -1173,6 +1189,7 @@ static int gf_amg_setup(plx_gf g) {
         L.n = H.n;
         L.b = H.b;
         L.nc = H.nc;
+        L.bc = H.bc;
         L.lmax = (float)H.lmax;
         if (l > 0 && ((err = upi(&L.Aptr, H.Aptr)) || (err = upi(&L.Acol, H.Acol)) || (err = upf(&L.Aval, H.Aval))))
             return err;
@@ -1200,11 +1217,12 @@ static int gf_amg_setup(plx_gf g) {
         for (size_t k = 0; k < rcol.size(); ++k) rcol[k] = g->perm[H.Rcol[k]];
         std::vector<int32_t> pptr(nn + 1, 0), pcol(H.Pcol.size());
         std::vector<double> pval(H.Pval.size());
+        const size_t bsz = (size_t)H.b * H.bc;  // values per prolongator block
         for (int v = 0; v < nn; ++v) {
             const int r = iperm[v], e0 = H.Pptr[r], e1 = H.Pptr[r + 1];
             pptr[v + 1] = pptr[v] + (e1 - e0);
-            std::copy(&H.Pcol[e0], &H.Pcol[e1], &pcol[pptr[v]]);
-            std::copy(&H.Pval[18 * (size_t)e0], &H.Pval[18 * (size_t)e1], &pval[18 * (size_t)pptr[v]]);
+            std::copy(H.Pcol.begin() + e0, H.Pcol.begin() + e1, pcol.begin() + pptr[v]);
+            std::copy(H.Pval.begin() + bsz * e0, H.Pval.begin() + bsz * e1, pval.begin() + bsz * pptr[v]);
         }
         if ((err = upf(&g->Dinv_dm, dinv)) || (err = upi(&g->Rcol_dm, rcol)) || (err = upi(&g->Pptr_dm, pptr)) ||
             (err = upi(&g->Pcol_dm, pcol)) || (err = upf(&g->Pval_dm, pval)))
@@ -1354,7 +1372,44 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
     g->ndof = 3 * (int64_t)nn;
     g->nnzb = nnzb;
     g->op_epoch = dm->op_epoch;
-    g->amg_theta = 0.1;
+    g->amg_theta = 0.05;  // of the P1 operator below the P2 -> P1 level
+    if (dm->cell_type == PLX_TET_P2) {
+        // vertices in row order; a vertex node's parent is its vertex, an edge node's the two ends
+        std::vector<int32_t> vidx(nn, -1), pa(2 * (size_t)nn, -1);
+        for (int c = 0; c < dm->num_cells; ++c) {
+            const int32_t *cl = &dm->closure[(size_t)c * nb];
+            for (int v = 0; v < 4; ++v) vidx[iperm[cl[v]]] = 0;
+            for (int e = 0; e < 6; ++e) {
+                const size_t r = (size_t)iperm[cl[4 + e]];
+                pa[2 * r] = iperm[cl[kEdge[e][0]]];
+                pa[2 * r + 1] = iperm[cl[kEdge[e][1]]];
+            }
+        }
+        int nv1 = 0;
+        for (int r = 0; r < nn; ++r)
+            if (vidx[r] == 0) vidx[r] = nv1++;
+        g->p2_nv = nv1;
+        g->p2_par_ptr.assign(nn + 1, 0);
+        g->p2_vxyz.resize(3 * (size_t)nv1);
+        for (int r = 0; r < nn; ++r) {
+            if (vidx[r] >= 0) {
+                g->p2_par.push_back(vidx[r]);
+                g->p2_parw.push_back(1.0);
+                for (int d = 0; d < 3; ++d) g->p2_vxyz[3 * (size_t)vidx[r] + d] = nx[3 * (size_t)perm[r] + d];
+            } else {
+                const int a = pa[2 * (size_t)r], b = pa[2 * (size_t)r + 1];
+                if (a < 0 || b < 0 || vidx[a] < 0 || vidx[b] < 0) {
+                    plx_gf_destroy(g);
+                    return set_error(dm, 1, "a P2 node that is neither a vertex nor an edge midpoint");
+                }
+                g->p2_par.push_back(std::min(vidx[a], vidx[b]));
+                g->p2_par.push_back(std::max(vidx[a], vidx[b]));
+                g->p2_parw.push_back(0.5);
+                g->p2_parw.push_back(0.5);
+            }
+            g->p2_par_ptr[r + 1] = (int32_t)g->p2_par.size();
+        }
+    }
     g->xyz.resize(3 * (size_t)nn);
     for (int r = 0; r < nn; ++r)
         for (int d = 0; d < 3; ++d) g->xyz[3 * (size_t)r + d] = nx[3 * (size_t)perm[r] + d];
@@ -1397,9 +1452,17 @@ static int gf_vcycle1_matfree(plx_gf g, const float *r, float *z) {
     int err = g->fine_op(g->fine_ctx, x, t);
     if (err) return err;
     hipLaunchKernelGGL(amg_sub, gf_grid(g->ndof), dim3(256), 0, g->stream, g->ndof, r, (const float *)t, r1);
-    amg_bsr_launch<6, 3, 1, kBsrPlain, 64>(g, L.nc, L.Rptr, g->Rcol_dm, L.Rval, r1, nullptr, nullptr, nb, 0.f, 0);
-    amg_vcycle<1>(g, 1, nb, nx);
-    amg_bsr_launch<3, 6, 1, kBsrPlain, 8>(g, L.n, g->Pptr_dm, g->Pcol_dm, g->Pval_dm, nx, nullptr, nullptr, x, 0.f, 1);
+    if (L.bc == 3) {  // P2 -> P1 transfer
+        amg_bsr_launch<3, 3, 1, kBsrPlain, 16>(g, L.nc, L.Rptr, g->Rcol_dm, L.Rval, r1, nullptr, nullptr, nb, 0.f, 0);
+        amg_vcycle<1>(g, 1, nb, nx);
+        amg_bsr_launch<3, 3, 1, kBsrPlain, 2>(g, L.n, g->Pptr_dm, g->Pcol_dm, g->Pval_dm, nx, nullptr, nullptr, x, 0.f,
+                                              1);
+    } else {
+        amg_bsr_launch<6, 3, 1, kBsrPlain, 64>(g, L.nc, L.Rptr, g->Rcol_dm, L.Rval, r1, nullptr, nullptr, nb, 0.f, 0);
+        amg_vcycle<1>(g, 1, nb, nx);
+        amg_bsr_launch<3, 6, 1, kBsrPlain, 8>(g, L.n, g->Pptr_dm, g->Pcol_dm, g->Pval_dm, nx, nullptr, nullptr, x, 0.f,
+                                              1);
+    }
     if ((err = g->fine_op(g->fine_ctx, x, t))) return err;
     hipLaunchKernelGGL(amg_post0, gf_grid(L.n), dim3(256), 0, g->stream, L.n, (const float *)g->Dinv_dm, r,
                        (const float *)t, (const float *)x, z, s);

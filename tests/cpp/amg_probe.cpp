@@ -32,7 +32,7 @@ void spmm(int nr, int br, int bc, const int32_t *p, const int32_t *c, const doub
 void level_A(int l, const double *x, double *y) {
     const AmgLevel &L = H.levels[l];
     if (l == 0) spmm(L.n, 3, 3, A0p.data(), A0c.data(), A0v.data(), x, y, false);
-    else spmm(L.n, 6, 6, L.Aptr.data(), L.Acol.data(), L.Aval.data(), x, y, false);
+    else spmm(L.n, L.b, L.b, L.Aptr.data(), L.Acol.data(), L.Aval.data(), x, y, false);
 }
 
 void dinv(int l, double s, const double *r, double *z) {
@@ -60,13 +60,13 @@ void vcycle(int l, const double *bv, double *x) {
     const AmgLevel &L = H.levels[l];
     const size_t nd = (size_t)L.n * L.b;
     const double s = 2.0 / (L.lmax * (1.0 + 1.0 / 30.0));
-    std::vector<double> xl(nd), r(nd), t(nd), z(nd), cb((size_t)L.nc * 6), cx((size_t)L.nc * 6);
+    std::vector<double> xl(nd), r(nd), t(nd), z(nd), cb((size_t)L.nc * L.bc), cx((size_t)L.nc * L.bc);
     dinv(l, s, bv, xl.data());
     level_A(l, xl.data(), t.data());
     for (size_t i = 0; i < nd; ++i) r[i] = bv[i] - t[i];
-    spmm(L.nc, 6, L.b, L.Rptr.data(), L.Rcol.data(), L.Rval.data(), r.data(), cb.data(), false);
+    spmm(L.nc, L.bc, L.b, L.Rptr.data(), L.Rcol.data(), L.Rval.data(), r.data(), cb.data(), false);
     vcycle(l + 1, cb.data(), cx.data());
-    spmm(L.n, L.b, 6, L.Pptr.data(), L.Pcol.data(), L.Pval.data(), cx.data(), xl.data(), true);
+    spmm(L.n, L.b, L.bc, L.Pptr.data(), L.Pcol.data(), L.Pval.data(), cx.data(), xl.data(), true);
     level_A(l, xl.data(), t.data());
     for (size_t i = 0; i < nd; ++i) r[i] = bv[i] - t[i];
     dinv(l, s, r.data(), z.data());
@@ -92,6 +92,19 @@ int amg_probe_build(int n, const int32_t *ptr, const int32_t *col, const double 
 }
 
 void amg_probe_set_theta(double t) { theta = t; }
+
+// The P2 form (a P2 -> P1 first level, then the aggregation hierarchy of the P1 operator).
+int amg_probe_build_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed,
+                       int nv, const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz,
+                       int coarse_max, int max_levels) {
+    A0p.assign(ptr, ptr + n + 1);
+    A0c.assign(col, col + ptr[n]);
+    A0v.assign(val, val + (size_t)ptr[n] * 9);
+    if (plx::amg_setup_p2(n, ptr, col, val, fixed, nv, par_ptr, par, parw, vxyz, coarse_max, max_levels, &H,
+                          &last_error, theta))
+        return -1;
+    return (int)H.levels.size();
+}
 
 const char *amg_probe_error() { return last_error.c_str(); }
 
@@ -125,6 +138,7 @@ int amg_probe_array(int l, int which, void *dst) {
 
 int amg_probe_coarse(double *dst) { return copy_out(H.coarse_inv, dst); }
 int amg_probe_coarse_n() { return H.coarse_n; }
+int amg_probe_level_bc(int l) { return H.levels[l].bc; }
 
 void amg_probe_vcycle(const double *r, double *z) { vcycle(0, r, z); }
 

@@ -11,7 +11,10 @@ Pins (each fails for a plausible set-up mistake):
 * on a P2 tet box with Dirichlet faces that fix one component of a node (a diagonal block with
   an identity row beside stiffness rows ~1e14), the stored inverse blocks still invert the
   diagonal blocks, lambda_max stays O(1), and the V-cycle over strong couplings (theta 0.1)
-  cuts PCG iterations more than tenfold against point-block Jacobi.
+  cuts PCG iterations more than tenfold against point-block Jacobi;
+* the P2 form with a P2 -> P1 first level: vertex rows of P are identity blocks to their vertex
+  and edge rows half blocks to both ends (zero at fixed dofs), R = P^T, the P1 operator is the
+  Galerkin product P^T A P, and the V-cycle is SPD and cuts PCG iterations more than tenfold.
 """
 import ctypes
 import os
@@ -43,6 +46,9 @@ def probe():
     lib.amg_probe_vcycle.argtypes = [P, P]
     lib.amg_probe_error.restype = ctypes.c_char_p
     lib.amg_probe_set_theta.argtypes = [ctypes.c_double]
+    lib.amg_probe_build_p2.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_int, P, P, P, P, ctypes.c_int,
+                                       ctypes.c_int]
+    lib.amg_probe_level_bc.argtypes = [ctypes.c_int]
     return lib
 
 
@@ -322,6 +328,112 @@ def test_p2_operator_with_partly_fixed_nodes(probe, oracle):
 
     rng = np.random.default_rng(1)
     b = rng.standard_normal(3 * nn) * ~fixed
+    xa, ita = _pcg(A, M, b)
+    xj, itj = _pcg(A, lambda r: np.einsum("nij,nj->ni", Dinv, r.reshape(-1, 3)).ravel(), b)
+    assert np.linalg.norm(A @ xa - b) <= 1e-8 * np.linalg.norm(b)
+    assert ita * 10 <= itj, (ita, itj)
+
+
+
+def _p2_operator(oracle, n):
+    m = mg.tet_p2_box(n, (0, 0, 0), (6000.0, 6000.0, 6000.0), brick=2)
+    aux, sizes = mg.elastic_aux(m.num_cells, seed=4, vs_sigma=0.2, vp_vs=1.732)
+    dofs, _ = mg.shear_bcs(m.node_coords, 0.0, 6000.0)
+    P = oracle.Problem(m, aux, sizes)
+    nn = m.num_nodes
+    rows, cols, vals = [], [], []
+    for c in range(m.num_cells):
+        d = (3 * m.closure[c][:, None] + np.arange(3)).ravel()
+        rows.append(np.repeat(d, d.size))
+        cols.append(np.tile(d, d.size))
+        vals.append(-P.element_matrix(c, JF3).ravel())
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(3 * nn, 3 * nn))
+    fixed = np.zeros(3 * nn, bool)
+    fixed[dofs] = True
+    keep = sp.diags((~fixed).astype(float))
+    A = (keep @ A @ keep + sp.diags(fixed.astype(float))).tocsr()
+    return m, A, fixed
+
+
+def test_p2_to_p1_first_level(probe, oracle):
+    """Runs last: it rebuilds the probe's hierarchy."""
+    m, A, fixed = _p2_operator(oracle, 6)
+    nn, nv = m.num_nodes, m.num_vertices
+    par = [None] * nn
+    edges = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    for c in range(m.num_cells):
+        cl, cn = m.closure[c], m.cone[c]
+        for b in range(4):
+            par[cl[b]] = ([int(cn[b])], [1.0])
+        for e, (a, b) in enumerate(edges):
+            par[cl[4 + e]] = (sorted([int(cn[a]), int(cn[b])]), [0.5, 0.5])
+    par_ptr = np.zeros(nn + 1, np.int32)
+    for i in range(nn):
+        par_ptr[i + 1] = par_ptr[i] + len(par[i][0])
+    pidx = np.concatenate([np.array(p[0], np.int32) for p in par])
+    pw = np.concatenate([np.array(p[1], float) for p in par])
+    Ab = sp.bsr_matrix(A, blocksize=(3, 3))
+    Ab.sort_indices()
+    ptr, col = Ab.indptr.astype(np.int32), Ab.indices.astype(np.int32)
+    val = np.ascontiguousarray(Ab.data, np.float64)
+    fx = fixed.astype(np.uint8)
+    vxyz = np.ascontiguousarray(m.coords, np.float64)
+    probe.amg_probe_set_theta(0.05)
+    try:
+        nl = probe.amg_probe_build_p2(nn, ptr.ctypes.data, col.ctypes.data, val.ctypes.data, fx.ctypes.data, nv,
+                                      par_ptr.ctypes.data, pidx.ctypes.data, pw.ctypes.data, vxyz.ctypes.data, 300, 8)
+    finally:
+        probe.amg_probe_set_theta(0.0)
+    assert nl >= 2, probe.amg_probe_error()
+    assert probe.amg_probe_level_bc(0) == 3 and probe.amg_probe_level_bc(1) == 6
+
+    def arr(l, which, dtype):
+        cnt = probe.amg_probe_array(l, which, None)
+        out = np.zeros(cnt, dtype)
+        probe.amg_probe_array(l, which, out.ctypes.data)
+        return out
+
+    Pm = sp.bsr_matrix((arr(0, 5, np.float64).reshape(-1, 3, 3), arr(0, 4, np.int32), arr(0, 3, np.int32)),
+                       shape=(3 * nn, 3 * nv)).tocsr()
+    Rm = sp.bsr_matrix((arr(0, 8, np.float64).reshape(-1, 3, 3), arr(0, 7, np.int32), arr(0, 6, np.int32)),
+                       shape=(3 * nv, 3 * nn)).tocsr()
+    assert abs(Rm - Pm.T).max() == 0.0
+    # the P1 interpolation, zero at fixed dofs (fine or coarse)
+    fixed1 = np.zeros(3 * nv, bool)
+    for i in range(nn):
+        if len(par[i][0]) == 1:
+            fixed1[3 * par[i][0][0]:3 * par[i][0][0] + 3] |= fixed[3 * i:3 * i + 3]
+    Pd = np.zeros((3 * nn, 3 * nv))
+    for i in range(nn):
+        for I, w in zip(*par[i]):
+            for c in range(3):
+                if not fixed[3 * i + c] and not fixed1[3 * I + c]:
+                    Pd[3 * i + c, 3 * I + c] = w
+    assert np.abs(Pm.toarray() - Pd).max() == 0.0
+    # the P1 operator: Galerkin product, identity rows / zero columns at fixed vertex dofs
+    A1 = sp.bsr_matrix((arr(1, 2, np.float64).reshape(-1, 3, 3), arr(1, 1, np.int32), arr(1, 0, np.int32)),
+                       shape=(3 * nv, 3 * nv)).toarray()
+    G = Pd.T @ A.toarray() @ Pd + np.diag(fixed1.astype(float))
+    assert np.abs(A1 - G).max() <= 1e-12 * np.abs(G).max()
+
+    def M(r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        probe.amg_probe_vcycle(r.ctypes.data, z.ctypes.data)
+        return z
+
+    rng = np.random.default_rng(2)
+    n = A.shape[0]
+    u, v = rng.standard_normal(n) * ~fixed, rng.standard_normal(n) * ~fixed
+    assert abs(u @ M(v) - v @ M(u)) <= 1e-11 * abs(u @ M(u))
+    assert all((w @ M(w)) > 0.0 for w in (rng.standard_normal(n) * ~fixed for _ in range(3)))
+    b = rng.standard_normal(n) * ~fixed
+    D = np.zeros((nn, 3, 3))
+    for i in range(nn):
+        for k in range(Ab.indptr[i], Ab.indptr[i + 1]):
+            if Ab.indices[k] == i:
+                D[i] = Ab.data[k]
+    Dinv = np.linalg.inv(D)
     xa, ita = _pcg(A, M, b)
     xj, itj = _pcg(A, lambda r: np.einsum("nij,nj->ni", Dinv, r.reshape(-1, 3)).ravel(), b)
     assert np.linalg.norm(A @ xa - b) <= 1e-8 * np.linalg.norm(b)
