@@ -1270,8 +1270,9 @@ static int gf_setup_assembled(plx_gf g, const std::vector<int32_t> &row_ptr, con
 // zero displacement), negated into the positive operator -J of the Q1 path: the path for P2
 // tetrahedra, whose element matrices the Q1 kernels above do not form.  Rows are the mesh
 // nodes in Morton order of their coordinates (an edge node sits at its edge's midpoint); a
-// constrained dof gets an identity row and a zero column.  The handle serves products and the
-// multigrid preconditioner; fault impulses need the Q1 handle.
+// constrained dof gets an identity row and a zero column.  The handle also records each row's
+// P1 parents (its vertex, or its edge's two ends) for the multigrid's P2 -> P1 first level.  It
+// serves products and the multigrid preconditioner; fault impulses need the Q1 handle.
 static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan *> &keys) {
     if (dm->fault) return set_error(dm, 1, "Green's functions with a fault are implemented for Q1 hexes");
     const int nn = dm->num_nodes, nb = dm->nb, nv = dm->nv;
