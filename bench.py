@@ -673,7 +673,8 @@ def main():
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic = tj.get("p2_fp64_residual_64cube")
-        traffic_build = tj.get("_build")
+        builds = tj.get("_build")
+        traffic_build = builds.get("p2_fp64_residual_64cube") if isinstance(builds, dict) else builds
     result = {
         "metric": "residual-apply GDoF/s (% HBM roofline)",
         "value": ndofs / sec / 1e9,
