@@ -743,7 +743,7 @@ struct plx_gf_s {
     // host copy of the row-ordered operator and its identity-row dofs, kept by the assembled
     // (P2) path for the multigrid set-up (which frees it) instead of a download
     std::vector<int32_t> h_ptr, h_col;
-    std::vector<double> h_val;
+    AmgVal h_val;  // default-initialised: filled by a download
     std::vector<uint8_t> h_fixed;
     // P2 handles: the P1 interpolation onto the vertices that the multigrid's first level takes
     // (parents of each row in vertex numbering, weights 1 or 1/2; vertex coordinates)
@@ -1123,7 +1123,7 @@ static int gf_amg_setup(plx_gf g) {
     plx_dm dm = g->dm;
     const int nn = g->nrows;
     std::vector<int32_t> ptr, col;
-    std::vector<double> val;
+    AmgVal val;
     std::vector<uint8_t> fixed;
     if (!g->h_val.empty()) {  // the assembled path's host copy
         ptr.swap(g->h_ptr);
@@ -1237,23 +1237,37 @@ int gf_setup(plx_gf g, int nc, const std::vector<int32_t> &row_ptr, const std::v
              const std::vector<uint8_t> &conmask, const std::vector<uint8_t> &is_pos);
 int gf_alloc_work(plx_gf g);
 
-// Handle over an operator assembled on the host side (row_ptr / col / val in row order, diagonal
-// block of row i at diag_pos[i]): upload, inverse diagonal blocks, FP32 copies, workspace.
-static int gf_setup_assembled(plx_gf g, const std::vector<int32_t> &row_ptr, const std::vector<int32_t> &col,
-                              const std::vector<int32_t> &diag_pos, const std::vector<double> &val) {
+// Rows of the assembled operator in Morton order: block o of row r is block src[o] of the DM-order
+// operator val0, negated into -J; a constrained dof's row and column become identity / zero
+// (con: 1 per constrained dof, DM order).  One wavefront per row, lanes over its 9 x blocks values.
+__global__ void __launch_bounds__(256) gf_permute_blocks(int nn, const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ col,
+                                                         const int32_t *__restrict__ perm,
+                                                         const int32_t *__restrict__ src,
+                                                         const uint8_t *__restrict__ con,
+                                                         const double *__restrict__ val0, double *val) {
+    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (r >= nn) return;
+    const int lane = threadIdx.x & 63;
+    const int i = perm[r];
+    const int o0 = rp[r], nv = 9 * (rp[r + 1] - o0);
+    for (int v = lane; v < nv; v += 64) {
+        const int q = v / 9, fh = v - 9 * q, f = fh / 3, h = fh - 3 * f;
+        const size_t o = (size_t)o0 + q;
+        const int c = col[o];
+        const int j = perm[c];
+        double x = -val0[9 * (size_t)src[o] + fh];
+        if (con[3 * (size_t)i + f] || con[3 * (size_t)j + h]) x = (c == r && f == h) ? 1.0 : 0.0;
+        val[9 * o + fh] = x;
+    }
+}
+
+// Handle over an operator whose Morton-order rows are already on the device (d_row_ptr, d_col,
+// d_perm, d_val64; diagonal block of row i at d_diag[i]): inverse diagonal blocks, FP32 copies,
+// workspace.
+static int gf_setup_assembled(plx_gf g, int32_t *d_diag) {
     plx_dm dm = g->dm;
     const int nn = g->nrows;
-    auto up = [&](auto **d, const auto &h) -> int {
-        using T = typename std::decay<decltype(h)>::type::value_type;
-        PLX_HIP(dm, hipMalloc((void **)d, std::max<size_t>(1, h.size()) * sizeof(T)));
-        PLX_HIP(dm, hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-        return 0;
-    };
-    int32_t *d_diag = nullptr;
-    int err = 0;
-    if ((err = up(&g->d_row_ptr, row_ptr)) || (err = up(&g->d_col, col)) || (err = up(&g->d_perm, g->perm)) ||
-        (err = up(&g->d_val64, val)) || (err = up(&d_diag, diag_pos)))
-        return err;
     PLX_HIP(dm, hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     PLX_HIP(dm, hipMalloc(&g->d_val32, (size_t)g->nnzb * 9 * sizeof(float)));
     PLX_HIP(dm, hipMalloc(&g->d_Binv64, 9 * (size_t)nn * sizeof(double)));
@@ -1262,7 +1276,6 @@ static int gf_setup_assembled(plx_gf g, const std::vector<int32_t> &row_ptr, con
                        g->d_Binv64, g->d_Binv32, g->d_val32, g->nnzb);
     PLX_HIP(dm, hipGetLastError());
     PLX_HIP(dm, hipStreamSynchronize(g->stream));
-    (void)hipFree(d_diag);
     return gf_alloc_work(g);
 }
 
@@ -1277,27 +1290,28 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
     if (dm->fault) return set_error(dm, 1, "Green's functions with a fault are implemented for Q1 hexes");
     const int nn = dm->num_nodes, nb = dm->nb, nv = dm->nv;
     int64_t nnzb = 0;
-    int err = plx_jacobian_pattern(dm, &nnzb, nullptr, nullptr);
+    int err = plx_jacobian_pattern(dm, &nnzb, nullptr, nullptr);  // builds the DM's pattern once
     if (err) return err;
-    std::vector<int32_t> rp0(nn + 1), col0((size_t)nnzb);
-    if ((err = plx_jacobian_pattern(dm, &nnzb, rp0.data(), col0.data()))) return err;
-    std::vector<double> val0((size_t)nnzb * 9);
-    {
-        double *d_val = nullptr, *d_x = nullptr;
-        if (hipMalloc(&d_val, val0.size() * sizeof(double)) != hipSuccess ||
-            hipMalloc(&d_x, std::max<int64_t>(1, dm->local_size) * sizeof(double)) != hipSuccess) {
-            (void)hipFree(d_val);
-            return set_error(dm, 1, "out of device memory for the assembled operator");
+    const int32_t *rp0 = dm->jac_row_ptr.data(), *col0 = dm->jac_col.data();
+    // device buffers of the set-up, freed on every exit
+    struct DevBuf {
+        void *p = nullptr;
+        ~DevBuf() {
+            if (p) (void)hipFree(p);
         }
+    } val0_buf, src_buf, con_buf, diag_buf;
+    {  // the operator in DM order, assembled on the device and kept there
+        double *d_x = nullptr;
+        if (hipMalloc(&val0_buf.p, (size_t)nnzb * 9 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&d_x, std::max<int64_t>(1, dm->local_size) * sizeof(double)) != hipSuccess)
+            return set_error(dm, 1, "out of device memory for the assembled operator");
         if (hipMemsetAsync(d_x, 0, dm->local_size * sizeof(double), dm->stream) != hipSuccess)
             err = set_error(dm, 76, "hipMemsetAsync failed");
         for (size_t k = 0; k < keys.size() && !err; ++k)
-            err = plx_compute_jacobian(dm, keys[k]->label_value, keys[k]->part, 0.0, 0.0, d_x, nullptr, d_val,
-                                       k == 0 ? PLX_INSERT_VALUES : PLX_ADD_VALUES);
-        if (!err && (hipStreamSynchronize(dm->stream) != hipSuccess ||
-                     hipMemcpy(val0.data(), d_val, val0.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
-            err = set_error(dm, 76, "downloading the assembled operator failed");
-        (void)hipFree(d_val);
+            err = plx_compute_jacobian(dm, keys[k]->label_value, keys[k]->part, 0.0, 0.0, d_x, nullptr,
+                                       static_cast<double *>(val0_buf.p), k == 0 ? PLX_INSERT_VALUES : PLX_ADD_VALUES);
+        if (!err && hipStreamSynchronize(dm->stream) != hipSuccess)
+            err = set_error(dm, 76, "assembling the operator failed");
         (void)hipFree(d_x);
         if (err) return err;
     }
@@ -1328,9 +1342,9 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
         if (d < 0 || d >= 3 * nn) return set_error(dm, 1, "constrained dof out of range");
         conmask[d] = 1;
     }
-    // rows in Morton order, columns renumbered and ascending
-    std::vector<int32_t> row_ptr(nn + 1, 0), col((size_t)nnzb), diag_pos(nn, -1);
-    std::vector<double> val((size_t)nnzb * 9);
+    // rows in Morton order, columns renumbered and ascending; src: each block's position in the
+    // DM-order operator (the values move on the device)
+    std::vector<int32_t> row_ptr(nn + 1, 0), col((size_t)nnzb), diag_pos(nn, -1), src((size_t)nnzb);
     for (int r = 0; r < nn; ++r) row_ptr[r + 1] = row_ptr[r] + (rp0[perm[r] + 1] - rp0[perm[r]]);
     {  // rows are independent: chunks of them on host threads
         const int T = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1346,16 +1360,9 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
                     std::sort(rowv.begin(), rowv.end());
                     int64_t o = row_ptr[r];
                     for (const auto &ce : rowv) {
-                        const int j = perm[ce.first];
                         col[o] = ce.first;
+                        src[o] = ce.second;
                         if (ce.first == r) diag_pos[r] = (int32_t)o;
-                        for (int f = 0; f < 3; ++f)
-                            for (int h = 0; h < 3; ++h) {
-                                double v = -val0[9 * (size_t)ce.second + 3 * f + h];  // the operator -J
-                                if (conmask[3 * (size_t)i + f] || conmask[3 * (size_t)j + h])
-                                    v = (i == j && f == h) ? 1.0 : 0.0;
-                                val[9 * (size_t)o + 3 * f + h] = v;
-                            }
                         ++o;
                     }
                 }
@@ -1364,7 +1371,6 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
     }
     for (int r = 0; r < nn; ++r)
         if (diag_pos[r] < 0) return set_error(dm, 1, "assembled operator without a diagonal block");
-    std::vector<double>().swap(val0);
     plx_gf g = new plx_gf_s();
     g->dm = dm;
     g->kp = keys[0];
@@ -1415,16 +1421,43 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
     for (int r = 0; r < nn; ++r)
         for (int d = 0; d < 3; ++d) g->xyz[3 * (size_t)r + d] = nx[3 * (size_t)perm[r] + d];
     g->perm = std::move(perm);
-    if ((err = gf_setup_assembled(g, row_ptr, col, diag_pos, val))) {
+    // upload the pattern, move the blocks into Morton order on the device, then keep a host copy
+    // of the result for the multigrid set-up
+    auto up = [&](void **d, const void *h, size_t bytes) -> int {
+        PLX_HIP(dm, hipMalloc(d, std::max<size_t>(1, bytes)));
+        PLX_HIP(dm, hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    auto fail = [&](int e) {
         plx_gf_destroy(g);
-        return err;
-    }
+        return e;
+    };
+    if ((err = up((void **)&g->d_row_ptr, row_ptr.data(), row_ptr.size() * sizeof(int32_t))) ||
+        (err = up((void **)&g->d_col, col.data(), col.size() * sizeof(int32_t))) ||
+        (err = up((void **)&g->d_perm, g->perm.data(), g->perm.size() * sizeof(int32_t))) ||
+        (err = up(&diag_buf.p, diag_pos.data(), diag_pos.size() * sizeof(int32_t))) ||
+        (err = up(&src_buf.p, src.data(), src.size() * sizeof(int32_t))) ||
+        (err = up(&con_buf.p, conmask.data(), conmask.size())))
+        return fail(err);
+    std::vector<int32_t>().swap(src);
+    if (hipMalloc(&g->d_val64, (size_t)nnzb * 9 * sizeof(double)) != hipSuccess)
+        return fail(set_error(dm, 1, "out of device memory for the assembled operator"));
+    hipLaunchKernelGGL(gf_permute_blocks, dim3((unsigned)((nn + 3) / 4)), dim3(256), 0, dm->stream, nn, g->d_row_ptr,
+                       g->d_col, g->d_perm, static_cast<const int32_t *>(src_buf.p),
+                       static_cast<const uint8_t *>(con_buf.p), static_cast<const double *>(val0_buf.p), g->d_val64);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(dm->stream) != hipSuccess)
+        return fail(set_error(dm, 76, "reordering the assembled operator failed"));
+    (void)hipFree(val0_buf.p);
+    val0_buf.p = nullptr;
+    if ((err = gf_setup_assembled(g, static_cast<int32_t *>(diag_buf.p)))) return fail(err);
+    g->h_val.resize((size_t)nnzb * 9);
+    if (hipMemcpy(g->h_val.data(), g->d_val64, g->h_val.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(set_error(dm, 76, "downloading the assembled operator failed"));
     g->h_fixed.resize(3 * (size_t)nn);
     for (int r = 0; r < nn; ++r)
         for (int c = 0; c < 3; ++c) g->h_fixed[3 * (size_t)r + c] = conmask[3 * (size_t)g->perm[r] + c];
     g->h_ptr = std::move(row_ptr);
     g->h_col = std::move(col);
-    g->h_val = std::move(val);
     *out = g;
     return 0;
 }

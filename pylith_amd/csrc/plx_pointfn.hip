@@ -4,6 +4,7 @@
 // Jacobian (DMPlexComputeJacobianByKey, IntegratorDomain.cc:489-490).
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "../../include/pylith_amd_pointfn.hpp"
 #include "plx_internal.hpp"
@@ -390,18 +391,41 @@ static int ensure_pattern(plx_dm dm) {
         for (int c = 0; c < dm->num_cells; ++c)
             for (int b = 0; b < nb; ++b) inc[pos[dm->closure[(size_t)c * nb + b]]++] = c;
     }
-    dm->jac_row_ptr.assign(nn + 1, 0);
-    dm->jac_col.clear();
-    std::vector<int32_t> row;
-    for (int i = 0; i < nn; ++i) {
-        row.clear();
-        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
-            for (int b = 0; b < nb; ++b) row.push_back(dm->closure[(size_t)inc[k] * nb + b]);
-        std::sort(row.begin(), row.end());
-        row.erase(std::unique(row.begin(), row.end()), row.end());
-        dm->jac_col.insert(dm->jac_col.end(), row.begin(), row.end());
-        if (dm->jac_col.size() > (size_t)INT32_MAX) return set_error(dm, 1, "Jacobian pattern too large");
-        dm->jac_row_ptr[i + 1] = (int32_t)dm->jac_col.size();
+    // rows are independent: each host thread sorts a contiguous range of rows into its own
+    // list, and the lists are joined in row order
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::vector<int32_t>> part(T);
+    std::vector<int64_t> len(nn + 1, 0);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                const int lo = (int)((int64_t)nn * t / T), hi = (int)((int64_t)nn * (t + 1) / T);
+                std::vector<int32_t> row;
+                for (int i = lo; i < hi; ++i) {
+                    row.clear();
+                    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+                        for (int b = 0; b < nb; ++b) row.push_back(dm->closure[(size_t)inc[k] * nb + b]);
+                    std::sort(row.begin(), row.end());
+                    row.erase(std::unique(row.begin(), row.end()), row.end());
+                    part[t].insert(part[t].end(), row.begin(), row.end());
+                    len[i + 1] = (int64_t)row.size();
+                }
+            });
+        for (auto &x : th) x.join();
+    }
+    for (int i = 0; i < nn; ++i) len[i + 1] += len[i];
+    if (len[nn] > (int64_t)INT32_MAX) return set_error(dm, 1, "Jacobian pattern too large");
+    dm->jac_row_ptr.resize(nn + 1);
+    for (int i = 0; i <= nn; ++i) dm->jac_row_ptr[i] = (int32_t)len[i];
+    dm->jac_col.resize((size_t)len[nn]);
+    {
+        size_t at = 0;
+        for (int t = 0; t < T; ++t) {
+            std::copy(part[t].begin(), part[t].end(), dm->jac_col.begin() + at);
+            at += part[t].size();
+            std::vector<int32_t>().swap(part[t]);
+        }
     }
     PLX_HIP(dm, hipMalloc(&dm->d_jac_row_ptr, dm->jac_row_ptr.size() * sizeof(int32_t)));
     PLX_HIP(dm, hipMalloc(&dm->d_jac_col, std::max<size_t>(1, dm->jac_col.size()) * sizeof(int32_t)));
