@@ -122,11 +122,8 @@ bool level_dinv(int n, int b, const int32_t *Ap, const int32_t *Ac, const double
 // 1.1 x the largest eigenvalue of D^-1 A, 12 power iterations from a fixed random vector.
 double level_lmax(int n, int b, const int32_t *Ap, const int32_t *Ac, const double *Av, const AmgVal &Dinv) {
     const int T = num_threads(), bb = b * b;
-    std::mt19937_64 rng(7);
-    std::normal_distribution<double> nd;
     const size_t nd_ = (size_t)n * b;
-    std::vector<double> v(nd_), w(nd_);
-    for (auto &x : v) x = nd(rng);
+    std::vector<double> v = lmax_start_vector(nd_), w(nd_);
     double lam = 0.0;
     std::vector<double> part(T);
     double vn = 0.0;
@@ -249,6 +246,14 @@ void spgemm_A_M(int n, int b, const int32_t *Aptr, const int32_t *Acol, const do
 }
 
 }  // namespace
+
+std::vector<double> lmax_start_vector(size_t n) {
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd;
+    std::vector<double> v(n);
+    for (auto &x : v) x = nd(rng);
+    return v;
+}
 
 int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *val0, const double *coords,
               const uint8_t *fixed0, int coarse_max, int max_levels, AmgHierarchy *h, std::string *err,
@@ -609,7 +614,7 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
 
 int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed, int nv,
                  const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz, int coarse_max,
-                 int max_levels, AmgHierarchy *h, std::string *err, double theta) {
+                 int max_levels, AmgHierarchy *h, std::string *err, double theta, double lmax0) {
     h->levels.clear();
     h->coarse_inv.clear();
     if (max_levels < 2) {
@@ -625,7 +630,7 @@ int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *va
         *err = "amg: missing diagonal block";
         return 1;
     }
-    L0.lmax = level_lmax(n, 3, ptr, col, val, L0.Dinv);
+    L0.lmax = lmax0 > 0.0 ? lmax0 : level_lmax(n, 3, ptr, col, val, L0.Dinv);
     // fixed vertex dofs: those of the vertex's own node (the node with that single parent)
     std::vector<uint8_t> fixed1(3 * (size_t)nv, 0);
     for (int i = 0; i < n; ++i) {

@@ -75,9 +75,14 @@ int amg_setup(int n, const int32_t *ptr, const int32_t *col, const double *val, 
 // interpolation (a vertex node takes its vertex's value, an edge node the mean of its two
 // vertices': parents par[par_ptr[i] ..] with weights parw), the Galerkin product gives the P1
 // operator, and amg_setup builds the rest of the hierarchy on it (vertex coordinates vxyz
-// [nv][3], threshold theta).  A vertex dof is fixed when its own node's dof is.
+// [nv][3], threshold theta).  A vertex dof is fixed when its own node's dof is.  lmax0 > 0: the
+// P2 level's lambda_max estimate of D^-1 A, computed by the caller (else by power iteration
+// here, as level_lmax does for every level).
 int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed, int nv,
                  const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz, int coarse_max,
-                 int max_levels, AmgHierarchy *h, std::string *err, double theta = 0.0);
+                 int max_levels, AmgHierarchy *h, std::string *err, double theta = 0.0, double lmax0 = 0.0);
+
+// Start vector of the lambda_max power iterations (n * b entries; the same on every host).
+std::vector<double> lmax_start_vector(size_t n);
 
 }  // namespace plx

@@ -165,6 +165,31 @@ __global__ void __launch_bounds__(256) gf_spmm(int nrows, int k, const int32_t *
     Y[(3 * (size_t)i + 2) * k + r] = y2;
 }
 
+// One step of the lambda_max power iteration of D^-1 A: v = Binv y * inv_vn (y = A v before), and
+// per-block partial sums of |Binv y|^2 (thread = row node).
+__global__ void __launch_bounds__(256) gf_binv_step(int nrows, const double *__restrict__ Binv,
+                                                    const double *__restrict__ y, double inv_vn, double *v,
+                                                    double *part) {
+    __shared__ double s[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < nrows) {
+        const double *B = Binv + 9 * (size_t)i;
+        const double *yi = y + 3 * (size_t)i;
+        for (int r = 0; r < 3; ++r) {
+            const double z = B[3 * r] * yi[0] + B[3 * r + 1] * yi[1] + B[3 * r + 2] * yi[2];
+            v[3 * (size_t)i + r] = z * inv_vn;
+            acc += z * z;
+        }
+    }
+    s[threadIdx.x] = acc;
+    for (int w = 128; w > 0; w >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
 // Per-column block reduction (threads with equal t % k share a column; k divides 256).
 __device__ __forceinline__ void col_reduce(double *s, int k) {
     for (int w = 128; w >= k; w >>= 1) {
@@ -1119,6 +1144,48 @@ static void gf_amg_free(plx_gf g) {
 
 // Build the multigrid hierarchy of the merged FP64 operator on the host and upload it in FP32
 // with workspace for 64 columns.
+// lambda_max estimate of D^-1 A on the device (A: the handle's FP64 operator, D^-1: its inverse
+// diagonal blocks): twelve power iterations from the host set-up's start vector, times 1.1 (the
+// host's level_lmax on the device).
+static int gf_lmax_device(plx_gf g, double *out) {
+    plx_dm dm = g->dm;
+    const int n = g->nrows;
+    const int nblk = (n + 255) / 256;
+    const size_t nd = 3 * (size_t)n;
+    std::vector<double> hv = lmax_start_vector(nd), part(nblk);
+    double vn = 0.0;
+    for (double x : hv) vn += x * x;
+    vn = std::sqrt(vn);
+    double *v = nullptr, *y = nullptr, *dpart = nullptr;
+    int err = 0;
+    if (hipMalloc(&v, nd * sizeof(double)) != hipSuccess || hipMalloc(&y, nd * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dpart, nblk * sizeof(double)) != hipSuccess ||
+        hipMemcpy(v, hv.data(), nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        err = set_error(dm, 76, "lambda_max estimate: device buffers");
+    double lam = 0.0;
+    for (int it = 0; it < 12 && !err; ++it) {
+        hipLaunchKernelGGL(gf_spmm<double>, dim3(nblk), dim3(256), 0, g->stream, n, 1, g->d_row_ptr, g->d_col,
+                           g->d_val64, v, y);
+        hipLaunchKernelGGL(gf_binv_step, dim3(nblk), dim3(256), 0, g->stream, n, g->d_Binv64, y, 1.0 / vn, v, dpart);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(part.data(), dpart, nblk * sizeof(double), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess) {
+            err = set_error(dm, 76, "lambda_max estimate failed");
+            break;
+        }
+        double acc = 0.0;
+        for (double x : part) acc += x;
+        const double wn = std::sqrt(acc) / vn;  // || D^-1 A v || / || v ||
+        lam = wn;
+        vn = wn;
+    }
+    (void)hipFree(v);
+    (void)hipFree(y);
+    (void)hipFree(dpart);
+    if (!err) *out = 1.1 * lam;
+    return err;
+}
+
 static int gf_amg_setup(plx_gf g) {
     plx_dm dm = g->dm;
     const int nn = g->nrows;
@@ -1158,10 +1225,15 @@ static int gf_amg_setup(plx_gf g) {
     AmgHierarchy h;
     std::string msg;
     if (g->xyz.size() != 3 * (size_t)nn) return set_error(dm, 1, "node coordinates missing");
+    double lmax0 = 0.0;  // the P2 level's estimate, from the device operator
+    if (g->p2_nv > 0) {
+        const int lerr = gf_lmax_device(g, &lmax0);
+        if (lerr) return lerr;
+    }
     const int aerr = g->p2_nv > 0
                          ? amg_setup_p2(nn, ptr.data(), col.data(), val.data(), fixed.data(), g->p2_nv,
                                         g->p2_par_ptr.data(), g->p2_par.data(), g->p2_parw.data(),
-                                        g->p2_vxyz.data(), 768, 8, &h, &msg, g->amg_theta)
+                                        g->p2_vxyz.data(), 768, 8, &h, &msg, g->amg_theta, lmax0)
                          : amg_setup(nn, ptr.data(), col.data(), val.data(), g->xyz.data(), fixed.data(), 768, 8,
                                      &h, &msg, g->amg_theta);
     if (aerr) return set_error(dm, 1, msg.c_str());
