@@ -100,7 +100,8 @@ inline void gemm_t6(const double *__restrict__ P, const double *__restrict__ X, 
 }
 
 // Inverse diagonal blocks of a b x b block operator; false when a row has no diagonal block.
-bool level_dinv(int n, int b, const int32_t *Ap, const int32_t *Ac, const double *Av, AmgVal *Dinv) {
+template <typename V>
+bool level_dinv(int n, int b, const int32_t *Ap, const int32_t *Ac, const V *Av, AmgVal *Dinv) {
     const int T = num_threads(), bb = b * b;
     Dinv->assign((size_t)n * bb, 0.0);
     std::vector<uint8_t> no_diag(T, 0);
@@ -113,14 +114,17 @@ bool level_dinv(int n, int b, const int32_t *Ap, const int32_t *Ac, const double
                 no_diag[t] = 1;
                 return;
             }
-            block_inverse(Av + (size_t)d * bb, b, Dinv->data() + (size_t)i * bb);
+            double blk[36];
+            for (int k = 0; k < bb; ++k) blk[k] = (double)Av[(size_t)d * bb + k];
+            block_inverse(blk, b, Dinv->data() + (size_t)i * bb);
         }
     });
     return std::find(no_diag.begin(), no_diag.end(), 1) == no_diag.end();
 }
 
 // 1.1 x the largest eigenvalue of D^-1 A, 12 power iterations from a fixed random vector.
-double level_lmax(int n, int b, const int32_t *Ap, const int32_t *Ac, const double *Av, const AmgVal &Dinv) {
+template <typename V>
+double level_lmax(int n, int b, const int32_t *Ap, const int32_t *Ac, const V *Av, const AmgVal &Dinv) {
     const int T = num_threads(), bb = b * b;
     const size_t nd_ = (size_t)n * b;
     std::vector<double> v = lmax_start_vector(nd_), w(nd_);
@@ -136,7 +140,7 @@ double level_lmax(int n, int b, const int32_t *Ap, const int32_t *Ac, const doub
             for (int i = lo; i < hi; ++i) {
                 double y[6] = {0, 0, 0, 0, 0, 0};
                 for (int e = Ap[i]; e < Ap[i + 1]; ++e) {
-                    const double *A = &Av[(size_t)e * bb];
+                    const V *A = &Av[(size_t)e * bb];
                     const double *x = &v[(size_t)Ac[e] * b];
                     for (int r = 0; r < b; ++r) {
                         double sacc = 0.0;
@@ -612,7 +616,8 @@ int amg_setup(int n0, const int32_t *ptr0, const int32_t *col0, const double *va
     return 0;
 }
 
-int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed, int nv,
+template <typename V>
+int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const V *val, const uint8_t *fixed, int nv,
                  const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz, int coarse_max,
                  int max_levels, AmgHierarchy *h, std::string *err, double theta, double lmax0) {
     h->levels.clear();
@@ -693,7 +698,7 @@ int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *va
                 const double *pi = &L0.Pval[(size_t)rsrc[u] * 9];
                 for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
                     const int j = col[e];
-                    const double *A = val + (size_t)e * 9;
+                    const V *A = val + (size_t)e * 9;
                     for (int k = par_ptr[j]; k < par_ptr[j + 1]; ++k) {
                         const int J = L0.Pcol[k];
                         const double *pj = &L0.Pval[(size_t)k * 9];
@@ -738,5 +743,12 @@ int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *va
     h->coarse_inv.swap(h1.coarse_inv);
     return 0;
 }
+
+template int amg_setup_p2<double>(int, const int32_t *, const int32_t *, const double *, const uint8_t *, int,
+                                  const int32_t *, const int32_t *, const double *, const double *, int, int,
+                                  AmgHierarchy *, std::string *, double, double);
+template int amg_setup_p2<float>(int, const int32_t *, const int32_t *, const float *, const uint8_t *, int,
+                                 const int32_t *, const int32_t *, const double *, const double *, int, int,
+                                 AmgHierarchy *, std::string *, double, double);
 
 }  // namespace plx

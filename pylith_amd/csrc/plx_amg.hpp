@@ -40,6 +40,7 @@ struct DefaultInitAlloc : std::allocator<T> {
 };
 using AmgIdx = std::vector<int32_t, DefaultInitAlloc<int32_t>>;
 using AmgVal = std::vector<double, DefaultInitAlloc<double>>;
+using AmgValF = std::vector<float, DefaultInitAlloc<float>>;
 
 struct AmgLevel {
     int n = 0;       // block rows
@@ -78,7 +79,8 @@ int amg_setup(int n, const int32_t *ptr, const int32_t *col, const double *val, 
 // [nv][3], threshold theta).  A vertex dof is fixed when its own node's dof is.  lmax0 > 0: the
 // P2 level's lambda_max estimate of D^-1 A, computed by the caller (else by power iteration
 // here, as level_lmax does for every level).
-int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const double *val, const uint8_t *fixed, int nv,
+template <typename V>  // operator values: double or float (instantiated for both)
+int amg_setup_p2(int n, const int32_t *ptr, const int32_t *col, const V *val, const uint8_t *fixed, int nv,
                  const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz, int coarse_max,
                  int max_levels, AmgHierarchy *h, std::string *err, double theta = 0.0, double lmax0 = 0.0);
 

@@ -768,7 +768,7 @@ struct plx_gf_s {
     // host copy of the row-ordered operator and its identity-row dofs, kept by the assembled
     // (P2) path for the multigrid set-up (which frees it) instead of a download
     std::vector<int32_t> h_ptr, h_col;
-    AmgVal h_val;  // default-initialised: filled by a download
+    AmgValF h_val;  // FP32 copy for the multigrid set-up (default-initialised: filled by a download)
     std::vector<uint8_t> h_fixed;
     // P2 handles: the P1 interpolation onto the vertices that the multigrid's first level takes
     // (parents of each row in vertex numbering, weights 1 or 1/2; vertex coordinates)
@@ -1190,12 +1190,14 @@ static int gf_amg_setup(plx_gf g) {
     plx_dm dm = g->dm;
     const int nn = g->nrows;
     std::vector<int32_t> ptr, col;
-    AmgVal val;
+    AmgVal val;    // FP64 operator (downloaded here: Q1 handles)
+    AmgValF valf;  // FP32 operator (the assembled P2 handle's host copy)
     std::vector<uint8_t> fixed;
     if (!g->h_val.empty()) {  // the assembled path's host copy
+        if (g->p2_nv <= 0) return set_error(dm, 1, "assembled operator copy without a P2 hierarchy");
         ptr.swap(g->h_ptr);
         col.swap(g->h_col);
-        val.swap(g->h_val);
+        valf.swap(g->h_val);
         fixed.swap(g->h_fixed);
     } else {
         ptr.resize(nn + 1);
@@ -1231,7 +1233,7 @@ static int gf_amg_setup(plx_gf g) {
         if (lerr) return lerr;
     }
     const int aerr = g->p2_nv > 0
-                         ? amg_setup_p2(nn, ptr.data(), col.data(), val.data(), fixed.data(), g->p2_nv,
+                         ? amg_setup_p2(nn, ptr.data(), col.data(), valf.data(), fixed.data(), g->p2_nv,
                                         g->p2_par_ptr.data(), g->p2_par.data(), g->p2_parw.data(),
                                         g->p2_vxyz.data(), 768, 8, &h, &msg, g->amg_theta, lmax0)
                          : amg_setup(nn, ptr.data(), col.data(), val.data(), g->xyz.data(), fixed.data(), 768, 8,
@@ -1522,8 +1524,9 @@ static int gf_create_assembled(plx_gf *out, plx_dm dm, const std::vector<KeyPlan
     (void)hipFree(val0_buf.p);
     val0_buf.p = nullptr;
     if ((err = gf_setup_assembled(g, static_cast<int32_t *>(diag_buf.p)))) return fail(err);
+    // the multigrid levels are FP32 on the device: their set-up reads the FP32 copy (half the bytes)
     g->h_val.resize((size_t)nnzb * 9);
-    if (hipMemcpy(g->h_val.data(), g->d_val64, g->h_val.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(g->h_val.data(), g->d_val32, g->h_val.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
         return fail(set_error(dm, 76, "downloading the assembled operator failed"));
     g->h_fixed.resize(3 * (size_t)nn);
     for (int r = 0; r < nn; ++r)
