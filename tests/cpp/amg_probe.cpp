@@ -106,6 +106,19 @@ int amg_probe_build_p2(int n, const int32_t *ptr, const int32_t *col, const doub
     return (int)H.levels.size();
 }
 
+// The same from FP32 operator values (the set-up's device path reads the operator's FP32 copy).
+int amg_probe_build_p2f(int n, const int32_t *ptr, const int32_t *col, const float *val, const uint8_t *fixed,
+                        int nv, const int32_t *par_ptr, const int32_t *par, const double *parw, const double *vxyz,
+                        int coarse_max, int max_levels) {
+    A0p.assign(ptr, ptr + n + 1);
+    A0c.assign(col, col + ptr[n]);
+    A0v.assign(val, val + (size_t)ptr[n] * 9);
+    if (plx::amg_setup_p2(n, ptr, col, val, fixed, nv, par_ptr, par, parw, vxyz, coarse_max, max_levels, &H,
+                          &last_error, theta))
+        return -1;
+    return (int)H.levels.size();
+}
+
 const char *amg_probe_error() { return last_error.c_str(); }
 
 // info: n, b, nc; lmax

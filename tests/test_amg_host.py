@@ -49,6 +49,7 @@ def probe():
     lib.amg_probe_build_p2.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_int, P, P, P, P, ctypes.c_int,
                                        ctypes.c_int]
     lib.amg_probe_level_bc.argtypes = [ctypes.c_int]
+    lib.amg_probe_build_p2f.argtypes = lib.amg_probe_build_p2.argtypes
     return lib
 
 
@@ -438,3 +439,73 @@ def test_p2_to_p1_first_level(probe, oracle):
     xj, itj = _pcg(A, lambda r: np.einsum("nij,nj->ni", Dinv, r.reshape(-1, 3)).ravel(), b)
     assert np.linalg.norm(A @ xa - b) <= 1e-8 * np.linalg.norm(b)
     assert ita * 10 <= itj, (ita, itj)
+
+
+def test_p2_hierarchy_from_fp32_values(probe, oracle):
+    """The device path's set-up reads the operator's FP32 copy: the hierarchy built from FP32 values
+    equals, bit for bit, the one built from the same values widened to FP64 (every level's
+    operator, transfers, inverse diagonal blocks and spectrum bound, and the coarsest inverse)."""
+    m, A, fixed = _p2_operator(oracle, 4)
+    nn, nv = m.num_nodes, m.num_vertices
+    edges = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    par = [None] * nn
+    for c in range(m.num_cells):
+        cl, cn = m.closure[c], m.cone[c]
+        for b in range(4):
+            par[cl[b]] = ([int(cn[b])], [1.0])
+        for e, (a, b) in enumerate(edges):
+            par[cl[4 + e]] = (sorted([int(cn[a]), int(cn[b])]), [0.5, 0.5])
+    par_ptr = np.zeros(nn + 1, np.int32)
+    for i in range(nn):
+        par_ptr[i + 1] = par_ptr[i] + len(par[i][0])
+    pidx = np.concatenate([np.array(p[0], np.int32) for p in par])
+    pw = np.concatenate([np.array(p[1], float) for p in par])
+    Ab = sp.bsr_matrix(A, blocksize=(3, 3))
+    Ab.sort_indices()
+    ptr, col = Ab.indptr.astype(np.int32), Ab.indices.astype(np.int32)
+    val32 = np.ascontiguousarray(Ab.data, np.float32)
+    val64 = val32.astype(np.float64)
+    fx = fixed.astype(np.uint8)
+    vxyz = np.ascontiguousarray(m.coords, np.float64)
+
+    def snapshot(nl):
+        out = []
+        for l in range(nl):
+            info, lmax = np.zeros(3, np.int32), ctypes.c_double()
+            probe.amg_probe_level(l, info.ctypes.data, ctypes.byref(lmax))
+            out.append(("info", l, info.tolist(), lmax.value))
+            for which in range(10):
+                cnt = probe.amg_probe_array(l, which, None)
+                dt = np.float64 if which in (2, 5, 8, 9) else np.int32
+                a = np.zeros(max(cnt, 0), dt)
+                if cnt > 0:
+                    probe.amg_probe_array(l, which, a.ctypes.data)
+                out.append((which, l, a))
+        cnt = probe.amg_probe_coarse(None)
+        cinv = np.zeros(cnt, np.float64)
+        probe.amg_probe_coarse(cinv.ctypes.data)
+        out.append(("coarse", cinv))
+        return out
+
+    probe.amg_probe_set_theta(0.05)
+    try:
+        args = (par_ptr.ctypes.data, pidx.ctypes.data, pw.ctypes.data, vxyz.ctypes.data, 300, 8)
+        nl64 = probe.amg_probe_build_p2(nn, ptr.ctypes.data, col.ctypes.data, val64.ctypes.data, fx.ctypes.data,
+                                        nv, *args)
+        assert nl64 >= 2, probe.amg_probe_error()
+        ref = snapshot(nl64)
+        nl32 = probe.amg_probe_build_p2f(nn, ptr.ctypes.data, col.ctypes.data, val32.ctypes.data, fx.ctypes.data,
+                                         nv, *args)
+        assert nl32 == nl64, probe.amg_probe_error()
+        got = snapshot(nl32)
+    finally:
+        probe.amg_probe_set_theta(0.0)
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        if g[0] == "info":
+            assert g == r
+        elif g[0] == "coarse":
+            assert np.array_equal(g[1], r[1])
+        else:
+            assert g[0] == r[0] and g[1] == r[1] and np.array_equal(g[2], r[2]), (g[0], g[1])
+
