@@ -320,7 +320,12 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
         const int next_slot = slot + G;
         const bool has_next = next_slot < A.num_list;
         Stage<Real, CT> nxt;
+        // the next tile's loads are issued at a raised wave priority, so the SIMD issues them
+        // ahead of the arithmetic of its other waves (P2 FP64 kernel -1.3 %; the write-out below
+        // at priority 1: Q1 -1.4 %)
+        __builtin_amdgcn_s_setprio(2);
         if (has_next) fetch<Real, CT, ACC>(nxt, A, tile_of(next_slot), tid);
+        __builtin_amdgcn_s_setprio(0);
 
 #pragma unroll
         for (int i = 0; i < S::CPT; ++i) {
@@ -394,6 +399,7 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
             }
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(1);
 
         if constexpr (EARLY) {
 #pragma unroll
@@ -424,6 +430,7 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
 #pragma unroll
             for (int c = 0; c < 3; ++c) halo[3 * s + c] = (Real)sR[3 * k + c];
         }
+        __builtin_amdgcn_s_setprio(0);
         if (!has_next) break;
         __syncthreads();  // every read of this tile's LDS precedes the next tile's staging
         cur = nxt;
