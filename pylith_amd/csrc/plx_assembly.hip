@@ -152,11 +152,14 @@ __device__ __forceinline__ int rec_get(const uint4 (&r)[W], int k) {
 
 // One tile's data on its way from HBM: gathered node values and coordinates, the cells'
 // local closures, moduli and colours (registers; the next tile's loads overlap the current
-// tile's arithmetic).
+// tile's arithmetic).  The gathers go out in two phases: the node / vertex ids (with the cells'
+// records and moduli) before the current tile's first cell, the values at those ids after it,
+// so the ids' latency hides behind that cell's arithmetic.
 template <typename Real, int CT>
 struct Stage {
     using S = Shape<CT, Real>;
     int tile, c0, nc, n0, nn, v0, nvt, nown, hoff;
+    int gn[CT == PLX_HEX_Q1 ? S::PFN : 1];  // Q1: node ids between the two phases (-1: past the tile)
     Real pu[S::PFN][3];
     double px[CT == PLX_HEX_Q1 ? S::PFN : S::PFV][3];
     uint4 rec[S::CPT][S::RECW];  // packed 16-bit local closure (and cone) of each cell
@@ -164,6 +167,63 @@ struct Stage {
     int color[S::CPT];
 };
 
+// Q1, first phase: extents, node ids, the cells' records and moduli.
+template <typename Real, int CT, int ACC>
+__device__ __forceinline__ void fetch_ids(Stage<Real, CT> &T, const ApplyArgs &A, int tile, int tid) {
+    using S = Shape<CT, Real>;
+    static_assert(CT == PLX_HEX_Q1, "two-phase fetch: Q1");
+    const TilePlanDev &P = A.p;
+    T.tile = tile;
+    T.c0 = P.tile_cell_ptr[tile];
+    T.nc = P.tile_cell_ptr[tile + 1] - T.c0;
+    T.n0 = P.tile_node_ptr[tile];
+    T.nn = P.tile_node_ptr[tile + 1] - T.n0;
+    T.v0 = P.tile_vert_ptr[tile];
+    T.nvt = P.tile_vert_ptr[tile + 1] - T.v0;
+    T.nown = P.tile_nown[tile];
+    T.hoff = P.tile_halo_off[tile];
+#pragma unroll
+    for (int j = 0; j < S::PFN; ++j) {
+        const int k = tid + j * S::NT;
+        T.gn[j] = k < T.nn ? P.tile_nodes[T.n0 + k] : -1;
+    }
+    const Real *mat = static_cast<const Real *>(A.mat);
+#pragma unroll
+    for (int i = 0; i < S::CPT; ++i) {
+        const int cell = tid + i * S::NT;
+        if (cell < T.nc) {
+            const int pos = T.c0 + cell;
+            const uint4 *recs = reinterpret_cast<const uint4 *>(A.records);
+#pragma unroll
+            for (int w = 0; w < S::RECW; ++w) T.rec[i][w] = recs[(size_t)pos * S::RECW + w];
+            // unscaled here (scaled where used): the fetch then has no use of these loads to wait for
+            T.lam[i] = mat[pos];
+            T.mu[i] = mat[P.stride + pos];
+            if (ACC == kAccColored) T.color[i] = A.color[pos];
+        }
+    }
+}
+
+// Q1, second phase: the values at the ids of fetch_ids.
+template <typename Real, int CT>
+__device__ __forceinline__ void fetch_values(Stage<Real, CT> &T, const ApplyArgs &A) {
+    using S = Shape<CT, Real>;
+    static_assert(CT == PLX_HEX_Q1, "two-phase fetch: Q1");
+    const TilePlanDev &P = A.p;
+    const Real *u = static_cast<const Real *>(A.u);
+#pragma unroll
+    for (int j = 0; j < S::PFN; ++j) {
+        if (T.gn[j] >= 0) {
+            const size_t g = (size_t)T.gn[j];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) T.pu[j][c] = u[3 * g + c];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
+        }
+    }
+}
+
+// One phase (the first tile of every kernel, the P2 kernels throughout).
 template <typename Real, int CT, int ACC>
 __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, int tile, int tid) {
     using S = Shape<CT, Real>;
@@ -323,8 +383,14 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
         // the next tile's loads are issued at a raised wave priority, so the SIMD issues them
         // ahead of the arithmetic of its other waves (P2 FP64 kernel -1.3 %; the write-out below
         // at priority 1: Q1 -1.4 %)
+        // Q1 factored path: the values go out after the first cell (see Stage); the P2 kernels
+        // and the quadrature-point paths keep one phase (split, their registers overflow)
+        constexpr bool SPLIT = CT == PLX_HEX_Q1 && PATH == kPathFactored && S::CPT > 1;
         __builtin_amdgcn_s_setprio(2);
-        if (has_next) fetch<Real, CT, ACC>(nxt, A, tile_of(next_slot), tid);
+        if (has_next) {
+            if constexpr (SPLIT) fetch_ids<Real, CT, ACC>(nxt, A, tile_of(next_slot), tid);
+            else fetch<Real, CT, ACC>(nxt, A, tile_of(next_slot), tid);
+        }
         __builtin_amdgcn_s_setprio(0);
 
 #pragma unroll
@@ -395,6 +461,13 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
 #pragma unroll
                             for (int c = 0; c < 3; ++c) sR[3 * lc[b] + c] += re[c][b];
                     }
+                }
+            }
+            if constexpr (SPLIT) {
+                if (i == 0 && has_next) {
+                    __builtin_amdgcn_s_setprio(2);
+                    fetch_values<Real, CT>(nxt, A);
+                    __builtin_amdgcn_s_setprio(0);
                 }
             }
         }
