@@ -159,7 +159,8 @@ template <typename Real, int CT>
 struct Stage {
     using S = Shape<CT, Real>;
     int tile, c0, nc, n0, nn, v0, nvt, nown, hoff;
-    int gn[CT == PLX_HEX_Q1 ? S::PFN : 1];  // Q1: node ids between the two phases (-1: past the tile)
+    int gn[S::PFN];                          // node ids between the two phases (-1: past the tile)
+    int gv[CT == PLX_HEX_Q1 ? 1 : S::PFV];   // P2 vertex ids
     Real pu[S::PFN][3];
     double px[CT == PLX_HEX_Q1 ? S::PFN : S::PFV][3];
     uint4 rec[S::CPT][S::RECW];  // packed 16-bit local closure (and cone) of each cell
@@ -167,11 +168,11 @@ struct Stage {
     int color[S::CPT];
 };
 
-// Q1, first phase: extents, node ids, the cells' records and moduli.
+// First phase of the two-phase fetch: extents, node (and P2 vertex) ids, the cells' records and
+// moduli.
 template <typename Real, int CT, int ACC>
 __device__ __forceinline__ void fetch_ids(Stage<Real, CT> &T, const ApplyArgs &A, int tile, int tid) {
     using S = Shape<CT, Real>;
-    static_assert(CT == PLX_HEX_Q1, "two-phase fetch: Q1");
     const TilePlanDev &P = A.p;
     T.tile = tile;
     T.c0 = P.tile_cell_ptr[tile];
@@ -186,6 +187,13 @@ __device__ __forceinline__ void fetch_ids(Stage<Real, CT> &T, const ApplyArgs &A
     for (int j = 0; j < S::PFN; ++j) {
         const int k = tid + j * S::NT;
         T.gn[j] = k < T.nn ? P.tile_nodes[T.n0 + k] : -1;
+    }
+    if constexpr (CT != PLX_HEX_Q1) {
+#pragma unroll
+        for (int j = 0; j < S::PFV; ++j) {
+            const int k = tid + j * S::NT;
+            T.gv[j] = k < T.nvt ? P.tile_verts[T.v0 + k] : -1;
+        }
     }
     const Real *mat = static_cast<const Real *>(A.mat);
 #pragma unroll
@@ -204,11 +212,10 @@ __device__ __forceinline__ void fetch_ids(Stage<Real, CT> &T, const ApplyArgs &A
     }
 }
 
-// Q1, second phase: the values at the ids of fetch_ids.
+// Second phase: the values and coordinates at the ids of fetch_ids.
 template <typename Real, int CT>
 __device__ __forceinline__ void fetch_values(Stage<Real, CT> &T, const ApplyArgs &A) {
     using S = Shape<CT, Real>;
-    static_assert(CT == PLX_HEX_Q1, "two-phase fetch: Q1");
     const TilePlanDev &P = A.p;
     const Real *u = static_cast<const Real *>(A.u);
 #pragma unroll
@@ -217,13 +224,25 @@ __device__ __forceinline__ void fetch_values(Stage<Real, CT> &T, const ApplyArgs
             const size_t g = (size_t)T.gn[j];
 #pragma unroll
             for (int c = 0; c < 3; ++c) T.pu[j][c] = u[3 * g + c];
+            if constexpr (CT == PLX_HEX_Q1) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
+                for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
+            }
+        }
+    }
+    if constexpr (CT != PLX_HEX_Q1) {
+#pragma unroll
+        for (int j = 0; j < S::PFV; ++j) {
+            if (T.gv[j] >= 0) {
+                const size_t g = (size_t)T.gv[j];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) T.px[j][d] = P.coords[3 * g + d];
+            }
         }
     }
 }
 
-// One phase (the first tile of every kernel, the P2 kernels throughout).
+// One phase (the first tile of every kernel; the kernels that keep one phase throughout).
 template <typename Real, int CT, int ACC>
 __device__ __forceinline__ void fetch(Stage<Real, CT> &T, const ApplyArgs &A, int tile, int tid) {
     using S = Shape<CT, Real>;
@@ -364,10 +383,16 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
         }
         __syncthreads();
 
+        // The next tile's loads go out in two phases (see Stage) in the factored Q1 kernels and
+        // the factored FP64 P2 kernel with LDS atomics (measured -3 % each); the other kernels
+        // overflow their registers with it and keep one phase.  The P2 kernel makes room by
+        // loading its write-out ids after the cells (below) instead of early.
+        constexpr bool P2SPLIT = CT != PLX_HEX_Q1 && sizeof(Real) == 8 && PATH == kPathFactored && ACC == kAccAtomic;
+        constexpr bool SPLIT = (CT == PLX_HEX_Q1 && PATH == kPathFactored && S::CPT > 1) || P2SPLIT;
         // node ids of this tile's owned write-out, loaded before the next tile's gathers so that
         // waiting for them does not wait for those (not for the FP64 Q1 kernel: its registers
-        // are full, and the extra ids spill)
-        constexpr bool EARLY = CT != PLX_HEX_Q1 || sizeof(Real) == 4;
+        // are full, and the extra ids spill; not for the two-phase P2 kernel)
+        constexpr bool EARLY = (CT != PLX_HEX_Q1 || sizeof(Real) == 4) && !P2SPLIT;
         constexpr int NW = EARLY ? S::PFN : 1;
         int wn[NW];
         if constexpr (EARLY) {
@@ -383,9 +408,6 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
         // the next tile's loads are issued at a raised wave priority, so the SIMD issues them
         // ahead of the arithmetic of its other waves (P2 FP64 kernel -1.3 %; the write-out below
         // at priority 1: Q1 -1.4 %)
-        // Q1 factored path: the values go out after the first cell (see Stage); the P2 kernels
-        // and the quadrature-point paths keep one phase (split, their registers overflow)
-        constexpr bool SPLIT = CT == PLX_HEX_Q1 && PATH == kPathFactored && S::CPT > 1;
         __builtin_amdgcn_s_setprio(2);
         if (has_next) {
             if constexpr (SPLIT) fetch_ids<Real, CT, ACC>(nxt, A, tile_of(next_slot), tid);
@@ -480,6 +502,23 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
                 const int k = tid + j * NT;
                 if (k < cur.nown) {
                     const size_t g = (size_t)wn[j];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const Real v = (Real)sR[3 * k + c];
+                        r[3 * g + c] = insert ? v : r[3 * g + c] + v;
+                    }
+                }
+            }
+        } else if constexpr (P2SPLIT) {
+            // the write-out ids, all slots at once (clamped positions, no branch)
+            int wl[S::PFN];
+#pragma unroll
+            for (int j = 0; j < S::PFN; ++j) wl[j] = A.p.tile_nodes[cur.n0 + min(tid + j * NT, max(cur.nown - 1, 0))];
+#pragma unroll
+            for (int j = 0; j < S::PFN; ++j) {
+                const int k = tid + j * NT;
+                if (k < cur.nown) {
+                    const size_t g = (size_t)wl[j];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const Real v = (Real)sR[3 * k + c];
