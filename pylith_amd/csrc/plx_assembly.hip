@@ -384,11 +384,11 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
         __syncthreads();
 
         // The next tile's loads go out in two phases (see Stage) in the factored Q1 kernels and
-        // the FP64 P2 kernels with LDS atomics (measured -3 % each for the factored kernels,
-        // -1.7 % for the P2 quadrature-point kernel); the other kernels overflow their registers
-        // with it and keep one phase.  The P2 kernel makes room by
+        // the FP64 P2 kernels (measured -3 % each for the factored kernels with LDS atomics or
+        // colour phases, -1.7 % for the P2 quadrature-point kernel); the other kernels overflow
+        // their registers with it and keep one phase.  The P2 kernel makes room by
         // loading its write-out ids after the cells (below) instead of early.
-        constexpr bool P2SPLIT = CT != PLX_HEX_Q1 && sizeof(Real) == 8 && (PATH == kPathFactored || PATH == kPathQpAffine) && ACC == kAccAtomic;
+        constexpr bool P2SPLIT = CT != PLX_HEX_Q1 && sizeof(Real) == 8 && (PATH == kPathFactored || PATH == kPathQpAffine);
         constexpr bool SPLIT = (CT == PLX_HEX_Q1 && PATH == kPathFactored && S::CPT > 1) || P2SPLIT;
         // node ids of this tile's owned write-out, loaded before the next tile's gathers so that
         // waiting for them does not wait for those (not for the FP64 Q1 kernel: its registers
