@@ -129,8 +129,9 @@ struct Shape<PLX_HEX_Q1, Real> {
 };
 // P2 FP64: 4-wave workgroups over the 384-cell tile (cells 0-255 on the first pass, 256-383 on
 // the second), two workgroups per CU at the kernel's register count (2 waves per SIMD: every
-// wave slot of the CU busy; measured 6 % faster than 3-wave workgroups with two full passes);
-// FP32 (4 waves per SIMD) keeps 6-wave workgroups.
+// wave slot of the CU busy; measured 6 % faster than 3-wave workgroups with two full passes).
+// FP32 takes the same shape (three workgroups per CU at its 142 VGPRs): two cells a thread give
+// the two-phase fetch its split point (3 % faster than 6-wave workgroups of one cell a thread).
 // record: 10 x uint16 closure, 4 x uint16 cone, 2 padding = 2 uint4 words
 template <>
 struct Shape<PLX_TET_P2, double> {
@@ -138,7 +139,7 @@ struct Shape<PLX_TET_P2, double> {
 };
 template <>
 struct Shape<PLX_TET_P2, float> {
-    static constexpr int NT = 384, CPT = 1, NB = 10, NV = 4, PFN = 2, PFV = 1, RECW = 2;
+    static constexpr int NT = 256, CPT = 2, NB = 10, NV = 4, PFN = 3, PFV = 1, RECW = 2;
 };
 
 // Entry k of a packed 16-bit cell record.
@@ -384,11 +385,11 @@ __global__ void __launch_bounds__((Shape<CT, Real>::NT), MINW) apply_tiles(Apply
         __syncthreads();
 
         // The next tile's loads go out in two phases (see Stage) in the factored Q1 kernels and
-        // the FP64 P2 kernels (measured -3 % each for the factored kernels with LDS atomics or
-        // colour phases, -1.7 % for the P2 quadrature-point kernel); the other kernels overflow
-        // their registers with it and keep one phase.  The P2 kernel makes room by
+        // the P2 kernels (measured -3 % each for the factored kernels with LDS atomics or colour
+        // phases, FP64 and FP32, -1.7 % for the P2 quadrature-point kernel); the Q1
+        // quadrature-point kernels overflow their registers with it and keep one phase.  The P2 kernel makes room by
         // loading its write-out ids after the cells (below) instead of early.
-        constexpr bool P2SPLIT = CT != PLX_HEX_Q1 && sizeof(Real) == 8 && (PATH == kPathFactored || PATH == kPathQpAffine);
+        constexpr bool P2SPLIT = CT != PLX_HEX_Q1 && (PATH == kPathFactored || PATH == kPathQpAffine);
         constexpr bool SPLIT = (CT == PLX_HEX_Q1 && PATH == kPathFactored && S::CPT > 1) || P2SPLIT;
         // node ids of this tile's owned write-out, loaded before the next tile's gathers so that
         // waiting for them does not wait for those (not for the FP64 Q1 kernel: its registers
